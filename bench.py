@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark of the det(I - M(omega)) hot path on one MI355X.
+
+Workload (the drop-in FP64 GPU configuration): Toomre-Zang n=4, q1=7, m=2 on
+the default grid NR=501, Ne=51, N_alpha=301, alpha_max=30, l in [-100, 100]
+(K = 201*501*51 = 5 135 751), nw/nwa = 100001/2001.  A step is one
+det(I - M(omega)) evaluation: resonance weights, the K-long complex
+contraction (FP64 MFMA), the G*N epilogue and the FP64 complex LU, with W
+resident in HBM.  Also reported: Newton roots/s (FP64, from
+omega0 = 0.8794 + 0.1231i), the complex64/FP32-MFMA det rate, the W
+quadrature time, and the CPU baseline (numpy/OpenBLAS BLAS-form det on a
+bounded sample of l-slabs, extrapolated to the full l range).
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("det(I−M(ω)) evals/s + Newton roots/s, mixed vs FP64 GPU at "
+          "|δω|≤1e-10")
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="default", help="configs/<name>.toml")
+    ap.add_argument("--cpu-slabs", type=int, default=4,
+                    help="l-slabs timed for the CPU baseline (0 = skip)")
+    ap.add_argument("--no-newton", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the omega-sharded multi-GPU form is described in DESIGN.md but not built",
+              file=sys.stderr)
+        return 2
+
+    import kalnajs_log_spirals_amd as K
+    from kalnajs_log_spirals_amd import _ffi
+
+    cfg_path = os.path.join(ROOT, "kalnajs_log_spirals_amd", "configs", f"{args.config}.toml")
+    model_path = os.path.join(ROOT, "kalnajs_log_spirals_amd", "models", "kalnajs_zang4.toml")
+    cfg = K.load_model(K.load_config(cfg_path), model_path)
+    g = cfg.grid
+    n_l = g.l_max - g.l_min + 1
+    NPh = g.NR * g.Ne
+    Kdim = n_l * NPh
+    N = g.N_alpha
+    log(f"[bench] library {_ffi.version()}, config {args.config}: NR={g.NR} Ne={g.Ne} "
+        f"N={N} l=[{g.l_min},{g.l_max}] K={Kdim}")
+
+    model = K.create_toomre_zang_model(cfg)
+    t0 = time.perf_counter()
+    grids = K.compute_grids(cfg, model)
+    od = K.compute_orbits(cfg, model, grids)
+    t_orb = time.perf_counter() - t0
+    log(f"[bench] host orbits: {t_orb:.2f} s ({_ffi.host_threads()} threads)")
+    t0 = time.perf_counter()
+    pre = K.precompute_all(cfg, model, K.GPUType.AMD)
+    t_pre = time.perf_counter() - t0
+    W_ms = pre.ctx.last_W_ms()
+    log(f"[bench] precompute_all: {t_pre:.2f} s (W quadrature on device {W_ms:.1f} ms)")
+    dev = K.to_device(pre, K.GPUType.AMD, np.float64)
+    ctx = dev.ctx
+    ctx.set_profiling(True)
+
+    om0 = complex(2 * 0.4397, 0.1231)
+    oms = [om0 + 0.001 * i * (1 + 1j) for i in range(args.warmup + args.steps)]
+    for i in range(args.warmup):
+        ctx.det([oms[i]], _ffi.PREC_FP64)
+    log(f"[bench] warmup done ({args.warmup} det)")
+    contract_ms = lu_ms = weights_ms = reduce_ms = 0.0
+    launches = 0
+    dets = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        dets.append(ctx.det([oms[args.warmup + i]], _ffi.PREC_FP64)[0])
+        t = ctx.last_timing()
+        contract_ms += t["contract_ms"]
+        lu_ms += t["lu_ms"]
+        weights_ms += t["weights_ms"]
+        reduce_ms += t["reduce_ms"]
+        launches += t["contract_launches"]
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    det_per_s = args.steps / elapsed
+    avg_contract = contract_ms / max(launches, 1)
+    flop_build = 8.0 * Kdim * N * N
+    achieved_tflops = flop_build / (avg_contract * 1e-3) / 1e12
+    log(f"[bench] FP64 det: {ms_per_step:.2f} ms/det ({det_per_s:.3f} det/s); contraction "
+        f"{avg_contract:.2f} ms = {achieved_tflops:.1f} TFLOP/s; LU {lu_ms / args.steps:.2f} ms; "
+        f"weights {weights_ms / args.steps:.3f} ms; reduce {reduce_ms / args.steps:.3f} ms")
+
+    extra = {"det_per_s_fp64": det_per_s, "contract_ms_fp64": avg_contract,
+             "lu_ms": lu_ms / args.steps, "weights_ms": weights_ms / args.steps,
+             "reduce_ms": reduce_ms / args.steps, "W_quadrature_ms": W_ms,
+             "host_orbits_s": t_orb, "precompute_s": t_pre,
+             "W_quadrature_tflops": 4.0 * n_l * g.nwa * N * NPh / (W_ms * 1e-3) / 1e12}
+
+    if not args.no_fp32:
+        ctx.prepare(_ffi.PREC_FP32)
+        ctx.det([om0], _ffi.PREC_FP32)
+        n32 = max(3, args.steps // 2)
+        c32 = 0.0
+        t0 = time.perf_counter()
+        d32 = []
+        for i in range(n32):
+            d32.append(ctx.det([oms[args.warmup + i]], _ffi.PREC_FP32)[0])
+            c32 += ctx.last_timing()["contract_ms"]
+        e32 = time.perf_counter() - t0
+        rel = max(abs(d32[i] - dets[i]) / abs(dets[i]) for i in range(min(n32, len(dets))))
+        extra.update({"det_per_s_fp32": n32 / e32, "contract_ms_fp32": c32 / n32,
+                      "fp32_contract_tflops": flop_build / (c32 / n32 * 1e-3) / 1e12,
+                      "fp32_det_rel_err_vs_fp64": rel})
+        log(f"[bench] FP32 det: {1e3 * e32 / n32:.2f} ms/det, contraction {c32 / n32:.2f} ms, "
+            f"det rel err vs FP64 {rel:.2e}")
+
+    if not args.no_newton:
+        t0 = time.perf_counter()
+        r = K.newton_search(cfg, pre, dev, K.GPUType.AMD, np.float64, 0.4397, 0.1231)
+        tn = time.perf_counter() - t0
+        extra.update({"newton_roots_per_s_fp64": 1.0 / tn, "newton_s_fp64": tn,
+                      "newton_iterations": r.iterations, "newton_det_evals": r.n_det_evals,
+                      "newton_converged": r.converged, "root_Omega_p": r.Omega_p,
+                      "root_gamma": r.gamma, "root_abs_det": abs(r.det_val)})
+        log(f"[bench] FP64 Newton: {tn:.2f} s, {r.iterations} iterations, {r.n_det_evals} dets, "
+            f"root Omega_p={r.Omega_p:.12f} gamma={r.gamma:.12f} |det|={abs(r.det_val):.2e}")
+
+    cpu = None
+    if args.cpu_slabs > 0:
+        from oracle import blas_form
+        try:
+            from threadpoolctl import threadpool_info
+            cores = max((i.get("num_threads", 1) for i in threadpool_info()
+                         if i.get("internal_api") == "openblas"), default=os.cpu_count())
+        except Exception:  # pragma: no cover
+            cores = os.cpu_count()
+        t0 = time.perf_counter()
+        s_det, s_slab, s_lu = blas_form.time_sample(NPh, N, n_l, args.cpu_slabs, repeats=2)
+        log(f"[bench] CPU BLAS-form: {s_slab * 1e3:.1f} ms per l-slab, LU {s_lu * 1e3:.2f} ms -> "
+            f"{s_det:.2f} s per det ({time.perf_counter() - t0:.1f} s of CPU work)")
+        cpu = {"value": 1.0 / s_det, "unit": "det/s", "cores": int(cores), "kind": "port",
+               "sample": f"numpy/OpenBLAS per-l ZGEMM W_l^T (c*conj W_l) on {args.cpu_slabs} of "
+                         f"{n_l} synthetic [{NPh}x{N}] l-slabs + one LU, extrapolated to {n_l} slabs"}
+
+    out = {
+        "metric": METRIC, "value": det_per_s, "unit": "det/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "deterministic Toomre-Zang model (no external data)",
+        "config": {"workload": f"FP64 det(I-M) at the {args.config} grid, W resident in HBM",
+                   "NR": g.NR, "Ne": g.Ne, "N_alpha": N, "l_min": g.l_min, "l_max": g.l_max,
+                   "K": Kdim, "nwa": g.nwa, "m": cfg.physics.m, "n_zang": cfg.model.n_zang,
+                   "q1": cfg.model.q1},
+        "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS,
+                     "traffic": None, "kernel": "k_contract_f64<1>",
+                     "flop_per_launch": flop_build},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,256 @@
+/*
+ * klsp.h -- C ABI of the MI355X-native Kalnajs log-spiral hot path.
+ *
+ * One shared library (libklsp.so) holds hand-written CDNA4 (gfx950) HIP
+ * kernels for the three stages of the reference's det(I - M(omega)) path and
+ * the host drivers above them.  Plain pointers and sizes only; no C++ or torch
+ * types cross this boundary.  Every function returns an int status (0 = OK);
+ * klsp_last_error() gives the message of the last failure on a context.
+ *
+ * Layouts are the reference's Julia column-major layouts, complex numbers are
+ * interleaved (re, im) doubles:
+ *   orbit arrays w1/ra/pha  [nwa, NR, Ne]   sample fastest, then iR, then ie
+ *   per-orbit arrays        [NR, Ne]        k = iR + NR*ie  ("NPh" = NR*Ne)
+ *   W_l_mat                 [NPh, N, n_l]   complex
+ *   F0l_all, Omega_res      [NPh, n_l]
+ *   M                       [N, N]          complex
+ *   det_grid                [n_gamma, n_Op] complex
+ *   Newton history          [iterations, 3] (Omega_p, gamma, |det|)
+ *
+ * Ownership: the caller owns every host buffer; the library never keeps a
+ * host pointer after a call returns.  A context owns its device memory until
+ * klsp_destroy().  Calls on one context are serialised on one HIP stream and
+ * are blocking; contexts are independent.  The library has no CPU fallback:
+ * every compute entry point runs on the GPU or fails with KLSP_ERR_NO_DEVICE.
+ */
+#ifndef KLSP_H
+#define KLSP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KLSP_ABI_VERSION 1
+
+/* Status codes. */
+enum klsp_status {
+    KLSP_OK = 0,
+    KLSP_ERR_INVALID_ARG = 1,  /* bad size, null pointer, inconsistent dims */
+    KLSP_ERR_OOM = 2,          /* device or host allocation failed          */
+    KLSP_ERR_HIP = 3,          /* a HIP runtime call failed                 */
+    KLSP_ERR_NOT_READY = 4,    /* e.g. det before the problem/W is on device */
+    KLSP_ERR_NO_DEVICE = 5     /* no gfx950 device visible                  */
+};
+
+/* Arithmetic of an M(omega) build.  The determinant (LU) is FP64 always. */
+enum klsp_precision {
+    KLSP_PREC_FP64 = 0,  /* W complex128, FP64 MFMA contraction            */
+    KLSP_PREC_FP32 = 1   /* W complex64, FP32 MFMA with FP64 K-tile sums   */
+};
+
+typedef struct klsp_ctx klsp_ctx;
+
+/* ------------------------------------------------------------------ */
+/* Library and context                                                 */
+/* ------------------------------------------------------------------ */
+
+/* Version string of the library ("klsp <semver> gfx950"). */
+const char* klsp_version(void);
+
+/* Number of visible HIP devices (0 if none); never fails on a CPU host. */
+int klsp_device_count(int* n_out);
+
+/* Create a context on HIP device `device`.  Replaces the device side of
+ * GPUBackend.gpu_device! + to_device (GPUBackend.jl:147, BasisFunctions.jl:397). */
+int klsp_create(int device, klsp_ctx** ctx_out);
+
+/* Free every device buffer of the context (the Julia finalizer calls this). */
+int klsp_destroy(klsp_ctx* ctx);
+
+/* Message of the last failed call on `ctx` (static string if ctx is NULL). */
+const char* klsp_last_error(const klsp_ctx* ctx);
+
+/* ------------------------------------------------------------------ */
+/* Host precompute (CPU, FP64; no device needed)                       */
+/* ------------------------------------------------------------------ */
+
+/* Model and grid parameters: Configuration.jl:31-67 (PhysicsConfig,
+ * ModelConfig, GridConfig); same keys as the TOML config files. */
+typedef struct klsp_params {
+    int m;            /* azimuthal mode number                          */
+    double G;         /* gravitational constant                         */
+    double L0;        /* taper angular-momentum scale                   */
+    int n_zang;       /* Zang taper exponent                            */
+    int q1;           /* velocity-dispersion index (q = q1 - 1)         */
+    int NR, Ne;       /* guiding-centre and circulation grid sizes      */
+    double Rc_min, Rc_max, v_min;
+    int N_alpha;      /* log-spiral wavenumbers alpha in [-amax, amax]   */
+    double alpha_max;
+    int l_min, l_max; /* radial harmonics                               */
+    int nw, nwa;      /* orbit integration points / stored samples      */
+} klsp_params;
+
+/* trapezoid_coef, BasisFunctions.jl:174 (= OrbitIntegration.jl:63). */
+int klsp_trapezoid_coef(const double* x, int64_t n, double* S_out);
+
+/* simpson_coef, OrbitIntegration.jl:79 (falls back to trapezoid if n even). */
+int klsp_simpson_coef(const double* x, int64_t n, double* S_out);
+
+/* Evenly spaced grid, Julia range(a, b, length=n). */
+int klsp_linrange(double a, double b, int64_t n, double* out);
+
+/* gamma_complex, BasisFunctions.jl:309 (Godfrey-Lanczos). out = {re, im}. */
+int klsp_gamma_complex(double re, double im, double* out);
+
+/* compute_N_kernel, BasisFunctions.jl:281: N[a] = Re ratio * pi * d_alpha[a]. */
+int klsp_compute_N_kernel(int m, const double* alpha, const double* d_alpha,
+                          int64_t N, double* N_out);
+
+/* compute_grids, OrbitIntegration.jl:102.  Per-orbit outputs are [NR*Ne]
+ * (k = iR + NR*ie); S_RC is [NR]. */
+int klsp_compute_grids(const klsp_params* p, double* Rc, double* e, double* v,
+                       double* R1, double* R2, double* E, double* L, double* L2,
+                       double* SGNL, double* S_RC, double* S_e);
+
+/* compute_orbits + compute_jacobian! + DF fill, OrbitIntegration.jl:181,
+ * :341, :317.  Inputs are the compute_grids outputs; w1/ra/pha are
+ * [nwa, NR, Ne].  Runs on `n_threads` host threads (<= 0: all cores). */
+int klsp_compute_orbits(const klsp_params* p, const double* Rc, const double* e,
+                        const double* R1, const double* R2, const double* E,
+                        const double* L, const double* L2, const double* SGNL,
+                        double* Omega_1, double* Omega_2, double* w1, double* ra,
+                        double* pha, double* jacobian, double* F0, double* FE,
+                        double* FL, int n_threads);
+
+/* omega-independent arrays of precompute_all, BasisFunctions.jl:127-153:
+ * DJ[NPh], F0l[NPh, n_l], Omega_res[NPh, n_l]. */
+int klsp_precompute_arrays(const klsp_params* p, const double* S_RC, const double* S_e,
+                           const double* jacobian, const double* Omega_1,
+                           const double* Omega_2, const double* FE, const double* FL,
+                           double* DJ, double* F0l, double* Omega_res);
+
+/* ------------------------------------------------------------------ */
+/* Device residency (PrecomputedData -> DevicePrecomputed)             */
+/* ------------------------------------------------------------------ */
+
+/* omega-independent arrays of PrecomputedData (BasisFunctions.jl:30-60):
+ * sets the problem dimensions of the context and copies the small arrays to
+ * the device (the reference keeps them on the host, BasisFunctions.jl:390). */
+typedef struct klsp_problem {
+    int64_t NPh;            /* number of orbits (NR*Ne)                     */
+    int64_t N_alpha;        /* N                                            */
+    int64_t n_l;            /* number of radial harmonics                   */
+    int m;
+    double G;
+    const double* DJ;       /* [NPh]                                        */
+    const double* F0l;      /* [NPh, n_l]                                   */
+    const double* Omega_res;/* [NPh, n_l]                                   */
+    const double* N_kernel; /* [N]                                          */
+} klsp_problem;
+
+int klsp_upload_problem(klsp_ctx* ctx, const klsp_problem* prob);
+
+/* compute_W_l, BasisFunctions.jl:199 (W_l(alpha) orbit quadrature, Eq. 41),
+ * evaluated on the device as one GEMM per orbit and kept device-resident in
+ * FP64.  Orbit arrays are [nwa, NPh]; alpha [N]; l_arr [n_l] (must match the
+ * uploaded problem).  If W_out is non-NULL the [NPh, N, n_l] complex array
+ * is also copied back (24.7 GB at the default grid: pass NULL normally). */
+int klsp_compute_W(klsp_ctx* ctx, int64_t nwa, const double* w1, const double* ra,
+                   const double* pha, const double* Omega_1, const double* Omega_2,
+                   const double* alpha, const int32_t* l_arr, double* W_out);
+
+/* Upload a caller-supplied W_l_mat [NPh, N, n_l] complex128 (to_device,
+ * BasisFunctions.jl:421). */
+int klsp_upload_W(klsp_ctx* ctx, const double* W);
+
+/* Copy the device W back in the reference layout [NPh, N, n_l]. */
+int klsp_download_W(klsp_ctx* ctx, double* W_out);
+
+/* Materialise the device copy of W used by `precision` (the complex64 copy
+ * for KLSP_PREC_FP32).  Called implicitly by the first build that needs it. */
+int klsp_prepare(klsp_ctx* ctx, int precision);
+
+/* ------------------------------------------------------------------ */
+/* The hot path                                                        */
+/* ------------------------------------------------------------------ */
+
+/* build_M_matrix, MatrixBuilder.jl:242 (= build_M_matrix_cpu :27 /
+ * build_M_matrix_gpu :100): M[b,a] = G*N[a]*sum_l sum_k W[k,b,l] c[k,l]
+ * conj(W[k,a,l]), c = DJ*F0l/(omega - Omega_res) with the 1e-12 clamp.
+ * M_out is [N, N] complex, column-major. */
+int klsp_build_M(klsp_ctx* ctx, double omega_re, double omega_im, int precision,
+                 double* M_out);
+
+/* compute_determinant / compute_determinant_batched, MatrixBuilder.jl:197,
+ * :215: det(I - M(omega_i)) for n values of omega (interleaved complex),
+ * FP64 complex LU with partial pivoting on the device. */
+int klsp_det(klsp_ctx* ctx, const double* omegas, int64_t n, int precision,
+             double* dets_out);
+
+/* newton_search, NewtonSolver.jl:36 (NewtonResult :19). */
+typedef struct klsp_newton_params {
+    int max_iter;          /* NewtonConfig.max_iter                         */
+    double tol;            /* converged when |det| < tol                    */
+    double delta;          /* forward-difference step in (Omega_p, gamma)   */
+    int use_line_search;   /* halving line search with bounds               */
+    int max_line_search;
+    double Omega_p_init;
+    double gamma_init;
+    int precision;         /* klsp_precision of every build                 */
+} klsp_newton_params;
+
+typedef struct klsp_newton_result {
+    double Omega_p, gamma;
+    double omega[2];       /* m*Omega_p + i*gamma                           */
+    double det[2];         /* det at the final (Omega_p, gamma)             */
+    int iterations;
+    int converged;
+    int n_det_evals;       /* number of det(I - M) evaluations performed    */
+} klsp_newton_result;
+
+/* history_out: [max_iter, 3] column-major (Omega_p, gamma, |det|); the first
+ * `iterations` rows are filled.  May be NULL. */
+int klsp_newton(klsp_ctx* ctx, const klsp_newton_params* np, klsp_newton_result* res,
+                double* history_out);
+
+/* grid_scan + find_minimum, GridScan.jl:25, :234.  det_grid_out is
+ * [n_gamma, n_Op] complex column-major; Omega_p in range(Op_min, Op_max,
+ * n_Op), gamma in range(g_min, g_max, n_gamma), omega = m*Omega_p + i*gamma. */
+typedef struct klsp_scan_params {
+    int n_Op, n_gamma;
+    double Omega_p_min, Omega_p_max;
+    double gamma_min, gamma_max;
+    int precision;
+} klsp_scan_params;
+
+int klsp_grid_scan(klsp_ctx* ctx, const klsp_scan_params* sp, double* det_grid_out,
+                   double* Omega_p_init_out, double* gamma_init_out);
+
+/* ------------------------------------------------------------------ */
+/* Measurement                                                         */
+/* ------------------------------------------------------------------ */
+
+/* Per-phase device timings of the most recent klsp_det / klsp_build_M call,
+ * measured with HIP events on the context's stream when profiling is on. */
+typedef struct klsp_timing {
+    double weights_ms;     /* c = DJ*F0l/(omega - Omega_res)                */
+    double contract_ms;    /* the M(omega) contraction kernel               */
+    double reduce_ms;      /* K-split partial sums + G*N epilogue           */
+    double lu_ms;          /* complex LU determinant                        */
+    int64_t n_omega;       /* omegas processed by the call                  */
+    int64_t contract_launches;
+} klsp_timing;
+
+int klsp_set_profiling(klsp_ctx* ctx, int on);
+int klsp_last_timing(klsp_ctx* ctx, klsp_timing* t_out);
+
+/* Device time of the last klsp_compute_W (orbit preprocessing + GEMM). */
+int klsp_last_W_ms(klsp_ctx* ctx, double* ms_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KLSP_H */

@@ -1,0 +1,298 @@
+"""ctypes binding of libklsp.so (the C ABI declared in include/klsp.h).
+
+There is no fallback: if the library is missing the import of any compute
+function raises, telling the user to build it.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import weakref
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libklsp.so")
+
+KLSP_OK = 0
+KLSP_ERR_INVALID_ARG = 1
+KLSP_ERR_OOM = 2
+KLSP_ERR_HIP = 3
+KLSP_ERR_NOT_READY = 4
+KLSP_ERR_NO_DEVICE = 5
+
+PREC_FP64 = 0
+PREC_FP32 = 1
+
+_STATUS_NAMES = {
+    KLSP_ERR_INVALID_ARG: "invalid argument",
+    KLSP_ERR_OOM: "out of memory",
+    KLSP_ERR_HIP: "HIP runtime error",
+    KLSP_ERR_NOT_READY: "not ready",
+    KLSP_ERR_NO_DEVICE: "no gfx950 device",
+}
+
+# Every symbol include/klsp.h declares (checked by tests/test_abi.py).
+EXPORTED = [
+    "klsp_version", "klsp_device_count", "klsp_create", "klsp_destroy", "klsp_last_error",
+    "klsp_trapezoid_coef", "klsp_simpson_coef", "klsp_linrange", "klsp_gamma_complex",
+    "klsp_compute_N_kernel", "klsp_compute_grids", "klsp_compute_orbits",
+    "klsp_precompute_arrays", "klsp_upload_problem", "klsp_compute_W", "klsp_upload_W",
+    "klsp_download_W", "klsp_prepare", "klsp_build_M", "klsp_det", "klsp_newton",
+    "klsp_grid_scan", "klsp_set_profiling", "klsp_last_timing", "klsp_last_W_ms",
+]
+
+
+class KlspError(RuntimeError):
+    """A non-zero status returned by libklsp."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"klsp: {_STATUS_NAMES.get(status, 'error')} ({status}): {message}")
+        self.status = status
+
+
+class klsp_params(ct.Structure):
+    _fields_ = [("m", ct.c_int), ("G", ct.c_double), ("L0", ct.c_double),
+                ("n_zang", ct.c_int), ("q1", ct.c_int), ("NR", ct.c_int), ("Ne", ct.c_int),
+                ("Rc_min", ct.c_double), ("Rc_max", ct.c_double), ("v_min", ct.c_double),
+                ("N_alpha", ct.c_int), ("alpha_max", ct.c_double), ("l_min", ct.c_int),
+                ("l_max", ct.c_int), ("nw", ct.c_int), ("nwa", ct.c_int)]
+
+
+_D = ct.POINTER(ct.c_double)
+
+
+class klsp_problem(ct.Structure):
+    _fields_ = [("NPh", ct.c_int64), ("N_alpha", ct.c_int64), ("n_l", ct.c_int64),
+                ("m", ct.c_int), ("G", ct.c_double), ("DJ", _D), ("F0l", _D),
+                ("Omega_res", _D), ("N_kernel", _D)]
+
+
+class klsp_newton_params(ct.Structure):
+    _fields_ = [("max_iter", ct.c_int), ("tol", ct.c_double), ("delta", ct.c_double),
+                ("use_line_search", ct.c_int), ("max_line_search", ct.c_int),
+                ("Omega_p_init", ct.c_double), ("gamma_init", ct.c_double),
+                ("precision", ct.c_int)]
+
+
+class klsp_newton_result(ct.Structure):
+    _fields_ = [("Omega_p", ct.c_double), ("gamma", ct.c_double),
+                ("omega", ct.c_double * 2), ("det", ct.c_double * 2),
+                ("iterations", ct.c_int), ("converged", ct.c_int), ("n_det_evals", ct.c_int)]
+
+
+class klsp_scan_params(ct.Structure):
+    _fields_ = [("n_Op", ct.c_int), ("n_gamma", ct.c_int), ("Omega_p_min", ct.c_double),
+                ("Omega_p_max", ct.c_double), ("gamma_min", ct.c_double),
+                ("gamma_max", ct.c_double), ("precision", ct.c_int)]
+
+
+class klsp_timing(ct.Structure):
+    _fields_ = [("weights_ms", ct.c_double), ("contract_ms", ct.c_double),
+                ("reduce_ms", ct.c_double), ("lu_ms", ct.c_double),
+                ("n_omega", ct.c_int64), ("contract_launches", ct.c_int64)]
+
+
+_lib = None
+
+
+def lib() -> ct.CDLL:
+    """Load libklsp.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"libklsp.so not found at {LIB_PATH}; build it with "
+            "`python -m kalnajs_log_spirals_amd.build_lib` (this package has no CPU fallback)")
+    L = ct.CDLL(LIB_PATH)
+    P = ct.c_void_p
+    i64 = ct.c_int64
+    L.klsp_version.restype = ct.c_char_p
+    L.klsp_version.argtypes = []
+    L.klsp_device_count.argtypes = [ct.POINTER(ct.c_int)]
+    L.klsp_create.argtypes = [ct.c_int, ct.POINTER(P)]
+    L.klsp_destroy.argtypes = [P]
+    L.klsp_last_error.restype = ct.c_char_p
+    L.klsp_last_error.argtypes = [P]
+    L.klsp_trapezoid_coef.argtypes = [_D, i64, _D]
+    L.klsp_simpson_coef.argtypes = [_D, i64, _D]
+    L.klsp_linrange.argtypes = [ct.c_double, ct.c_double, i64, _D]
+    L.klsp_gamma_complex.argtypes = [ct.c_double, ct.c_double, _D]
+    L.klsp_compute_N_kernel.argtypes = [ct.c_int, _D, _D, i64, _D]
+    L.klsp_compute_grids.argtypes = [ct.POINTER(klsp_params)] + [_D] * 11
+    L.klsp_compute_orbits.argtypes = [ct.POINTER(klsp_params)] + [_D] * 17 + [ct.c_int]
+    L.klsp_precompute_arrays.argtypes = [ct.POINTER(klsp_params)] + [_D] * 10
+    L.klsp_upload_problem.argtypes = [P, ct.POINTER(klsp_problem)]
+    L.klsp_compute_W.argtypes = [P, i64, _D, _D, _D, _D, _D, _D, ct.POINTER(ct.c_int32), _D]
+    L.klsp_upload_W.argtypes = [P, _D]
+    L.klsp_download_W.argtypes = [P, _D]
+    L.klsp_prepare.argtypes = [P, ct.c_int]
+    L.klsp_build_M.argtypes = [P, ct.c_double, ct.c_double, ct.c_int, _D]
+    L.klsp_det.argtypes = [P, _D, i64, ct.c_int, _D]
+    L.klsp_newton.argtypes = [P, ct.POINTER(klsp_newton_params),
+                              ct.POINTER(klsp_newton_result), _D]
+    L.klsp_grid_scan.argtypes = [P, ct.POINTER(klsp_scan_params), _D, _D, _D]
+    L.klsp_set_profiling.argtypes = [P, ct.c_int]
+    L.klsp_last_timing.argtypes = [P, ct.POINTER(klsp_timing)]
+    L.klsp_last_W_ms.argtypes = [P, _D]
+    for name in EXPORTED:
+        if name not in ("klsp_version", "klsp_last_error"):
+            getattr(L, name).restype = ct.c_int
+    _lib = L
+    return L
+
+
+def dptr(a: np.ndarray):
+    return a.ctypes.data_as(_D)
+
+
+def f64c(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def check(status: int, ctx_handle=None, what: str = "") -> None:
+    if status != KLSP_OK:
+        msg = lib().klsp_last_error(ctx_handle).decode() if ctx_handle else what
+        raise KlspError(status, msg or what)
+
+
+def version() -> str:
+    return lib().klsp_version().decode()
+
+
+def device_count() -> int:
+    n = ct.c_int(0)
+    lib().klsp_device_count(ct.byref(n))
+    return n.value
+
+
+def host_threads() -> int:
+    env = os.environ.get("KLSP_HOST_THREADS")
+    if env:
+        return max(1, int(env))
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
+def _destroy(handle):
+    if handle:
+        lib().klsp_destroy(handle)
+
+
+class Context:
+    """One device context (stream + device-resident problem and W)."""
+
+    def __init__(self, device: int = 0):
+        h = ct.c_void_p()
+        st = lib().klsp_create(device, ct.byref(h))
+        if st != KLSP_OK:
+            raise KlspError(st, f"cannot create a context on device {device}")
+        self.handle = h
+        self.device = device
+        self.dims = None
+        self._fin = weakref.finalize(self, _destroy, h)
+
+    def close(self):
+        self._fin()
+
+    def _chk(self, st, what=""):
+        check(st, self.handle, what)
+
+    # -- residency -----------------------------------------------------------
+    def upload_problem(self, NPh: int, N: int, n_l: int, m: int, G: float, DJ, F0l, Omega_res,
+                       N_kernel):
+        DJ = f64c(DJ)
+        F0l = np.asfortranarray(np.asarray(F0l, dtype=np.float64)).reshape(-1, order="F")
+        Ores = np.asfortranarray(np.asarray(Omega_res, dtype=np.float64)).reshape(-1, order="F")
+        Nk = f64c(N_kernel)
+        if DJ.size != NPh or F0l.size != NPh * n_l or Ores.size != NPh * n_l or Nk.size != N:
+            raise ValueError("upload_problem: array sizes do not match (NPh, N, n_l)")
+        F0l, Ores = f64c(F0l), f64c(Ores)
+        pr = klsp_problem(NPh, N, n_l, int(m), float(G), dptr(DJ), dptr(F0l), dptr(Ores), dptr(Nk))
+        self._chk(lib().klsp_upload_problem(self.handle, ct.byref(pr)))
+        self.dims = (int(NPh), int(N), int(n_l))
+
+    def compute_W(self, w1, ra, pha, Omega_1, Omega_2, alpha, l_arr, download: bool = False):
+        NPh, N, n_l = self.dims
+        w1, ra, pha = f64c(w1), f64c(ra), f64c(pha)
+        nwa = w1.size // NPh
+        if w1.size != NPh * nwa or ra.size != w1.size or pha.size != w1.size:
+            raise ValueError("compute_W: orbit arrays must be [nwa, NPh]")
+        o1, o2, al = f64c(Omega_1), f64c(Omega_2), f64c(alpha)
+        la = np.ascontiguousarray(l_arr, dtype=np.int32)
+        if o1.size != NPh or o2.size != NPh or al.size != N or la.size != n_l:
+            raise ValueError("compute_W: sizes do not match the uploaded problem")
+        out = np.empty(2 * NPh * N * n_l) if download else None
+        self._chk(lib().klsp_compute_W(self.handle, nwa, dptr(w1), dptr(ra), dptr(pha), dptr(o1),
+                                       dptr(o2), dptr(al), la.ctypes.data_as(ct.POINTER(ct.c_int32)),
+                                       dptr(out) if download else None))
+        if download:
+            return out.view(np.complex128).reshape((NPh, N, n_l), order="F")
+        return None
+
+    def upload_W(self, W):
+        NPh, N, n_l = self.dims
+        W = np.asarray(W, dtype=np.complex128)
+        if W.shape != (NPh, N, n_l):
+            raise ValueError(f"upload_W: expected shape {(NPh, N, n_l)}, got {W.shape}")
+        flat = np.ascontiguousarray(W.reshape(-1, order="F")).view(np.float64)
+        self._chk(lib().klsp_upload_W(self.handle, dptr(flat)))
+
+    def download_W(self) -> np.ndarray:
+        NPh, N, n_l = self.dims
+        out = np.empty(2 * NPh * N * n_l)
+        self._chk(lib().klsp_download_W(self.handle, dptr(out)))
+        return out.view(np.complex128).reshape((NPh, N, n_l), order="F")
+
+    def prepare(self, precision: int):
+        self._chk(lib().klsp_prepare(self.handle, precision))
+
+    # -- hot path ------------------------------------------------------------
+    def build_M(self, omega: complex, precision: int = PREC_FP64) -> np.ndarray:
+        N = self.dims[1]
+        out = np.empty(2 * N * N)
+        self._chk(lib().klsp_build_M(self.handle, float(np.real(omega)), float(np.imag(omega)),
+                                     precision, dptr(out)))
+        return out.view(np.complex128).reshape((N, N), order="F")
+
+    def det(self, omegas, precision: int = PREC_FP64) -> np.ndarray:
+        om = np.ascontiguousarray(np.atleast_1d(np.asarray(omegas, dtype=np.complex128)))
+        out = np.empty(om.size, dtype=np.complex128)
+        self._chk(lib().klsp_det(self.handle, dptr(om.view(np.float64)), om.size, precision,
+                                 dptr(out.view(np.float64))))
+        return out
+
+    def newton(self, Omega_p_init, gamma_init, max_iter=50, tol=1e-12, delta=1e-6,
+               use_line_search=False, max_line_search=10, precision=PREC_FP64):
+        npar = klsp_newton_params(int(max_iter), float(tol), float(delta), int(bool(use_line_search)),
+                                  int(max_line_search), float(Omega_p_init), float(gamma_init),
+                                  int(precision))
+        res = klsp_newton_result()
+        hist = np.zeros(3 * max(int(max_iter), 1))
+        self._chk(lib().klsp_newton(self.handle, ct.byref(npar), ct.byref(res), dptr(hist)))
+        n = res.iterations
+        H = hist.reshape((max(int(max_iter), 1), 3), order="F")[:n].copy()
+        return res, H
+
+    def grid_scan(self, n_Op, n_gamma, Op_min, Op_max, g_min, g_max, precision=PREC_FP64):
+        sp = klsp_scan_params(int(n_Op), int(n_gamma), float(Op_min), float(Op_max),
+                              float(g_min), float(g_max), int(precision))
+        dg = np.empty(2 * n_Op * n_gamma)
+        o1, o2 = ct.c_double(), ct.c_double()
+        self._chk(lib().klsp_grid_scan(self.handle, ct.byref(sp), dptr(dg), ct.byref(o1),
+                                       ct.byref(o2)))
+        return dg.view(np.complex128).reshape((n_gamma, n_Op), order="F"), o1.value, o2.value
+
+    # -- measurement ---------------------------------------------------------
+    def set_profiling(self, on: bool):
+        self._chk(lib().klsp_set_profiling(self.handle, int(bool(on))))
+
+    def last_timing(self) -> dict:
+        t = klsp_timing()
+        self._chk(lib().klsp_last_timing(self.handle, ct.byref(t)))
+        return {k: getattr(t, k) for k, _ in klsp_timing._fields_}
+
+    def last_W_ms(self) -> float:
+        v = ct.c_double()
+        self._chk(lib().klsp_last_W_ms(self.handle, ct.byref(v)))
+        return v.value

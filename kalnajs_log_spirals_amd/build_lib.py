@@ -1,0 +1,60 @@
+"""Build libklsp.so in-tree with hipcc for gfx950.
+
+    python -m kalnajs_log_spirals_amd.build_lib
+
+The shared library lands in kalnajs_log_spirals_amd/lib/ (git-ignored, but it
+travels with the source tree).  hipcc cross-compiles for gfx950 without a GPU.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+LIBDIR = os.path.join(HERE, "lib")
+LIB = os.path.join(LIBDIR, "libklsp.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# -ffp-contract=off: no implicit FMA contraction, so scalar FP64 code rounds
+# like the reference; MFMA builtins are unaffected.  Packed FP32 ops are
+# disabled for all device code.
+COMMON = [
+    "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
+    "-Wno-unused-function", "-Wno-unused-value", "-Wno-unused-result", f"-I{INCLUDE}", f"-I{CSRC}",
+]
+DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+SOURCES = [
+    ("klsp_host.cpp", False),
+    ("klsp_device.hip", True),
+]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose: bool = True) -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    objs = []
+    for src, is_hip in SOURCES:
+        obj = os.path.join(LIBDIR, src.rsplit(".", 1)[0] + ".o")
+        cmd = [HIPCC] + COMMON + (DEVICE + ["-x", "hip"] if is_hip else ["-x", "c++"])
+        cmd += ["-c", os.path.join(CSRC, src), "-o", obj]
+        _run(cmd)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
+    os.replace(tmp, LIB)
+    for o in objs:
+        os.remove(o)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,1179 @@
+// klsp_device.hip -- CDNA4 (gfx950) kernels of the det(I - M(omega)) path and
+// the C-ABI drivers above them.
+//
+// Stage map (reference -> kernel):
+//   compute_W_l        BasisFunctions.jl:199  -> k_orbit_prep + k_W_f64
+//   weights c(omega)   MatrixBuilder.jl:124   -> k_weights
+//   sum_l W^T c conj W MatrixBuilder.jl:119   -> k_contract_f64 / k_contract_f32
+//   *G, *N_kernel      MatrixBuilder.jl:144   -> k_reduce
+//   det(I - M)         MatrixBuilder.jl:182   -> k_lu_det
+//
+// Device layout of W ("rows of K"): W[K][Np] with K = l_index*NPh + k
+// (the reference's column order of W_l_mat, flattened over l) and the alpha
+// axis padded to Np = roundup(N, 16) with zeros; K is padded to Kp =
+// roundup(K, 4) with zero rows.  A 16x16x4 MFMA operand (16 alphas x 4 K)
+// is then four 256-byte row segments, so every operand load is coalesced,
+// and the same array feeds both operands of M = W^T diag(c) conj(W).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "klsp.h"
+#include "klsp_internal.hpp"
+
+using namespace klsp;
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+#define KLSP_VERSION_STRING "klsp 0.1.0 gfx950"
+
+// =====================================================================
+// Device helpers
+// =====================================================================
+
+// 1/(c + i d) by Smith's algorithm with the same range scaling as Julia's
+// inv(::ComplexF64); the weights use it so c(omega) rounds like the reference.
+__device__ __forceinline__ double2 d_inv(double c, double d)
+{
+    double s = 1.0;
+    const double cd = fmax(fabs(c), fabs(d));
+    const double eps = 2.220446049250313e-16;
+    if (cd >= 0.5 * 1.7976931348623157e308) { c *= 0.5; d *= 0.5; s *= 0.5; }
+    if (cd <= 2.2250738585072014e-308 * 2.0 / eps) {
+        const double bs = 2.0 / (eps * eps);
+        c *= bs; d *= bs; s *= bs;
+    }
+    double p, q;
+    if (fabs(d) <= fabs(c)) {
+        const double r = d / c, t = 1.0 / (c + d * r);
+        p = t; q = -r * t;
+    } else {
+        const double r = c / d, t = 1.0 / (d + c * r);
+        p = r * t; q = -t;
+    }
+    return make_double2(p * s, q * s);
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// =====================================================================
+// Stage 1: W_l(alpha) orbit quadrature (BasisFunctions.jl:199-266)
+// =====================================================================
+
+// Per-orbit preprocessing, one workgroup per orbit k: validity (any
+// non-finite w1/ra/pha skips the orbit, :224-226), trapezoid weights of the
+// non-uniform w1 samples (:229), phi = (Om2/Om1) w1 - pha (:232-235).  Emits
+// per sample: u = ln r, wt = Sw1 / (pi sqrt r) (0 where the integrand is
+// non-finite, :248), w1 and m*phi, padded to nwa_p samples with wt = 0.
+__global__ __launch_bounds__(256) void k_orbit_prep(
+    const double* __restrict__ w1, const double* __restrict__ ra,
+    const double* __restrict__ pha, const double* __restrict__ Om1,
+    const double* __restrict__ Om2, int nwa, int nwa_p, int m, double* __restrict__ u,
+    double* __restrict__ wt, double* __restrict__ w1o, double* __restrict__ mph,
+    int* __restrict__ valid)
+{
+    const int64_t k = blockIdx.x;
+    const double* W1 = w1 + k * nwa;
+    const double* RA = ra + k * nwa;
+    const double* PH = pha + k * nwa;
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    for (int t = threadIdx.x; t < nwa; t += blockDim.x)
+        if (!isfinite(W1[t]) || !isfinite(RA[t]) || !isfinite(PH[t])) atomicOr(&bad, 1);
+    __syncthreads();
+    const bool ok = (bad == 0);
+    const double r21 = Om2[k] / Om1[k];
+    const double inv_pi = 1.0 / 3.14159265358979323846;
+    for (int t = threadIdx.x; t < nwa_p; t += blockDim.x) {
+        double lu = 0.0, w = 0.0, x1 = 0.0, mp = 0.0;
+        if (ok && t < nwa) {
+            double sw;
+            if (t == 0) sw = (W1[1] - W1[0]) / 2;
+            else if (t == nwa - 1) sw = (W1[nwa - 1] - W1[nwa - 2]) / 2;
+            else sw = (W1[t + 1] - W1[t - 1]) / 2;
+            const double r = RA[t];
+            const double phi = r21 * W1[t] - PH[t];
+            x1 = W1[t];
+            mp = (double)m * phi;
+            lu = log(r);
+            w = sw / sqrt(r) * inv_pi;
+            if (!(r > 0.0) || !isfinite(lu) || !isfinite(w)) { lu = 0.0; w = 0.0; }
+        }
+        const int64_t o = k * nwa_p + t;
+        u[o] = lu; wt[o] = w; w1o[o] = x1; mph[o] = mp;
+    }
+    if (threadIdx.x == 0) valid[k] = ok ? 1 : 0;
+}
+
+// W_k = A_k B_k per orbit on FP64 MFMA: A[l, t] = wt_t cos(l w1_t + m phi_t)
+// (real, n_l x nwa), B[t, a] = exp(-i alpha_a ln r_t) (nwa x N).
+// Workgroup = (orbit k, 32 alphas); wave w owns l-blocks w, w+4, w+8, w+12.
+// Operands are generated in registers in the MFMA lane layout:
+//   A (16 l x 4 t): lane = i + 16*kk holds A[l0+i][t0+kk]
+//   B (4 t x 16 a): lane = i + 16*kk holds B[t0+kk][a0+i]
+//   D (16 l x 16 a): lane holds D[4*kk + v][i], v = 0..3
+constexpr int kWMaxLB = 4;
+
+__global__ __launch_bounds__(256) void k_W_f64(
+    const double* __restrict__ u, const double* __restrict__ wt,
+    const double* __restrict__ w1o, const double* __restrict__ mph,
+    const int* __restrict__ valid, const double* __restrict__ alpha,
+    const int* __restrict__ l_arr, int64_t NPh, int N, int Np, int n_l, int nwa_p,
+    double2* __restrict__ W)
+{
+    const int64_t k = blockIdx.x;
+    if (!valid[k]) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = lane & 15, kk = lane >> 4;
+    const int a0 = blockIdx.y * 32;
+    const int nLB = (n_l + 15) / 16;
+    const bool bj0 = a0 < N, bj1 = a0 + 16 < N;
+    double al[2];
+    for (int j = 0; j < 2; ++j) {
+        const int a = a0 + 16 * j + i;
+        al[j] = a < N ? alpha[a] : 0.0;
+    }
+    double lv[kWMaxLB];
+    for (int q = 0; q < kWMaxLB; ++q) {
+        const int li = (wave + 4 * q) * 16 + i;
+        lv[q] = li < n_l ? (double)l_arr[li] : 0.0;
+    }
+    v4d accr[kWMaxLB][2], acci[kWMaxLB][2];
+    for (int q = 0; q < kWMaxLB; ++q)
+        for (int j = 0; j < 2; ++j) { accr[q][j] = (v4d){0, 0, 0, 0}; acci[q][j] = accr[q][j]; }
+
+    const double* U = u + k * nwa_p;
+    const double* WT = wt + k * nwa_p;
+    const double* X1 = w1o + k * nwa_p;
+    const double* MP = mph + k * nwa_p;
+    for (int t0 = 0; t0 < nwa_p; t0 += 4) {
+        const int t = t0 + kk;
+        const double ut = U[t], wtt = WT[t], w1t = X1[t], mp = MP[t];
+        double br[2], bi[2];
+        for (int j = 0; j < 2; ++j) {
+            const double y = -al[j] * ut;
+            sincos(y, &bi[j], &br[j]);
+        }
+#pragma unroll
+        for (int q = 0; q < kWMaxLB; ++q) {
+            if (wave + 4 * q >= nLB) break;
+            const double a = wtt * cos(lv[q] * w1t + mp);
+            if (bj0) {
+                accr[q][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, br[0], accr[q][0], 0, 0, 0);
+                acci[q][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bi[0], acci[q][0], 0, 0, 0);
+            }
+            if (bj1) {
+                accr[q][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, br[1], accr[q][1], 0, 0, 0);
+                acci[q][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bi[1], acci[q][1], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kWMaxLB; ++q) {
+        const int lb = wave + 4 * q;
+        if (lb >= nLB) break;
+        for (int j = 0; j < 2; ++j) {
+            const int a = a0 + 16 * j + i;
+            if (a >= N) continue;
+            for (int v = 0; v < 4; ++v) {
+                const int li = lb * 16 + 4 * kk + v;
+                if (li < n_l)
+                    W[((int64_t)li * NPh + k) * Np + a] = make_double2(accr[q][j][v], acci[q][j][v]);
+            }
+        }
+    }
+}
+
+// Device rows-of-K layout <-> reference [NPh, N, n_l] for one l-slab.
+__global__ void k_W_slab_to_ref(const double2* __restrict__ W, int64_t NPh, int N, int Np,
+                                int li, double2* __restrict__ out)
+{
+    const int64_t n = NPh * N;
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
+         x += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = x % NPh, a = x / NPh;
+        out[x] = W[((int64_t)li * NPh + k) * Np + a];
+    }
+}
+
+__global__ void k_W_ref_to_slab(const double2* __restrict__ in, int64_t NPh, int N, int Np,
+                                int li, double2* __restrict__ W)
+{
+    const int64_t n = NPh * N;
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
+         x += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = x % NPh, a = x / NPh;
+        W[((int64_t)li * NPh + k) * Np + a] = in[x];
+    }
+}
+
+__global__ void k_W64_to_32(const double2* __restrict__ W, int64_t n, float2* __restrict__ Wf)
+{
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
+         x += (int64_t)gridDim.x * blockDim.x) {
+        const double2 w = W[x];
+        Wf[x] = make_float2((float)w.x, (float)w.y);
+    }
+}
+
+// =====================================================================
+// Stage 2: resonance weights c[b][K] (MatrixBuilder.jl:43-51)
+// d = omega - Omega_res (|d| < 1e-12 -> 1e-12 + 1e-12 i); c = DJ*F0l / d.
+// =====================================================================
+__global__ void k_weights(const double* __restrict__ DJ, const double* __restrict__ F0l,
+                          const double* __restrict__ Ores, int64_t NPh, int64_t K, int64_t Kp,
+                          const double* __restrict__ om, int nb, double2* __restrict__ c)
+{
+    const int64_t n = (int64_t)nb * Kp;
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
+         x += (int64_t)gridDim.x * blockDim.x) {
+        const int b = (int)(x / Kp);
+        const int64_t q = x - (int64_t)b * Kp;
+        double2 r = make_double2(0.0, 0.0);
+        if (q < K) {
+            double dr = om[2 * b] - Ores[q], di = om[2 * b + 1] - 0.0;
+            if (hypot(dr, di) < 1e-12) { dr = 1e-12; di = 1e-12; }
+            const double xw = DJ[q % NPh] * F0l[q];
+            const double2 iv = d_inv(dr, di);
+            r = make_double2(xw * iv.x, xw * iv.y);
+        }
+        c[x] = r;
+    }
+}
+
+// =====================================================================
+// Stage 3: the contraction M_b[beta, alpha] = sum_K W[K][beta] c_b[K]
+// conj(W[K][alpha]) (MatrixBuilder.jl:119-141), split over K.
+//
+// Grid: T*S workgroups of 4 waves; T = tilesN^2 output tiles of 64x64, S
+// K-splits (S % 8 == 0).  Linear block L is remapped so that the T tiles of
+// one K-split land on the same XCD (blocks go round-robin over the 8 XCDs),
+// which keeps the W rows they share in that XCD's L2.  Each wave owns a
+// 32x32 sub-tile = 2x2 MFMA blocks, for NB omegas at once (the W operand
+// loads are shared by the batch).  Complex products use four real MFMAs:
+//   Cr += Ar*Br + (-Ai)*Bi,   Ci += Ar*Bi + Ai*Br,   B = c * conj(W_alpha).
+// Partials go to P[b][s][alpha][beta] (column-major M per split).
+// =====================================================================
+__device__ __forceinline__ void tile_coords(int T, int tilesN, int& tile, int& s, int& tb, int& ta)
+{
+    const int L = blockIdx.x;
+    const int xcd = L & 7, j = L >> 3;
+    tile = j % T;
+    s = (j / T) * 8 + xcd;
+    tb = tile / tilesN;
+    ta = tile % tilesN;
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_contract_f64(
+    const double2* __restrict__ W, const double2* __restrict__ cw, double2* __restrict__ P,
+    int64_t Kp, int Np, int64_t Ks, int T, int tilesN, int S)
+{
+    int tile, s, tb, ta;
+    tile_coords(T, tilesN, tile, s, tb, ta);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = lane & 15, kk = lane >> 4;
+    const int b0 = tb * kTile + (wave >> 1) * 32;
+    const int a0 = ta * kTile + (wave & 1) * 32;
+    if (b0 >= Np || a0 >= Np) return;
+    const bool vb1 = b0 + 16 < Np, va1 = a0 + 16 < Np;
+    const int64_t Kb = (int64_t)s * Ks;
+    const int64_t Ke = min(Kp, Kb + Ks);
+
+    v4d cr[NB][2][2], ci[NB][2][2];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) { cr[b][x][y] = (v4d){0, 0, 0, 0}; ci[b][x][y] = cr[b][x][y]; }
+
+    const double2 zero2 = make_double2(0.0, 0.0);
+    for (int64_t K0 = Kb; K0 < Ke; K0 += kKStep) {
+        const int64_t K = K0 + kk;
+        const double2* row = W + K * Np;
+        const double2 A0 = row[b0 + i];
+        const double2 A1 = vb1 ? row[b0 + 16 + i] : zero2;
+        const double2 X0 = row[a0 + i];
+        const double2 X1 = va1 ? row[a0 + 16 + i] : zero2;
+        const double nA0 = -A0.y, nA1 = -A1.y;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const double2 c = cw[(int64_t)b * Kp + K];
+            // B = c * conj(X)
+            const double2 B0 = make_double2(c.x * X0.x + c.y * X0.y, c.y * X0.x - c.x * X0.y);
+            const double2 B1 = make_double2(c.x * X1.x + c.y * X1.y, c.y * X1.x - c.x * X1.y);
+#define KLSP_CMAC(ACR, ACI, AR, AI, NAI, BB)                                              \
+    ACR = __builtin_amdgcn_mfma_f64_16x16x4f64(AR, BB.x, ACR, 0, 0, 0);                    \
+    ACR = __builtin_amdgcn_mfma_f64_16x16x4f64(NAI, BB.y, ACR, 0, 0, 0);                   \
+    ACI = __builtin_amdgcn_mfma_f64_16x16x4f64(AR, BB.y, ACI, 0, 0, 0);                    \
+    ACI = __builtin_amdgcn_mfma_f64_16x16x4f64(AI, BB.x, ACI, 0, 0, 0);
+            KLSP_CMAC(cr[b][0][0], ci[b][0][0], A0.x, A0.y, nA0, B0)
+            if (va1) { KLSP_CMAC(cr[b][0][1], ci[b][0][1], A0.x, A0.y, nA0, B1) }
+            if (vb1) {
+                KLSP_CMAC(cr[b][1][0], ci[b][1][0], A1.x, A1.y, nA1, B0)
+                if (va1) { KLSP_CMAC(cr[b][1][1], ci[b][1][1], A1.x, A1.y, nA1, B1) }
+            }
+        }
+    }
+    // D lane layout: rows (beta) 4*kk+v, column (alpha) i.
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        double2* Pb = P + ((int64_t)b * S + s) * (int64_t)Np * Np;
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+            if (x == 1 && !vb1) continue;
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                if (y == 1 && !va1) continue;
+                const int a = a0 + 16 * y + i;
+                const int bb = b0 + 16 * x + 4 * kk;
+                double2* dst = Pb + (int64_t)a * Np + bb;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) dst[v] = make_double2(cr[b][x][y][v], ci[b][x][y][v]);
+            }
+        }
+    }
+}
+
+// FP32 variant: W in complex64, weights rounded to FP32 after being formed
+// in FP64, FP32 MFMA over K-tiles of kF32Tile, K-tile partials added in FP64.
+constexpr int kF32Tile = 256;
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_contract_f32(
+    const float2* __restrict__ W, const double2* __restrict__ cw, double2* __restrict__ P,
+    int64_t Kp, int Np, int64_t Ks, int T, int tilesN, int S)
+{
+    int tile, s, tb, ta;
+    tile_coords(T, tilesN, tile, s, tb, ta);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = lane & 15, kk = lane >> 4;
+    const int b0 = tb * kTile + (wave >> 1) * 32;
+    const int a0 = ta * kTile + (wave & 1) * 32;
+    if (b0 >= Np || a0 >= Np) return;
+    const bool vb1 = b0 + 16 < Np, va1 = a0 + 16 < Np;
+    const int64_t Kb = (int64_t)s * Ks;
+    const int64_t Ke = min(Kp, Kb + Ks);
+
+    v4d dr[NB][2][2], di[NB][2][2];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) { dr[b][x][y] = (v4d){0, 0, 0, 0}; di[b][x][y] = dr[b][x][y]; }
+
+    const float2 zero2 = make_float2(0.f, 0.f);
+    for (int64_t T0 = Kb; T0 < Ke; T0 += kF32Tile) {
+        const int64_t Te = min(Ke, T0 + kF32Tile);
+        v4f cr[NB][2][2], ci[NB][2][2];
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) { cr[b][x][y] = (v4f){0, 0, 0, 0}; ci[b][x][y] = cr[b][x][y]; }
+        for (int64_t K0 = T0; K0 < Te; K0 += kKStep) {
+            const int64_t K = K0 + kk;
+            const float2* row = W + K * Np;
+            const float2 A0 = row[b0 + i];
+            const float2 A1 = vb1 ? row[b0 + 16 + i] : zero2;
+            const float2 X0 = row[a0 + i];
+            const float2 X1 = va1 ? row[a0 + 16 + i] : zero2;
+            const float nA0 = -A0.y, nA1 = -A1.y;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const double2 c64 = cw[(int64_t)b * Kp + K];
+                const float2 c = make_float2((float)c64.x, (float)c64.y);
+                const float2 B0 = make_float2(c.x * X0.x + c.y * X0.y, c.y * X0.x - c.x * X0.y);
+                const float2 B1 = make_float2(c.x * X1.x + c.y * X1.y, c.y * X1.x - c.x * X1.y);
+#define KLSP_CMAC32(ACR, ACI, AR, AI, NAI, BB)                                            \
+    ACR = __builtin_amdgcn_mfma_f32_16x16x4f32(AR, BB.x, ACR, 0, 0, 0);                    \
+    ACR = __builtin_amdgcn_mfma_f32_16x16x4f32(NAI, BB.y, ACR, 0, 0, 0);                   \
+    ACI = __builtin_amdgcn_mfma_f32_16x16x4f32(AR, BB.y, ACI, 0, 0, 0);                    \
+    ACI = __builtin_amdgcn_mfma_f32_16x16x4f32(AI, BB.x, ACI, 0, 0, 0);
+                KLSP_CMAC32(cr[b][0][0], ci[b][0][0], A0.x, A0.y, nA0, B0)
+                if (va1) { KLSP_CMAC32(cr[b][0][1], ci[b][0][1], A0.x, A0.y, nA0, B1) }
+                if (vb1) {
+                    KLSP_CMAC32(cr[b][1][0], ci[b][1][0], A1.x, A1.y, nA1, B0)
+                    if (va1) { KLSP_CMAC32(cr[b][1][1], ci[b][1][1], A1.x, A1.y, nA1, B1) }
+                }
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        dr[b][x][y][v] += (double)cr[b][x][y][v];
+                        di[b][x][y][v] += (double)ci[b][x][y][v];
+                    }
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        double2* Pb = P + ((int64_t)b * S + s) * (int64_t)Np * Np;
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+            if (x == 1 && !vb1) continue;
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                if (y == 1 && !va1) continue;
+                const int a = a0 + 16 * y + i;
+                const int bb = b0 + 16 * x + 4 * kk;
+                double2* dst = Pb + (int64_t)a * Np + bb;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) dst[v] = make_double2(dr[b][x][y][v], di[b][x][y][v]);
+            }
+        }
+    }
+}
+
+// Sum of the S K-split partials (fixed order, FP64), then *G and *N[alpha]
+// (MatrixBuilder.jl:144-149).  M is [nb][N, N] column-major.
+__global__ void k_reduce(const double2* __restrict__ P, int S, int Np, int N, int nb, double G,
+                         const double* __restrict__ Nk, double2* __restrict__ M)
+{
+    const int64_t NN = (int64_t)N * N, n = nb * NN;
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
+         x += (int64_t)gridDim.x * blockDim.x) {
+        const int b = (int)(x / NN);
+        const int64_t r = x - b * NN;
+        const int beta = (int)(r % N), alpha = (int)(r / N);
+        const double2* src = P + (int64_t)b * S * Np * Np + (int64_t)alpha * Np + beta;
+        double sr = 0.0, si = 0.0;
+        for (int s = 0; s < S; ++s) {
+            const double2 v = src[(int64_t)s * Np * Np];
+            sr += v.x;
+            si += v.y;
+        }
+        const double g = Nk[alpha];
+        M[x] = make_double2(sr * G * g, si * G * g);
+    }
+}
+
+// =====================================================================
+// Stage 4: det(I - M) by complex LU with partial pivoting, one workgroup
+// per matrix (MatrixBuilder.jl:182 -> LinearAlgebra det -> LAPACK zgetrf).
+// Pivot = first row maximising |re|+|im| (izamax), multipliers scaled by the
+// reciprocal pivot, rank-1 trailing update; det = prod U_jj * (-1)^swaps
+// accumulated in LinearAlgebra's order.  The working copy A = I - M lives in
+// global memory (L2-resident); the pivot column and row are cached in LDS.
+// =====================================================================
+__device__ __forceinline__ void argmax_merge(double& v, int& idx, double ov, int oi)
+{
+    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+}
+
+__global__ __launch_bounds__(1024) void k_lu_det(const double2* __restrict__ M,
+                                                  double2* __restrict__ Awork, int N,
+                                                  double2* __restrict__ det_out)
+{
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int64_t NN = (int64_t)N * N;
+    const double2* Mb = M + b * NN;
+    double2* A = Awork + b * NN;
+    __shared__ double2 colj[kMaxN];
+    __shared__ double2 rowj[kMaxN];
+    __shared__ double redv[32];
+    __shared__ int redi[32];
+    __shared__ int s_piv;
+
+    for (int64_t x = tid; x < NN; x += nth) {
+        const double2 m = Mb[x];
+        const int r = (int)(x % N), c = (int)(x / N);
+        A[x] = make_double2((r == c ? 1.0 : 0.0) - m.x, 0.0 - m.y);
+    }
+    __syncthreads();
+
+    double2 det = make_double2(1.0, 0.0);
+    int swaps = 0;
+    bool singular = false;
+    for (int j = 0; j < N; ++j) {
+        // pivot search over rows j..N-1 of column j
+        double bv = -1.0;
+        int bi = N;
+        for (int r = j + tid; r < N; r += nth) {
+            const double2 a = A[r + (int64_t)N * j];
+            const double v = fabs(a.x) + fabs(a.y);
+            if (v > bv) { bv = v; bi = r; }
+        }
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double ov = __shfl_xor(bv, off);
+            const int oi = __shfl_xor(bi, off);
+            argmax_merge(bv, bi, ov, oi);
+        }
+        const int wid = tid >> 6;
+        if ((tid & 63) == 0) { redv[wid] = bv; redi[wid] = bi; }
+        __syncthreads();
+        if (tid == 0) {
+            double v = redv[0];
+            int idx = redi[0];
+            for (int w = 1; w < (nth + 63) / 64; ++w) argmax_merge(v, idx, redv[w], redi[w]);
+            s_piv = (v > 0.0) ? idx : -1;
+        }
+        __syncthreads();
+        const int p = s_piv;
+        if (p < 0) { singular = true; __syncthreads(); continue; }
+        if (p != j) {
+            ++swaps;
+            for (int c = j + tid; c < N; c += nth) {
+                const double2 t = A[j + (int64_t)N * c];
+                A[j + (int64_t)N * c] = A[p + (int64_t)N * c];
+                A[p + (int64_t)N * c] = t;
+            }
+        }
+        __syncthreads();
+        const double2 piv = A[j + (int64_t)N * j];
+        det = cmul(det, piv);
+        const double2 rp = d_inv(piv.x, piv.y);
+        for (int r = j + 1 + tid; r < N; r += nth) {
+            const double2 l = cmul(A[r + (int64_t)N * j], rp);
+            A[r + (int64_t)N * j] = l;
+            colj[r] = l;
+        }
+        for (int c = j + 1 + tid; c < N; c += nth) rowj[c] = A[j + (int64_t)N * c];
+        __syncthreads();
+        const int n = N - j - 1;
+        const int64_t nn = (int64_t)n * n;
+        for (int64_t x = tid; x < nn; x += nth) {
+            const int r = j + 1 + (int)(x % n), c = j + 1 + (int)(x / n);
+            const double2 lu = cmul(colj[r], rowj[c]);
+            double2* a = A + r + (int64_t)N * c;
+            const double2 av = *a;
+            *a = make_double2(av.x - lu.x, av.y - lu.y);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        if (singular) det = make_double2(0.0, 0.0);
+        else if (swaps & 1) det = make_double2(-det.x, -det.y);
+        det_out[b] = det;
+    }
+}
+
+// =====================================================================
+// Host side: context, buffers, drivers
+// =====================================================================
+struct klsp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int n_cu = 256;
+    // problem
+    bool have_problem = false;
+    int64_t NPh = 0, N = 0, n_l = 0, K = 0, Kp = 0;
+    int Np = 0, m = 0;
+    double G = 0.0;
+    double *d_DJ = nullptr, *d_F0l = nullptr, *d_Ores = nullptr, *d_Nk = nullptr;
+    // W
+    double2* d_W64 = nullptr;
+    float2* d_W32 = nullptr;
+    bool have_W = false;
+    // workspaces
+    double* d_om = nullptr;
+    double2* d_c = nullptr;   size_t c_cap = 0;
+    double2* d_P = nullptr;   size_t P_cap = 0;
+    double2* d_M = nullptr;   size_t M_cap = 0;
+    double2* d_A = nullptr;   size_t A_cap = 0;
+    double2* d_det = nullptr;
+    // profiling
+    bool prof = false;
+    hipEvent_t ev[6] = {};
+    klsp_timing last{};
+    double last_W_ms = 0.0;
+};
+
+static const char* g_noctx_err = "klsp: null context";
+
+#define KLSP_HIP(ctx, call)                                                                 \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            (ctx)->err = std::string("HIP error in ") + #call + ": " + hipGetErrorString(e_); \
+            return e_ == hipErrorOutOfMemory ? KLSP_ERR_OOM : KLSP_ERR_HIP;                 \
+        }                                                                                   \
+    } while (0)
+
+static int fail(klsp_ctx* ctx, int code, const std::string& msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+// Grow-only device workspace.
+template <typename T>
+static int ensure(klsp_ctx* ctx, T*& ptr, size_t& cap, size_t n)
+{
+    if (n <= cap) return KLSP_OK;
+    if (ptr) KLSP_HIP(ctx, hipFree(ptr));
+    ptr = nullptr;
+    cap = 0;
+    KLSP_HIP(ctx, hipMalloc((void**)&ptr, n * sizeof(T)));
+    cap = n;
+    return KLSP_OK;
+}
+
+static int grid_for(int64_t n, int block = 256)
+{
+    const int64_t g = (n + block - 1) / block;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, 65536));
+}
+
+// K-split count and K per split for the contraction.
+struct SplitPlan {
+    int tilesN, T, S;
+    int64_t Ks;
+};
+
+static SplitPlan plan_splits(const klsp_ctx* ctx)
+{
+    SplitPlan sp;
+    sp.tilesN = (int)((ctx->Np + kTile - 1) / kTile);
+    sp.T = sp.tilesN * sp.tilesN;
+    const int64_t target_wg = (int64_t)ctx->n_cu * 8;
+    int64_t S = (target_wg + sp.T - 1) / sp.T;
+    // at least kF32Tile K per split
+    S = std::min<int64_t>(S, std::max<int64_t>(1, ctx->Kp / kF32Tile));
+    S = round_up(std::max<int64_t>(S, 1), 8);
+    sp.Ks = round_up((ctx->Kp + S - 1) / S, kF32Tile);
+    // drop splits that would be empty
+    const int64_t used = (ctx->Kp + sp.Ks - 1) / sp.Ks;
+    sp.S = (int)round_up(std::max<int64_t>(used, 1), 8);
+    return sp;
+}
+
+static int free_problem(klsp_ctx* ctx)
+{
+    hipFree(ctx->d_DJ); hipFree(ctx->d_F0l); hipFree(ctx->d_Ores); hipFree(ctx->d_Nk);
+    hipFree(ctx->d_W64); hipFree(ctx->d_W32);
+    ctx->d_DJ = ctx->d_F0l = ctx->d_Ores = ctx->d_Nk = nullptr;
+    ctx->d_W64 = nullptr;
+    ctx->d_W32 = nullptr;
+    ctx->have_problem = ctx->have_W = false;
+    return KLSP_OK;
+}
+
+extern "C" {
+
+const char* klsp_version(void) { return KLSP_VERSION_STRING; }
+
+int klsp_device_count(int* n_out)
+{
+    if (!n_out) return KLSP_ERR_INVALID_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *n_out = n;
+    return KLSP_OK;
+}
+
+int klsp_create(int device, klsp_ctx** ctx_out)
+{
+    if (!ctx_out) return KLSP_ERR_INVALID_ARG;
+    *ctx_out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return KLSP_ERR_NO_DEVICE;
+    if (device < 0 || device >= n) return KLSP_ERR_INVALID_ARG;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return KLSP_ERR_HIP;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return KLSP_ERR_NO_DEVICE;
+    klsp_ctx* ctx = new klsp_ctx();
+    ctx->device = device;
+    ctx->n_cu = prop.multiProcessorCount;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
+        delete ctx;
+        return KLSP_ERR_HIP;
+    }
+    for (auto& e : ctx->ev) hipEventCreate(&e);
+    if (hipMalloc((void**)&ctx->d_om, sizeof(double) * 2 * kMaxBatch) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_det, sizeof(double2) * kMaxBatch) != hipSuccess) {
+        delete ctx;
+        return KLSP_ERR_OOM;
+    }
+    *ctx_out = ctx;
+    return KLSP_OK;
+}
+
+int klsp_destroy(klsp_ctx* ctx)
+{
+    if (!ctx) return KLSP_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    free_problem(ctx);
+    hipFree(ctx->d_om); hipFree(ctx->d_c); hipFree(ctx->d_P); hipFree(ctx->d_M);
+    hipFree(ctx->d_A); hipFree(ctx->d_det);
+    for (auto& e : ctx->ev) hipEventDestroy(e);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return KLSP_OK;
+}
+
+const char* klsp_last_error(const klsp_ctx* ctx) { return ctx ? ctx->err.c_str() : g_noctx_err; }
+
+int klsp_upload_problem(klsp_ctx* ctx, const klsp_problem* pr)
+{
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    if (!pr || pr->NPh < 1 || pr->N_alpha < 1 || pr->n_l < 1 || !pr->DJ || !pr->F0l ||
+        !pr->Omega_res || !pr->N_kernel)
+        return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_upload_problem: bad problem description");
+    if (pr->N_alpha > kMaxN)
+        return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_upload_problem: N_alpha exceeds 1024");
+    KLSP_HIP(ctx, hipSetDevice(ctx->device));
+    KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    free_problem(ctx);
+    ctx->NPh = pr->NPh;
+    ctx->N = pr->N_alpha;
+    ctx->n_l = pr->n_l;
+    ctx->K = pr->NPh * pr->n_l;
+    ctx->Kp = round_up(ctx->K, kKStep);
+    ctx->Np = (int)round_up(ctx->N, kColPad);
+    ctx->m = pr->m;
+    ctx->G = pr->G;
+    const size_t nK = (size_t)ctx->K;
+    KLSP_HIP(ctx, hipMalloc((void**)&ctx->d_DJ, sizeof(double) * ctx->NPh));
+    KLSP_HIP(ctx, hipMalloc((void**)&ctx->d_F0l, sizeof(double) * nK));
+    KLSP_HIP(ctx, hipMalloc((void**)&ctx->d_Ores, sizeof(double) * nK));
+    KLSP_HIP(ctx, hipMalloc((void**)&ctx->d_Nk, sizeof(double) * ctx->N));
+    KLSP_HIP(ctx, hipMemcpy(ctx->d_DJ, pr->DJ, sizeof(double) * ctx->NPh, hipMemcpyHostToDevice));
+    KLSP_HIP(ctx, hipMemcpy(ctx->d_F0l, pr->F0l, sizeof(double) * nK, hipMemcpyHostToDevice));
+    KLSP_HIP(ctx, hipMemcpy(ctx->d_Ores, pr->Omega_res, sizeof(double) * nK, hipMemcpyHostToDevice));
+    KLSP_HIP(ctx, hipMemcpy(ctx->d_Nk, pr->N_kernel, sizeof(double) * ctx->N, hipMemcpyHostToDevice));
+    ctx->have_problem = true;
+    return KLSP_OK;
+}
+
+static int alloc_W64(klsp_ctx* ctx)
+{
+    if (ctx->d_W32) { KLSP_HIP(ctx, hipFree(ctx->d_W32)); ctx->d_W32 = nullptr; }
+    if (!ctx->d_W64)
+        KLSP_HIP(ctx, hipMalloc((void**)&ctx->d_W64, sizeof(double2) * ctx->Kp * ctx->Np));
+    KLSP_HIP(ctx, hipMemsetAsync(ctx->d_W64, 0, sizeof(double2) * ctx->Kp * ctx->Np, ctx->stream));
+    return KLSP_OK;
+}
+
+int klsp_compute_W(klsp_ctx* ctx, int64_t nwa, const double* w1, const double* ra,
+                   const double* pha, const double* Om1, const double* Om2,
+                   const double* alpha, const int32_t* l_arr, double* W_out)
+{
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    if (!ctx->have_problem)
+        return fail(ctx, KLSP_ERR_NOT_READY, "klsp_compute_W: upload the problem first");
+    if (nwa < 2 || !w1 || !ra || !pha || !Om1 || !Om2 || !alpha || !l_arr)
+        return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_compute_W: bad arguments");
+    if (ctx->n_l > 16 * 4 * kWMaxLB)
+        return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_compute_W: n_l exceeds 256");
+    KLSP_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t NPh = ctx->NPh, nwa_p = round_up(nwa, 4);
+    const size_t no = (size_t)NPh * nwa, np = (size_t)NPh * nwa_p;
+    double *d_w1 = nullptr, *d_ra = nullptr, *d_pha = nullptr, *d_o1 = nullptr, *d_o2 = nullptr;
+    double *d_u = nullptr, *d_wt = nullptr, *d_x1 = nullptr, *d_mp = nullptr, *d_al = nullptr;
+    int *d_valid = nullptr, *d_l = nullptr;
+    auto cleanup = [&]() {
+        hipFree(d_w1); hipFree(d_ra); hipFree(d_pha); hipFree(d_o1); hipFree(d_o2);
+        hipFree(d_u); hipFree(d_wt); hipFree(d_x1); hipFree(d_mp); hipFree(d_al);
+        hipFree(d_valid); hipFree(d_l);
+    };
+    hipError_t e = hipSuccess;
+#define KLSP_TRY(call) if (e == hipSuccess) e = (call)
+    KLSP_TRY(hipMalloc((void**)&d_w1, no * 8));
+    KLSP_TRY(hipMalloc((void**)&d_ra, no * 8));
+    KLSP_TRY(hipMalloc((void**)&d_pha, no * 8));
+    KLSP_TRY(hipMalloc((void**)&d_o1, NPh * 8));
+    KLSP_TRY(hipMalloc((void**)&d_o2, NPh * 8));
+    KLSP_TRY(hipMalloc((void**)&d_u, np * 8));
+    KLSP_TRY(hipMalloc((void**)&d_wt, np * 8));
+    KLSP_TRY(hipMalloc((void**)&d_x1, np * 8));
+    KLSP_TRY(hipMalloc((void**)&d_mp, np * 8));
+    KLSP_TRY(hipMalloc((void**)&d_al, ctx->N * 8));
+    KLSP_TRY(hipMalloc((void**)&d_valid, NPh * 4));
+    KLSP_TRY(hipMalloc((void**)&d_l, ctx->n_l * 4));
+    KLSP_TRY(hipMemcpy(d_w1, w1, no * 8, hipMemcpyHostToDevice));
+    KLSP_TRY(hipMemcpy(d_ra, ra, no * 8, hipMemcpyHostToDevice));
+    KLSP_TRY(hipMemcpy(d_pha, pha, no * 8, hipMemcpyHostToDevice));
+    KLSP_TRY(hipMemcpy(d_o1, Om1, NPh * 8, hipMemcpyHostToDevice));
+    KLSP_TRY(hipMemcpy(d_o2, Om2, NPh * 8, hipMemcpyHostToDevice));
+    KLSP_TRY(hipMemcpy(d_al, alpha, ctx->N * 8, hipMemcpyHostToDevice));
+    KLSP_TRY(hipMemcpy(d_l, l_arr, ctx->n_l * 4, hipMemcpyHostToDevice));
+#undef KLSP_TRY
+    if (e != hipSuccess) {
+        cleanup();
+        return fail(ctx, e == hipErrorOutOfMemory ? KLSP_ERR_OOM : KLSP_ERR_HIP,
+                    std::string("klsp_compute_W: ") + hipGetErrorString(e));
+    }
+    int rc = alloc_W64(ctx);
+    if (rc != KLSP_OK) { cleanup(); return rc; }
+    hipEventRecord(ctx->ev[0], ctx->stream);
+    hipLaunchKernelGGL(k_orbit_prep, dim3((unsigned)NPh), dim3(256), 0, ctx->stream, d_w1, d_ra,
+                       d_pha, d_o1, d_o2, (int)nwa, (int)nwa_p, ctx->m, d_u, d_wt, d_x1, d_mp,
+                       d_valid);
+    hipLaunchKernelGGL(k_W_f64, dim3((unsigned)NPh, (unsigned)((ctx->N + 31) / 32)), dim3(256), 0,
+                       ctx->stream, d_u, d_wt, d_x1, d_mp, d_valid, d_al, d_l, NPh, (int)ctx->N,
+                       ctx->Np, (int)ctx->n_l, (int)nwa_p, ctx->d_W64);
+    hipEventRecord(ctx->ev[1], ctx->stream);
+    e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    ctx->last_W_ms = ms;
+    cleanup();
+    if (e != hipSuccess)
+        return fail(ctx, KLSP_ERR_HIP, std::string("klsp_compute_W: ") + hipGetErrorString(e));
+    ctx->have_W = true;
+    if (W_out) return klsp_download_W(ctx, W_out);
+    return KLSP_OK;
+}
+
+int klsp_upload_W(klsp_ctx* ctx, const double* W)
+{
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    if (!ctx->have_problem)
+        return fail(ctx, KLSP_ERR_NOT_READY, "klsp_upload_W: upload the problem first");
+    if (!W) return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_upload_W: null W");
+    KLSP_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = alloc_W64(ctx);
+    if (rc) return rc;
+    const int64_t slab = ctx->NPh * ctx->N;
+    double2* d_slab = nullptr;
+    KLSP_HIP(ctx, hipMalloc((void**)&d_slab, sizeof(double2) * slab));
+    for (int64_t li = 0; li < ctx->n_l; ++li) {
+        hipError_t e = hipMemcpyAsync(d_slab, W + 2 * slab * li, sizeof(double2) * slab,
+                                      hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess) { hipFree(d_slab); KLSP_HIP(ctx, e); }
+        hipLaunchKernelGGL(k_W_ref_to_slab, dim3(grid_for(slab)), dim3(256), 0, ctx->stream,
+                           d_slab, ctx->NPh, (int)ctx->N, ctx->Np, (int)li, ctx->d_W64);
+    }
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    hipFree(d_slab);
+    KLSP_HIP(ctx, e);
+    ctx->have_W = true;
+    return KLSP_OK;
+}
+
+int klsp_download_W(klsp_ctx* ctx, double* W_out)
+{
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    if (!ctx->have_W) return fail(ctx, KLSP_ERR_NOT_READY, "klsp_download_W: no W on device");
+    if (!W_out) return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_download_W: null output");
+    KLSP_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t slab = ctx->NPh * ctx->N;
+    double2* d_slab = nullptr;
+    KLSP_HIP(ctx, hipMalloc((void**)&d_slab, sizeof(double2) * slab));
+    for (int64_t li = 0; li < ctx->n_l; ++li) {
+        hipLaunchKernelGGL(k_W_slab_to_ref, dim3(grid_for(slab)), dim3(256), 0, ctx->stream,
+                           ctx->d_W64, ctx->NPh, (int)ctx->N, ctx->Np, (int)li, d_slab);
+        hipError_t e = hipMemcpyAsync(W_out + 2 * slab * li, d_slab, sizeof(double2) * slab,
+                                      hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) { hipFree(d_slab); KLSP_HIP(ctx, e); }
+    }
+    hipFree(d_slab);
+    return KLSP_OK;
+}
+
+int klsp_prepare(klsp_ctx* ctx, int precision)
+{
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    if (precision != KLSP_PREC_FP64 && precision != KLSP_PREC_FP32)
+        return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_prepare: unknown precision");
+    if (!ctx->have_W) return fail(ctx, KLSP_ERR_NOT_READY, "klsp_prepare: no W on device");
+    if (precision == KLSP_PREC_FP32 && !ctx->d_W32) {
+        KLSP_HIP(ctx, hipSetDevice(ctx->device));
+        const int64_t n = ctx->Kp * ctx->Np;
+        KLSP_HIP(ctx, hipMalloc((void**)&ctx->d_W32, sizeof(float2) * n));
+        hipLaunchKernelGGL(k_W64_to_32, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
+                           ctx->d_W64, n, ctx->d_W32);
+        KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return KLSP_OK;
+}
+
+}  // extern "C"
+
+// Builds M for nb <= kMaxBatch omegas into ctx->d_M ([nb][N,N]).
+static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision)
+{
+    const int64_t Kp = ctx->Kp;
+    const SplitPlan sp = plan_splits(ctx);
+    int rc;
+    if ((rc = ensure(ctx, ctx->d_c, ctx->c_cap, (size_t)kMaxBatch * Kp))) return rc;
+    if ((rc = ensure(ctx, ctx->d_P, ctx->P_cap, (size_t)kMaxBatch * sp.S * ctx->Np * ctx->Np)))
+        return rc;
+    if ((rc = ensure(ctx, ctx->d_M, ctx->M_cap, (size_t)kMaxBatch * ctx->N * ctx->N))) return rc;
+    KLSP_HIP(ctx, hipMemcpyAsync(ctx->d_om, om, sizeof(double) * 2 * nb, hipMemcpyHostToDevice,
+                                 ctx->stream));
+    if (ctx->prof) hipEventRecord(ctx->ev[0], ctx->stream);
+    hipLaunchKernelGGL(k_weights, dim3(grid_for((int64_t)nb * Kp)), dim3(256), 0, ctx->stream,
+                       ctx->d_DJ, ctx->d_F0l, ctx->d_Ores, ctx->NPh, ctx->K, Kp, ctx->d_om, nb,
+                       ctx->d_c);
+    if (ctx->prof) hipEventRecord(ctx->ev[1], ctx->stream);
+    const dim3 grid((unsigned)(sp.T * sp.S)), block(256);
+    if (precision == KLSP_PREC_FP64) {
+        if (nb == 1)
+            hipLaunchKernelGGL(k_contract_f64<1>, grid, block, 0, ctx->stream, ctx->d_W64, ctx->d_c,
+                               ctx->d_P, Kp, ctx->Np, sp.Ks, sp.T, sp.tilesN, sp.S);
+        else if (nb == 2)
+            hipLaunchKernelGGL(k_contract_f64<2>, grid, block, 0, ctx->stream, ctx->d_W64, ctx->d_c,
+                               ctx->d_P, Kp, ctx->Np, sp.Ks, sp.T, sp.tilesN, sp.S);
+        else
+            hipLaunchKernelGGL(k_contract_f64<3>, grid, block, 0, ctx->stream, ctx->d_W64, ctx->d_c,
+                               ctx->d_P, Kp, ctx->Np, sp.Ks, sp.T, sp.tilesN, sp.S);
+    } else {
+        if (nb == 1)
+            hipLaunchKernelGGL(k_contract_f32<1>, grid, block, 0, ctx->stream, ctx->d_W32, ctx->d_c,
+                               ctx->d_P, Kp, ctx->Np, sp.Ks, sp.T, sp.tilesN, sp.S);
+        else
+            hipLaunchKernelGGL(k_contract_f32<2>, grid, block, 0, ctx->stream, ctx->d_W32, ctx->d_c,
+                               ctx->d_P, Kp, ctx->Np, sp.Ks, sp.T, sp.tilesN, sp.S);
+    }
+    if (ctx->prof) hipEventRecord(ctx->ev[2], ctx->stream);
+    hipLaunchKernelGGL(k_reduce, dim3(grid_for((int64_t)nb * ctx->N * ctx->N)), dim3(256), 0,
+                       ctx->stream, ctx->d_P, sp.S, ctx->Np, (int)ctx->N, nb, ctx->G, ctx->d_Nk,
+                       ctx->d_M);
+    if (ctx->prof) hipEventRecord(ctx->ev[3], ctx->stream);
+    KLSP_HIP(ctx, hipGetLastError());
+    return KLSP_OK;
+}
+
+static void add_timing(klsp_ctx* ctx, bool with_lu, int nb)
+{
+    if (!ctx->prof) return;
+    hipEventSynchronize(with_lu ? ctx->ev[4] : ctx->ev[3]);
+    float a = 0, b = 0, c = 0, d = 0;
+    hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
+    hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]);
+    hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]);
+    if (with_lu) hipEventElapsedTime(&d, ctx->ev[3], ctx->ev[4]);
+    ctx->last.weights_ms += a;
+    ctx->last.contract_ms += b;
+    ctx->last.reduce_ms += c;
+    ctx->last.lu_ms += d;
+    ctx->last.n_omega += nb;
+    ctx->last.contract_launches += 1;
+}
+
+static int check_ready(klsp_ctx* ctx, int precision, const char* who)
+{
+    if (!ctx->have_problem || !ctx->have_W)
+        return fail(ctx, KLSP_ERR_NOT_READY, std::string(who) + ": problem and W must be on device");
+    if (precision != KLSP_PREC_FP64 && precision != KLSP_PREC_FP32)
+        return fail(ctx, KLSP_ERR_INVALID_ARG, std::string(who) + ": unknown precision");
+    if (precision == KLSP_PREC_FP32 && !ctx->d_W32) return klsp_prepare(ctx, precision);
+    return KLSP_OK;
+}
+
+// det(I - M(omega_i)) for n omegas; the workhorse of klsp_det/newton/scan.
+static int det_many(klsp_ctx* ctx, const double* om, int64_t n, int precision, double* out)
+{
+    int rc = check_ready(ctx, precision, "klsp_det");
+    if (rc) return rc;
+    KLSP_HIP(ctx, hipSetDevice(ctx->device));
+    if ((rc = ensure(ctx, ctx->d_A, ctx->A_cap, (size_t)kMaxBatch * ctx->N * ctx->N))) return rc;
+    const int maxb = precision == KLSP_PREC_FP64 ? 3 : 2;
+    for (int64_t i0 = 0; i0 < n; i0 += maxb) {
+        const int nb = (int)std::min<int64_t>(maxb, n - i0);
+        if ((rc = build_batch(ctx, om + 2 * i0, nb, precision))) return rc;
+        hipLaunchKernelGGL(k_lu_det, dim3(nb), dim3(1024), 0, ctx->stream, ctx->d_M, ctx->d_A,
+                           (int)ctx->N, ctx->d_det);
+        if (ctx->prof) hipEventRecord(ctx->ev[4], ctx->stream);
+        KLSP_HIP(ctx, hipGetLastError());
+        KLSP_HIP(ctx, hipMemcpyAsync(out + 2 * i0, ctx->d_det, sizeof(double2) * nb,
+                                     hipMemcpyDeviceToHost, ctx->stream));
+        KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        add_timing(ctx, true, nb);
+    }
+    return KLSP_OK;
+}
+
+static void reset_timing(klsp_ctx* ctx) { ctx->last = klsp_timing{}; }
+
+extern "C" {
+
+int klsp_build_M(klsp_ctx* ctx, double om_re, double om_im, int precision, double* M_out)
+{
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    if (!M_out) return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_build_M: null output");
+    int rc = check_ready(ctx, precision, "klsp_build_M");
+    if (rc) return rc;
+    KLSP_HIP(ctx, hipSetDevice(ctx->device));
+    reset_timing(ctx);
+    const double om[2] = {om_re, om_im};
+    if ((rc = build_batch(ctx, om, 1, precision))) return rc;
+    KLSP_HIP(ctx, hipMemcpyAsync(M_out, ctx->d_M, sizeof(double2) * ctx->N * ctx->N,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    add_timing(ctx, false, 1);
+    return KLSP_OK;
+}
+
+int klsp_det(klsp_ctx* ctx, const double* omegas, int64_t n, int precision, double* dets_out)
+{
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    if (n < 0 || (n > 0 && (!omegas || !dets_out)))
+        return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_det: bad arguments");
+    reset_timing(ctx);
+    if (n == 0) return KLSP_OK;
+    return det_many(ctx, omegas, n, precision, dets_out);
+}
+
+// newton_search, NewtonSolver.jl:36-172, with the forward-difference pair
+// (omega + m*delta, omega + i*delta) evaluated as one two-omega batch.
+int klsp_newton(klsp_ctx* ctx, const klsp_newton_params* np, klsp_newton_result* res,
+                double* history)
+{
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    if (!np || !res || np->max_iter < 0 || np->max_line_search < 0)
+        return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_newton: bad arguments");
+    int rc = check_ready(ctx, np->precision, "klsp_newton");
+    if (rc) return rc;
+    reset_timing(ctx);
+    const double m = (double)ctx->m;
+    const int prec = np->precision;
+    double Op = np->Omega_p_init, gv = np->gamma_init;
+    const double tol = np->tol, delta = np->delta;
+    int conv = 0, iter_done = 0, n_eval = 0;
+    klsp_timing acc{};
+    auto det1 = [&](double wr, double wi, double* d) -> int {
+        const double om[2] = {wr, wi};
+        ++n_eval;
+        int r = det_many(ctx, om, 1, prec, d);
+        acc.weights_ms += ctx->last.weights_ms; acc.contract_ms += ctx->last.contract_ms;
+        acc.reduce_ms += ctx->last.reduce_ms; acc.lu_ms += ctx->last.lu_ms;
+        acc.n_omega += ctx->last.n_omega; acc.contract_launches += ctx->last.contract_launches;
+        ctx->last = klsp_timing{};
+        return r;
+    };
+    auto det2 = [&](const double* om4, double* d4) -> int {
+        n_eval += 2;
+        int r = det_many(ctx, om4, 2, prec, d4);
+        acc.weights_ms += ctx->last.weights_ms; acc.contract_ms += ctx->last.contract_ms;
+        acc.reduce_ms += ctx->last.reduce_ms; acc.lu_ms += ctx->last.lu_ms;
+        acc.n_omega += ctx->last.n_omega; acc.contract_launches += ctx->last.contract_launches;
+        ctx->last = klsp_timing{};
+        return r;
+    };
+    for (int iter = 1; iter <= np->max_iter; ++iter) {
+        iter_done = iter;
+        double d[2];
+        if ((rc = det1(m * Op, gv, d))) return rc;
+        const double res_norm = std::hypot(d[0], d[1]);
+        if (history) {
+            history[iter - 1] = Op;
+            history[iter - 1 + np->max_iter] = gv;
+            history[iter - 1 + 2 * np->max_iter] = std::hypot(d[0], d[1]);
+        }
+        if (res_norm < tol) { conv = 1; break; }
+        const double om4[4] = {m * (Op + delta), gv, m * Op, gv + delta};
+        double d4[4];
+        if ((rc = det2(om4, d4))) return rc;
+        const double J11 = (d4[0] - d[0]) / delta, J12 = (d4[2] - d[0]) / delta;
+        const double J21 = (d4[1] - d[1]) / delta, J22 = (d4[3] - d[1]) / delta;
+        // delta_x = -J \ F by 2x2 LU with partial pivoting
+        double a11 = J11, a12 = J12, a21 = J21, a22 = J22, b1 = d[0], b2 = d[1];
+        if (std::fabs(a21) > std::fabs(a11)) {
+            std::swap(a11, a21); std::swap(a12, a22); std::swap(b1, b2);
+        }
+        const double l21 = a21 * (1.0 / a11);
+        const double x2 = (b2 - l21 * b1) / (a22 - l21 * a12);
+        const double x1 = (b1 - a12 * x2) / a11;
+        const double dx1 = -x1, dx2 = -x2;
+        if (np->use_line_search) {
+            double al = 1.0;
+            for (int ls = 1; ls <= np->max_line_search; ++ls) {
+                const double Opn = Op + al * dx1, gn = gv + al * dx2;
+                if (gn < 1e-4 || Opn < 0.0 || Opn > 1.0) { al *= 0.5; continue; }
+                double dn[2];
+                if ((rc = det1(m * Opn, gn, dn))) return rc;
+                if (std::hypot(dn[0], dn[1]) < res_norm) { Op = Opn; gv = gn; break; }
+                al *= 0.5;
+                if (ls == np->max_line_search) { Op += dx1; gv += dx2; }
+            }
+        } else {
+            Op += dx1;
+            gv += dx2;
+        }
+        if (!std::isnan(gv) && gv < 1e-4) gv = 1e-4;
+        if (Op > 1.0) Op = 1.0; else if (Op < 0.0) Op = 0.0;
+    }
+    double df[2];
+    if ((rc = det1(m * Op, gv, df))) return rc;
+    res->Omega_p = Op;
+    res->gamma = gv;
+    res->omega[0] = m * Op;
+    res->omega[1] = gv;
+    res->det[0] = df[0];
+    res->det[1] = df[1];
+    res->iterations = iter_done;
+    res->converged = conv;
+    res->n_det_evals = n_eval;
+    ctx->last = acc;
+    return KLSP_OK;
+}
+
+int klsp_grid_scan(klsp_ctx* ctx, const klsp_scan_params* sp, double* det_grid,
+                   double* Op_init, double* g_init)
+{
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    if (!sp || sp->n_Op < 1 || sp->n_gamma < 1 || !det_grid)
+        return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_grid_scan: bad arguments");
+    int rc = check_ready(ctx, sp->precision, "klsp_grid_scan");
+    if (rc) return rc;
+    reset_timing(ctx);
+    std::vector<double> Opg(sp->n_Op), gg(sp->n_gamma);
+    klsp_linrange(sp->Omega_p_min, sp->Omega_p_max, sp->n_Op, Opg.data());
+    klsp_linrange(sp->gamma_min, sp->gamma_max, sp->n_gamma, gg.data());
+    const int64_t n = (int64_t)sp->n_Op * sp->n_gamma;
+    std::vector<double> om(2 * n);
+    for (int j = 0; j < sp->n_Op; ++j)
+        for (int i = 0; i < sp->n_gamma; ++i) {
+            const int64_t x = i + (int64_t)sp->n_gamma * j;
+            om[2 * x] = ctx->m * Opg[j];
+            om[2 * x + 1] = gg[i];
+        }
+    if ((rc = det_many(ctx, om.data(), n, sp->precision, det_grid))) return rc;
+    int64_t best = 0;
+    double bv = INFINITY;
+    for (int64_t x = 0; x < n; ++x) {
+        const double v = std::hypot(det_grid[2 * x], det_grid[2 * x + 1]);
+        if (v < bv) { bv = v; best = x; }
+    }
+    if (Op_init) *Op_init = Opg[best / sp->n_gamma];
+    if (g_init) *g_init = gg[best % sp->n_gamma];
+    return KLSP_OK;
+}
+
+int klsp_set_profiling(klsp_ctx* ctx, int on)
+{
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    ctx->prof = on != 0;
+    return KLSP_OK;
+}
+
+int klsp_last_timing(klsp_ctx* ctx, klsp_timing* t)
+{
+    if (!ctx || !t) return KLSP_ERR_INVALID_ARG;
+    *t = ctx->last;
+    return KLSP_OK;
+}
+
+int klsp_last_W_ms(klsp_ctx* ctx, double* ms)
+{
+    if (!ctx || !ms) return KLSP_ERR_INVALID_ARG;
+    *ms = ctx->last_W_ms;
+    return KLSP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,225 @@
+"""GPU parity: libklsp (HIP, gfx950) against the FP64 CPU oracle.
+
+Tolerances (FP64 path): W and M normwise relative <= 1e-12, det relative
+<= 1e-11, Newton roots |d omega| <= 1e-10.  The FP32 path is checked against
+the FP64 device result.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import kalnajs_log_spirals_amd as K
+from kalnajs_log_spirals_amd import _ffi
+from kalnajs_log_spirals_amd.BasisFunctions import from_arrays
+
+pytestmark = pytest.mark.gpu
+
+
+def cfg_from_params(p) -> K.KalnajsConfig:
+    c = K.KalnajsConfig()
+    c.physics.m, c.physics.G = p.m, p.G
+    c.model.L0, c.model.n_zang, c.model.q1 = p.L0, p.n_zang, p.q1
+    for k in ("NR", "Ne", "Rc_min", "Rc_max", "v_min", "N_alpha", "alpha_max", "l_min", "l_max",
+              "nw", "nwa"):
+        setattr(c.grid, k, getattr(p, k))
+    c.gpu.precision_double = c.cpu.precision_double = True
+    return c
+
+
+def relnorm(a, b) -> float:
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def device_pre(pre_oracle):
+    return from_arrays(pre_oracle["m"], pre_oracle["G"], pre_oracle["W"], pre_oracle["DJ"],
+                       pre_oracle["F0l"], pre_oracle["Omega_res"], pre_oracle["N_kernel"])
+
+
+OMEGAS = [2 * 0.4397142267 + 0.1230663944j, 0.6 + 0.05j, 1.1 + 0.2j, 0.9 - 0.1j]
+
+
+def test_library_reports_gfx950():
+    print("library:", _ffi.version(), "devices:", _ffi.device_count(), flush=True)
+    assert _ffi.device_count() >= 1
+    assert K.detect_gpu_type() == K.GPUType.AMD
+
+
+def test_W_quadrature_matches_oracle(oracle_tiny):
+    """compute_W_l (BasisFunctions.jl:199) on FP64 MFMA vs the direct sum."""
+    p = oracle_tiny["params"]
+    cfg = cfg_from_params(p)
+    pre = K.precompute_all(cfg, K.create_toomre_zang_model(cfg), K.GPUType.AMD)
+    W = pre.W_l_mat
+    assert W.shape == oracle_tiny["W"].shape
+    assert relnorm(W, oracle_tiny["W"]) <= 1e-12
+    np.testing.assert_allclose(pre.DJ_vec, oracle_tiny["DJ"], rtol=1e-13, atol=0)
+    np.testing.assert_allclose(pre.F0l_all, oracle_tiny["F0l"], rtol=1e-13, atol=1e-300)
+    np.testing.assert_allclose(pre.N_kernel, oracle_tiny["N_kernel"], rtol=1e-14)
+    # the standalone entry point returns the same array
+    W2 = K.compute_W_l(cfg, pre.orbit_data, pre.alpha_arr, pre.l_arr, K.GPUType.AMD)
+    assert np.array_equal(W2, W)
+
+
+def test_W_quadrature_small_grid(oracle_small, small_cfg_paths):
+    cfg = K.load_model(K.load_config(small_cfg_paths[0]), small_cfg_paths[1])
+    pre = K.precompute_all(cfg, K.create_toomre_zang_model(cfg), K.GPUType.AMD)
+    W = pre.W_l_mat
+    for il in range(W.shape[2]):
+        assert relnorm(W[:, :, il], oracle_small["W"][:, :, il]) <= 1e-12, il
+
+
+@pytest.mark.parametrize("omega", OMEGAS)
+def test_M_fp64_matches_oracle(oracle_tiny, omega):
+    """build_M_matrix (MatrixBuilder.jl:27) on FP64 MFMA."""
+    pre = device_pre(oracle_tiny)
+    M = K.build_M_matrix(omega, pre)
+    Mo = oracle.build_M(oracle_tiny, omega)
+    assert relnorm(M, Mo) <= 1e-12
+
+
+@pytest.mark.parametrize("omega", OMEGAS)
+def test_det_fp64_matches_oracle_small(oracle_small, omega):
+    pre = device_pre(oracle_small)
+    dev = K.to_device(pre, K.GPUType.AMD, np.float64)
+    d = K.compute_determinant(omega, pre, dev, K.GPUType.AMD, np.float64)
+    do = oracle.compute_det(oracle_small, omega)
+    assert abs(d - do) <= 1e-11 * abs(do)
+    M = K.build_M_matrix(omega, pre)
+    assert relnorm(M, oracle.build_M(oracle_small, omega)) <= 1e-12
+
+
+def test_lu_matches_lapack(oracle_small):
+    """The device LU vs LAPACK zgetrf on the same device-built M."""
+    pre = device_pre(oracle_small)
+    for om in OMEGAS:
+        M = pre.ctx.build_M(om, _ffi.PREC_FP64)
+        d = pre.ctx.det([om], _ffi.PREC_FP64)[0]
+        ref = np.linalg.det(np.eye(M.shape[0]) - M)
+        assert abs(d - ref) <= 1e-12 * abs(ref)
+        assert abs(d - oracle.det_matrix(np.eye(M.shape[0]) - M)) <= 1e-12 * abs(ref)
+
+
+RAGGED = [(1, 1, 1), (3, 5, 2), (13, 16, 1), (13, 17, 3), (7, 37, 2), (5, 64, 2), (11, 65, 3),
+          (4, 130, 1)]
+
+
+@pytest.mark.parametrize("NPh,N,n_l", RAGGED)
+def test_random_problem_ragged_sizes(NPh, N, n_l):
+    """Sizes that are not multiples of the 16x16x4 MFMA tiles or the 64 tile."""
+    rng = np.random.default_rng(1000 * NPh + 10 * N + n_l)
+    W = rng.standard_normal((NPh, N, n_l)) + 1j * rng.standard_normal((NPh, N, n_l))
+    pre_o = dict(m=2, G=0.7, W=W / np.sqrt(NPh * n_l), DJ=rng.uniform(0.1, 1, NPh),
+                 F0l=rng.standard_normal((NPh, n_l)), Omega_res=rng.uniform(-2, 2, (NPh, n_l)),
+                 N_kernel=rng.uniform(0.05, 0.3, N))
+    pre = device_pre(pre_o)
+    for om in (0.3 + 0.4j, -1.2 + 0.05j):
+        assert relnorm(K.build_M_matrix(om, pre), oracle.build_M(pre_o, om)) <= 1e-12
+        d = pre.ctx.det([om], _ffi.PREC_FP64)[0]
+        do = oracle.compute_det(pre_o, om)
+        assert abs(d - do) <= 1e-11 * max(abs(do), 1e-300)
+
+
+def test_resonance_clamp():
+    """|omega - Omega_res| < 1e-12 -> denominator 1e-12 + 1e-12 i (MatrixBuilder.jl:47)."""
+    rng = np.random.default_rng(7)
+    NPh, N, n_l = 6, 9, 2
+    W = (rng.standard_normal((NPh, N, n_l)) + 1j * rng.standard_normal((NPh, N, n_l))) * 1e-7
+    Ores = rng.uniform(0.5, 1.5, (NPh, n_l))
+    om = complex(Ores[2, 1], 0.0)
+    Ores[4, 0] = om.real + 3e-13
+    pre_o = dict(m=2, G=1.0, W=W, DJ=np.ones(NPh), F0l=np.ones((NPh, n_l)), Omega_res=Ores,
+                 N_kernel=np.ones(N))
+    pre = device_pre(pre_o)
+    M = K.build_M_matrix(om, pre)
+    Mo = oracle.build_M(pre_o, om)
+    assert relnorm(M, Mo) <= 1e-12
+    assert np.max(np.abs(M)) > 1e-3  # the clamped terms dominate
+
+
+def test_invalid_orbit_is_skipped():
+    """An orbit with a non-finite sample gives W = 0 (BasisFunctions.jl:224-226)."""
+    p = oracle.Params(NR=5, Ne=5, Rc_min=0.1, Rc_max=10.0, N_alpha=17, alpha_max=5.0,
+                      l_min=-2, l_max=2, nw=201, nwa=21)
+    g = oracle.compute_grids(p)
+    o = oracle.compute_orbits(p, g)
+    o["ra"][7, 3] = np.nan
+    o["pha"][11, 0] = np.inf
+    alpha = oracle.linrange(-5, 5, 17)
+    l_arr = np.arange(-2, 3, dtype=np.int32)
+    Wo = oracle.compute_W(p, o, alpha, l_arr)
+    assert np.all(Wo[7] == 0) and np.all(Wo[11] == 0)
+    ctx = _ffi.Context(0)
+    ctx.upload_problem(p.NPh, 17, 5, p.m, 1.0, np.zeros(p.NPh), np.zeros((p.NPh, 5)),
+                       np.zeros((p.NPh, 5)), np.zeros(17))
+    W = ctx.compute_W(o["w1"], o["ra"], o["pha"], o["Om1"], o["Om2"], alpha, l_arr, download=True)
+    assert np.all(W[7] == 0) and np.all(W[11] == 0)
+    assert relnorm(W, Wo) <= 1e-12
+    ctx.close()
+
+
+def test_conjugate_symmetry_small(oracle_small):
+    """det(I - M(conj w)) = conj det(I - M(w)) while the clamp is inactive."""
+    pre = device_pre(oracle_small)
+    for om in OMEGAS[:3]:
+        d1, d2 = pre.ctx.det([om, np.conj(om)], _ffi.PREC_FP64)
+        assert abs(d2 - np.conj(d1)) <= 1e-11 * abs(d1)
+
+
+def test_batched_equals_single(oracle_small):
+    pre = device_pre(oracle_small)
+    oms = np.array(OMEGAS + [0.7 + 0.3j, 0.8 + 0.11j, 0.95 + 0.02j])
+    db = pre.ctx.det(oms, _ffi.PREC_FP64)
+    for i, om in enumerate(oms):
+        assert abs(pre.ctx.det([om], _ffi.PREC_FP64)[0] - db[i]) <= 1e-13 * abs(db[i])
+
+
+def test_newton_small_grid_known_answer(small_cfg_paths):
+    """Full product pipeline on the published grid: the root of
+    convergence_table.txt:13 (Omega_p = 0.4397142267, gamma = 0.1230663944)."""
+    cfg = K.load_model(K.load_config(small_cfg_paths[0]), small_cfg_paths[1])
+    model = K.create_toomre_zang_model(cfg)
+    pre = K.precompute_all(cfg, model, K.GPUType.AMD)
+    dev = K.to_device(pre, K.GPUType.AMD, np.float64)
+    r = K.newton_search(cfg, pre, dev, K.GPUType.AMD, np.float64, 0.44, 0.125)
+    print("root", r.Omega_p, r.gamma, r.iterations, r.det_val, flush=True)
+    assert r.converged
+    assert abs(r.Omega_p - 0.4397142267) <= 1e-10
+    assert abs(r.gamma - 0.1230663944) <= 1e-10
+    pre_o = oracle.precompute(oracle.load_toml_params(*small_cfg_paths))
+    ro = oracle.newton(pre_o, 0.44, 0.125)
+    assert r.iterations == ro["iterations"]
+    assert abs(complex(2 * r.Omega_p, r.gamma) - ro["omega"]) <= 1e-10
+    np.testing.assert_allclose(r.history[:, :2], ro["history"][:, :2], rtol=0, atol=1e-10)
+
+
+def test_newton_line_search_and_bounds(oracle_small):
+    pre = device_pre(oracle_small)
+    res, H = pre.ctx.newton(0.45, 0.11, max_iter=20, tol=1e-12, use_line_search=True,
+                            max_line_search=10)
+    ro = oracle.newton(oracle_small, 0.45, 0.11, max_iter=20, use_line_search=True)
+    assert res.converged and ro["converged"]
+    assert abs(complex(res.omega[0], res.omega[1]) - ro["omega"]) <= 1e-10
+    assert res.iterations == ro["iterations"]
+
+
+def test_grid_scan_small(oracle_small):
+    pre = device_pre(oracle_small)
+    dg, op0, g0 = pre.ctx.grid_scan(6, 5, 0.40, 0.48, 0.08, 0.16)
+    dgo, opo, go = oracle.grid_scan(oracle_small, 6, 5, 0.40, 0.48, 0.08, 0.16)
+    assert np.max(np.abs(dg - dgo) / np.abs(dgo)) <= 1e-11
+    assert (op0, g0) == (opo, go)
+    assert K.find_minimum(dg, oracle.linrange(0.40, 0.48, 6), oracle.linrange(0.08, 0.16, 5)) \
+        == (op0, g0)
+
+
+def test_fp32_path_close_to_fp64(oracle_small):
+    """complex64 W + FP32 MFMA with FP64 K-tile sums vs the FP64 device path."""
+    pre = device_pre(oracle_small)
+    for om in OMEGAS:
+        M64 = pre.ctx.build_M(om, _ffi.PREC_FP64)
+        M32 = pre.ctx.build_M(om, _ffi.PREC_FP32)
+        assert relnorm(M32, M64) <= 2e-6
+        d64 = pre.ctx.det([om], _ffi.PREC_FP64)[0]
+        d32 = pre.ctx.det([om], _ffi.PREC_FP32)[0]
+        assert abs(d32 - d64) <= 1e-4 * abs(d64) + 1e-9

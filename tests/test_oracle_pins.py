@@ -1,0 +1,331 @@
+"""Pins of the FP64 CPU oracle against what the mathematics fixes.
+
+Each check is chosen so that a plausible slip in the restatement (a dropped
+factor, a wrong sign, a transposed index) fails it: polynomial exactness of
+the quadrature rules, an independent Gamma (mpmath), exact recurrences of
+N_m(alpha), the DF normalisation to the Mestel surface density, turning-point
+identities of (E, L), scipy quadrature of the orbital frequencies, a finite-
+difference Jacobian, the circular-orbit closed form and scale invariance of
+W, an independent BLAS-form build of M, LAPACK/mpmath determinants, and a
+Newton run on a holomorphic function with a known root.
+"""
+import math
+
+import mpmath as mp
+import numpy as np
+import pytest
+from scipy import integrate
+
+import oracle
+from oracle import blas_form
+
+
+# --------------------------------------------------------------------------
+# quadrature weights (BasisFunctions.jl:174, OrbitIntegration.jl:79)
+# --------------------------------------------------------------------------
+def test_trapezoid_exact_for_linear_on_nonuniform_grid():
+    rng = np.random.default_rng(0)
+    x = np.sort(rng.uniform(-2, 3, 37))
+    S = oracle.trapezoid_coef(x)
+    assert S.sum() == pytest.approx(x[-1] - x[0], rel=1e-14)
+    f = 3.0 * x - 1.5
+    exact = 1.5 * (x[-1] ** 2 - x[0] ** 2) - 1.5 * (x[-1] - x[0])
+    assert S @ f == pytest.approx(exact, rel=1e-13)
+    # not exact for a quadratic (so a Simpson rule here would be a mistake)
+    assert abs(S @ x**2 - (x[-1] ** 3 - x[0] ** 3) / 3) > 1e-6
+
+
+def test_simpson_exact_for_cubics_and_fallbacks():
+    x = np.linspace(0.01, 1.0, 21)
+    S = oracle.simpson_coef(x)
+    for p in range(4):
+        assert S @ x**p == pytest.approx((1.0 ** (p + 1) - 0.01 ** (p + 1)) / (p + 1), rel=1e-13)
+    # even n and n < 3 fall back to trapezoid (OrbitIntegration.jl:81-82)
+    for n in (2, 4, 20):
+        y = np.linspace(0, 1, n)
+        assert np.array_equal(oracle.simpson_coef(y), oracle.trapezoid_coef(y))
+
+
+# --------------------------------------------------------------------------
+# Gamma and N_m(alpha) (BasisFunctions.jl:281-366)
+# --------------------------------------------------------------------------
+GAMMA_POINTS = [0.25 + 0j, 1.0 + 0j, 2.0 + 0j, 0.5 + 3j, 1.25 - 7.5j, 2.75 + 15j, 0.1 + 0.01j,
+                -0.3 + 0.5j, -2.7 + 1.1j, 5.5 + 0.2j, 0.75 + 25j]
+
+
+@pytest.mark.parametrize("z", GAMMA_POINTS)
+def test_gamma_complex_against_mpmath(z):
+    mp.mp.dps = 40
+    ref = complex(mp.gamma(mp.mpc(z.real, z.imag)))
+    got = oracle.gamma_complex(z)
+    assert abs(got - ref) <= 2e-14 * abs(ref)
+
+
+def test_N_kernel_identities():
+    mp.mp.dps = 40
+    alpha = oracle.linrange(-30.0, 30.0, 61)
+    da = oracle.trapezoid_coef(alpha)
+    for m in (0, 1, 2, 3):
+        N = oracle.N_kernel(m, alpha, da)
+        assert np.all(N > 0)
+        # independent evaluation |G(z1)|^2/|G(z3)|^2 * pi * dalpha
+        for a, d, n in zip(alpha[::7], da[::7], N[::7]):
+            z1 = mp.mpc((m + 0.5) / 2, a / 2)
+            z3 = mp.mpc((m + 1.5) / 2, a / 2)
+            ref = float(abs(mp.gamma(z1)) ** 2 / abs(mp.gamma(z3)) ** 2 * mp.pi * d)
+            assert n == pytest.approx(ref, rel=1e-13)
+    # exact recurrence from G(z+1) = z G(z): N_{m+2}/N_m = |z1|^2/|z3|^2
+    N0, N2 = oracle.N_kernel(0, alpha, da), oracle.N_kernel(2, alpha, da)
+    z1 = (0.5 + 1j * alpha) / 2
+    z3 = (1.5 + 1j * alpha) / 2
+    np.testing.assert_allclose(N2 / N0, np.abs(z1) ** 2 / np.abs(z3) ** 2, rtol=1e-13)
+    # alpha = 0 with real Gamma, and the large-|alpha| asymptote 2*pi/|alpha|
+    i0 = 30
+    assert N0[i0] / da[i0] == pytest.approx(math.pi * math.gamma(0.25) ** 2 / math.gamma(0.75) ** 2,
+                                            rel=1e-13)
+    big = np.array([150.0, 200.0])   # |Gamma| underflows beyond |alpha| ~ 400, as in the reference
+    Nb = oracle.N_kernel(2, big, np.ones(2))
+    assert np.all(np.abs(Nb * big / (2 * math.pi) - 1.0) <= 4.0 / big**2)
+
+
+# --------------------------------------------------------------------------
+# Model: DF normalisation and derivatives (Models.jl:40-169)
+# --------------------------------------------------------------------------
+def test_df_normalisation_is_mestel_surface_density():
+    """Untapered (n_zang=0) DF integrates over velocities to 1/(2 pi r)."""
+    p = oracle.Params(n_zang=0, q1=7)
+    for r in (0.3, 1.0, 4.0):
+        def f(vphi, vr):
+            E = 0.5 * (vr**2 + vphi**2) + math.log(r)
+            return oracle.df_eval(p, E, r * vphi)[0]
+        s = p.q1 ** -0.5
+        val, _ = integrate.dblquad(f, -12 * s, 12 * s, 0.0, 12 * s, epsabs=0, epsrel=1e-10)
+        assert val == pytest.approx(1 / (2 * math.pi * r), rel=1e-7)
+
+
+def test_df_derivatives_match_finite_differences():
+    p = oracle.Params(n_zang=4, q1=7, L0=1.0)
+    for E, L in ((0.2, 0.7), (-0.5, 1.3), (1.0, 2.5)):
+        F, FE, FL = oracle.df_eval(p, E, L)
+        h = 1e-6
+        dE = (oracle.df_eval(p, E + h, L)[0] - oracle.df_eval(p, E - h, L)[0]) / (2 * h)
+        dL = (oracle.df_eval(p, E, L + h)[0] - oracle.df_eval(p, E, L - h)[0]) / (2 * h)
+        assert FE == pytest.approx(dE, rel=1e-7)
+        assert FL == pytest.approx(dL, rel=1e-7)
+    assert oracle.df_eval(p, 0.0, 1e-13) == (0.0, 0.0, 0.0)
+
+
+# --------------------------------------------------------------------------
+# grids and orbits (OrbitIntegration.jl:102-382)
+# --------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def orbits_small():
+    p = oracle.Params(NR=9, Ne=11, Rc_min=0.05, Rc_max=20.0, v_min=0.01, nw=10001, nwa=201)
+    g = oracle.compute_grids(p)
+    o = oracle.compute_orbits(p, g)
+    return p, g, o
+
+
+def test_energy_and_angular_momentum_turning_points(orbits_small):
+    """E = ln r + L^2/(2 r^2) at both turning points r1, r2."""
+    p, g, _ = orbits_small
+    gen = g["e"] > 1e-10
+    for r in (g["R1"][gen], g["R2"][gen]):
+        np.testing.assert_allclose(np.log(r) + g["L2"][gen] / (2 * r**2), g["E"][gen],
+                                   rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(g["R1"] + g["R2"], 2 * g["Rc"], rtol=1e-15)
+
+
+def test_orbit_frequencies_against_quadrature(orbits_small):
+    """Omega_1 = pi / int dr/v_r, Omega_2 = Omega_1/pi * int L/r^2 dr/v_r (half orbit)."""
+    p, g, o = orbits_small
+    for k in (1, 12, 40, 58, 75, 88):
+        if g["e"][k] < 0.01:
+            continue
+        E, L2, r1, r2 = g["E"][k], g["L2"][k], g["R1"][k], g["R2"][k]
+        # substitute r = rs - drs cos(w) to remove the endpoint singularities
+        rs, drs = (r1 + r2) / 2, (r2 - r1) / 2
+
+        def vr(w):
+            r = rs - drs * math.cos(w)
+            return math.sqrt(max(2 * (E - math.log(r)) - L2 / r**2, 1e-300)), r
+        Tr = integrate.quad(lambda w: drs * math.sin(w) / vr(w)[0], 0, math.pi, epsabs=0,
+                            epsrel=1e-12, limit=200)[0]
+        Ph = integrate.quad(lambda w: drs * math.sin(w) * math.sqrt(L2) / vr(w)[1] ** 2 / vr(w)[0],
+                            0, math.pi, epsabs=0, epsrel=1e-12, limit=200)[0]
+        O1 = math.pi / Tr
+        assert o["Om1"][k] == pytest.approx(O1, rel=2e-5)
+        assert o["Om2"][k] == pytest.approx(O1 * Ph / math.pi, rel=2e-5)
+
+
+def test_circular_limit_and_scale_invariance(orbits_small):
+    p, g, o = orbits_small
+    NR, Ne = p.NR, p.Ne
+    circ = np.abs(g["e"]) < 1e-10
+    assert circ.sum() == NR  # the v = 1 column
+    np.testing.assert_allclose(o["Om1"][circ], math.sqrt(2) / g["Rc"][circ], rtol=1e-15)
+    np.testing.assert_allclose(o["Om2"][circ], 1 / g["Rc"][circ], rtol=1e-15)
+    Rc = g["Rc"].reshape((NR, Ne), order="F")
+    O1 = o["Om1"].reshape((NR, Ne), order="F") * Rc
+    O2 = o["Om2"].reshape((NR, Ne), order="F") * Rc
+    # exact in the mathematics; the discretised orbit integral's own rounding
+    # floor is ~1e-11, up to ~1e-9 for Omega_2 of the v = 0.01 orbits
+    np.testing.assert_allclose(O1, np.broadcast_to(O1[:1], O1.shape), rtol=2e-10)
+    np.testing.assert_allclose(O2, np.broadcast_to(O2[:1], O2.shape), rtol=2e-9)
+    w1 = o["w1"].reshape((Ne, NR, -1))
+    np.testing.assert_allclose(w1, np.broadcast_to(w1[:, :1], w1.shape), rtol=0, atol=1e-9)
+
+
+def test_jacobian_is_dEdL_over_Omega1(orbits_small):
+    """jac = |d(E,L)/d(Rc,v)| / Omega_1 (so DJ ~ dJr dL), by finite differences."""
+    p, g, o = orbits_small
+
+    def EL(Rc, v):
+        e = 1 - abs(v)
+        r1, r2 = Rc * (1 - e), Rc * (1 + e)
+        E = (math.log(r2) * r2**2 - math.log(r1) * r1**2) / (r2**2 - r1**2)
+        return E, math.sqrt(2 * (E - math.log(r2)) * r2**2)
+    for k in (3, 20, 50, 77):
+        Rc, v = g["Rc"][k], g["v"][k]
+        if not (0.02 < v < 0.98):
+            continue
+        hR, hv = 1e-6 * Rc, 1e-6
+        dE_dR = (EL(Rc + hR, v)[0] - EL(Rc - hR, v)[0]) / (2 * hR)
+        dL_dR = (EL(Rc + hR, v)[1] - EL(Rc - hR, v)[1]) / (2 * hR)
+        dE_dv = (EL(Rc, v + hv)[0] - EL(Rc, v - hv)[0]) / (2 * hv)
+        dL_dv = (EL(Rc, v + hv)[1] - EL(Rc, v - hv)[1]) / (2 * hv)
+        J = abs(dE_dR * dL_dv - dE_dv * dL_dR) / o["Om1"][k]
+        assert o["jac"][k] == pytest.approx(J, rel=1e-6)
+
+
+# --------------------------------------------------------------------------
+# W_l(alpha) (BasisFunctions.jl:199-266)
+# --------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def pre_tiny():
+    p = oracle.Params(NR=7, Ne=6, Rc_min=0.05, Rc_max=20.0, v_min=0.01, N_alpha=15,
+                      alpha_max=7.0, l_min=-3, l_max=3, nw=2001, nwa=101)
+    return oracle.precompute(p)
+
+
+def test_W_circular_orbit_closed_form(pre_tiny):
+    """v = 1: W = (1/pi) Rc^(-1/2 - i alpha) sum_t Sw1_t cos(l w_t)."""
+    p = pre_tiny["params"]
+    W, g, o = pre_tiny["W"], pre_tiny["grids"], pre_tiny["orbits"]
+    alpha = pre_tiny["alpha"]
+    circ = np.where(np.abs(g["e"]) < 1e-10)[0]
+    for k in circ:
+        w = o["w1"][k]
+        Sw = oracle.trapezoid_coef(w)
+        Rc = g["Rc"][k]
+        for il, l in enumerate(pre_tiny["l_arr"]):
+            ref = (Sw @ np.cos(l * w)) / math.pi * Rc ** (-0.5 - 1j * alpha)
+            np.testing.assert_allclose(W[k, :, il], ref, rtol=1e-12, atol=1e-14 * Rc**-0.5)
+        # l = 0: the quadrature of 1 over [1e-8, pi - 1e-8]
+        l0 = list(pre_tiny["l_arr"]).index(0)
+        np.testing.assert_allclose(W[k, :, l0] * Rc ** (0.5 + 1j * alpha),
+                                   (math.pi - 2e-8) / math.pi, rtol=1e-13)
+
+
+def test_W_scale_invariance(pre_tiny):
+    """log potential: W(s Rc) = s^(-1/2 - i alpha) W(Rc) at fixed v."""
+    p = pre_tiny["params"]
+    W, g = pre_tiny["W"], pre_tiny["grids"]
+    alpha = pre_tiny["alpha"]
+    NR = p.NR
+    for ie in range(p.Ne):
+        k0 = NR * ie
+        for iR in range(1, NR):
+            k = iR + NR * ie
+            s = g["Rc"][k] / g["Rc"][k0]
+            ref = W[k0] * (s ** (-0.5 - 1j * alpha))[:, None]
+            # exact in the mathematics; the orbit phases carry a ~1e-10 rounding floor
+            assert np.max(np.abs(W[k] - ref)) <= 1e-9 * np.max(np.abs(ref))
+
+
+def test_W_matches_numpy_gemm_form(pre_tiny):
+    p = pre_tiny["params"]
+    o = pre_tiny["orbits"]
+    alpha, l_arr = pre_tiny["alpha"], pre_tiny["l_arr"]
+    for k in (0, 9, 25, 41):
+        w1, r, th = o["w1"][k], o["ra"][k], o["pha"][k]
+        Sw = oracle.trapezoid_coef(w1)
+        phi = o["Om2"][k] / o["Om1"][k] * w1 - th
+        A = Sw * np.cos(np.outer(l_arr, w1) + p.m * phi) / np.sqrt(r)
+        B = np.exp(-1j * np.outer(np.log(r), alpha))
+        ref = (A @ B).T / math.pi
+        assert np.max(np.abs(pre_tiny["W"][k] - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+# --------------------------------------------------------------------------
+# M(omega) and det (MatrixBuilder.jl:27-87)
+# --------------------------------------------------------------------------
+OMS = [0.88 + 0.12j, 0.3 + 0.02j, 1.5 + 0.4j]
+
+
+@pytest.mark.parametrize("om", OMS)
+def test_M_matches_blas_form(pre_tiny, om):
+    M = oracle.build_M(pre_tiny, om)
+    Mb = blas_form.build_M(pre_tiny["W"], pre_tiny["DJ"], pre_tiny["F0l"], pre_tiny["Omega_res"],
+                           pre_tiny["N_kernel"], pre_tiny["G"], om)
+    assert np.max(np.abs(M - Mb)) <= 1e-13 * np.max(np.abs(Mb))
+
+
+def test_M_structure(pre_tiny):
+    """H(w) = M(w) diag(N)^-1 / G satisfies H(conj w) = H(w)^H; M is linear in G."""
+    om = 0.9 + 0.15j
+    M1 = oracle.build_M(pre_tiny, om)
+    M2 = oracle.build_M(pre_tiny, np.conj(om))
+    Nk, G = pre_tiny["N_kernel"], pre_tiny["G"]
+    H1 = M1 / Nk[None, :] / G
+    H2 = M2 / Nk[None, :] / G
+    assert np.max(np.abs(H2 - H1.conj().T)) <= 1e-13 * np.max(np.abs(H1))
+    pre2 = dict(pre_tiny, G=2.5 * G)
+    np.testing.assert_allclose(oracle.build_M(pre2, om), 2.5 * M1, rtol=1e-14)
+    d1 = oracle.compute_det(pre_tiny, om)
+    d2 = oracle.compute_det(pre_tiny, np.conj(om))
+    assert abs(d2 - np.conj(d1)) <= 1e-13 * abs(d1)
+
+
+def test_det_against_lapack_and_mpmath():
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 5, 17, 64):
+        A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        assert abs(oracle.det_matrix(A) - np.linalg.det(A)) <= 1e-12 * abs(np.linalg.det(A))
+    A = rng.standard_normal((6, 6)) + 1j * rng.standard_normal((6, 6))
+    mp.mp.dps = 40
+    ref = complex(mp.det(mp.matrix(A.tolist())))
+    assert abs(oracle.det_matrix(A) - ref) <= 1e-13 * abs(ref)
+    assert oracle.det_matrix(np.eye(9)) == 1.0
+    U = np.triu(A)
+    assert abs(oracle.det_matrix(U) - np.prod(np.diag(U))) <= 1e-14 * abs(np.prod(np.diag(U)))
+    P = np.eye(5)[[1, 0, 2, 4, 3]]                      # two swaps: det = +1
+    assert oracle.det_matrix(P) == 1.0
+    assert oracle.det_matrix(np.eye(4)[[1, 0, 2, 3]]) == -1.0
+    S = A.copy()
+    S[:, 2] = 0
+    assert oracle.det_matrix(S) == 0
+
+
+# --------------------------------------------------------------------------
+# Newton driver (NewtonSolver.jl:36-172) and grid scan (GridScan.jl)
+# --------------------------------------------------------------------------
+def test_newton_on_holomorphic_function_with_known_root():
+    root = 0.8794 + 0.1231j
+    f = lambda w: (w - root) * (w + 2.0) * np.exp(0.3 * w)  # noqa: E731
+    r = oracle.newton_fn(f, 2, 0.43, 0.11, tol=1e-13)
+    assert r["converged"]
+    assert abs(complex(2 * r["Omega_p"], r["gamma"]) - root) <= 1e-12
+    assert r["iterations"] <= 8
+    # bounds: gamma is clamped at 1e-4 and Omega_p into [0, 1]
+    g = lambda w: (w - (0.5 - 0.3j))  # noqa: E731  root below the gamma bound
+    r = oracle.newton_fn(g, 2, 0.25, 0.05, max_iter=6, tol=1e-14)
+    assert not r["converged"] and r["gamma"] == pytest.approx(1e-4)
+
+
+def test_grid_scan_argmin_column_major():
+    pre = dict(W=np.zeros((1, 1, 1), complex), DJ=np.zeros(1), F0l=np.zeros((1, 1)),
+               Omega_res=np.zeros((1, 1)), N_kernel=np.ones(1), G=1.0, m=2)
+    dg, op, g = oracle.grid_scan(pre, 4, 3, 0.2, 0.5, 0.01, 0.2)
+    assert np.all(dg == 1.0)          # M = 0 -> det(I) = 1 everywhere; first index wins
+    assert (op, g) == (0.2, 0.01)
