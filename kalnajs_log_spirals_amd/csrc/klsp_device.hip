@@ -121,7 +121,8 @@ __global__ __launch_bounds__(256) void k_orbit_prep(
 // Operands are generated in registers in the MFMA lane layout:
 //   A (16 l x 4 t): lane = i + 16*kk holds A[l0+i][t0+kk]
 //   B (4 t x 16 a): lane = i + 16*kk holds B[t0+kk][a0+i]
-//   D (16 l x 16 a): lane holds D[4*kk + v][i], v = 0..3
+//   D (16 l x 16 a): lane holds D[kk + 4*v][i], v = 0..3 (the FP64 MFMA's
+//   accumulator layout; the FP32 16x16x4 MFMA uses D[4*kk + v][i])
 constexpr int kWMaxLB = 4;
 
 __global__ __launch_bounds__(256) void k_W_f64(
@@ -186,7 +187,7 @@ __global__ __launch_bounds__(256) void k_W_f64(
             const int a = a0 + 16 * j + i;
             if (a >= N) continue;
             for (int v = 0; v < 4; ++v) {
-                const int li = lb * 16 + 4 * kk + v;
+                const int li = lb * 16 + kk + 4 * v;   // f64 D layout: row = lane/16 + 4v
                 if (li < n_l)
                     W[((int64_t)li * NPh + k) * Np + a] = make_double2(accr[q][j][v], acci[q][j][v]);
             }
@@ -262,7 +263,8 @@ __global__ void k_weights(const double* __restrict__ DJ, const double* __restric
 // 32x32 sub-tile = 2x2 MFMA blocks, for NB omegas at once (the W operand
 // loads are shared by the batch).  Complex products use four real MFMAs:
 //   Cr += Ar*Br + (-Ai)*Bi,   Ci += Ar*Bi + Ai*Br,   B = c * conj(W_alpha).
-// Partials go to P[b][s][alpha][beta] (column-major M per split).
+// Partials go to P[b][s][alpha][beta] (column-major M per split).  FP64
+// accumulator lanes hold rows kk + 4v, FP32 ones rows 4kk + v.
 // =====================================================================
 __device__ __forceinline__ void tile_coords(int T, int tilesN, int& tile, int& s, int& tb, int& ta)
 {
@@ -326,7 +328,7 @@ __global__ __launch_bounds__(256) void k_contract_f64(
             }
         }
     }
-    // D lane layout: rows (beta) 4*kk+v, column (alpha) i.
+    // D lane layout (FP64): rows (beta) kk + 4v, column (alpha) i.
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         double2* Pb = P + ((int64_t)b * S + s) * (int64_t)Np * Np;
@@ -337,10 +339,9 @@ __global__ __launch_bounds__(256) void k_contract_f64(
             for (int y = 0; y < 2; ++y) {
                 if (y == 1 && !va1) continue;
                 const int a = a0 + 16 * y + i;
-                const int bb = b0 + 16 * x + 4 * kk;
-                double2* dst = Pb + (int64_t)a * Np + bb;
+                double2* dst = Pb + (int64_t)a * Np + b0 + 16 * x + kk;
 #pragma unroll
-                for (int v = 0; v < 4; ++v) dst[v] = make_double2(cr[b][x][y][v], ci[b][x][y][v]);
+                for (int v = 0; v < 4; ++v) dst[4 * v] = make_double2(cr[b][x][y][v], ci[b][x][y][v]);
             }
         }
     }

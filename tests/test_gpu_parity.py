@@ -39,6 +39,13 @@ def device_pre(pre_oracle):
 OMEGAS = [2 * 0.4397142267 + 0.1230663944j, 0.6 + 0.05j, 1.1 + 0.2j, 0.9 - 0.1j]
 
 
+def det_scale(pre_o, om) -> float:
+    """Size of det near omega: the first OMEGAS entry is the published root,
+    where det itself is ~1e-13, so errors are measured against the det one
+    1e-3 away (an error e there moves the root by ~e / |det'| ~ 1e-3 e / scale)."""
+    return max(abs(oracle.compute_det(pre_o, om)), abs(oracle.compute_det(pre_o, om + 1e-3)))
+
+
 def test_library_reports_gfx950():
     print("library:", _ffi.version(), "devices:", _ffi.device_count(), flush=True)
     assert _ffi.device_count() >= 1
@@ -84,7 +91,7 @@ def test_det_fp64_matches_oracle_small(oracle_small, omega):
     dev = K.to_device(pre, K.GPUType.AMD, np.float64)
     d = K.compute_determinant(omega, pre, dev, K.GPUType.AMD, np.float64)
     do = oracle.compute_det(oracle_small, omega)
-    assert abs(d - do) <= 1e-11 * abs(do)
+    assert abs(d - do) <= 1e-11 * det_scale(oracle_small, omega)
     M = K.build_M_matrix(omega, pre)
     assert relnorm(M, oracle.build_M(oracle_small, omega)) <= 1e-12
 
@@ -96,8 +103,9 @@ def test_lu_matches_lapack(oracle_small):
         M = pre.ctx.build_M(om, _ffi.PREC_FP64)
         d = pre.ctx.det([om], _ffi.PREC_FP64)[0]
         ref = np.linalg.det(np.eye(M.shape[0]) - M)
-        assert abs(d - ref) <= 1e-12 * abs(ref)
-        assert abs(d - oracle.det_matrix(np.eye(M.shape[0]) - M)) <= 1e-12 * abs(ref)
+        sc = det_scale(oracle_small, om)
+        assert abs(d - ref) <= 1e-12 * sc
+        assert abs(d - oracle.det_matrix(np.eye(M.shape[0]) - M)) <= 1e-12 * sc
 
 
 RAGGED = [(1, 1, 1), (3, 5, 2), (13, 16, 1), (13, 17, 3), (7, 37, 2), (5, 64, 2), (11, 65, 3),
@@ -163,7 +171,7 @@ def test_conjugate_symmetry_small(oracle_small):
     pre = device_pre(oracle_small)
     for om in OMEGAS[:3]:
         d1, d2 = pre.ctx.det([om, np.conj(om)], _ffi.PREC_FP64)
-        assert abs(d2 - np.conj(d1)) <= 1e-11 * abs(d1)
+        assert abs(d2 - np.conj(d1)) <= 1e-11 * det_scale(oracle_small, om)
 
 
 def test_batched_equals_single(oracle_small):
