@@ -84,7 +84,7 @@ def main() -> int:
     ctx.set_profiling(True)
 
     om0 = complex(2 * 0.4397, 0.1231)
-    oms = [om0 + 0.001 * i * (1 + 1j) for i in range(args.warmup + args.steps)]
+    oms = [om0 + 0.001 * i * (1 + 1j) for i in range(args.warmup + max(args.steps, 3))]
     for i in range(args.warmup):
         ctx.det([oms[i]], _ffi.PREC_FP64)
     log(f"[bench] warmup done ({args.warmup} det)")
@@ -119,7 +119,7 @@ def main() -> int:
     if not args.no_fp32:
         ctx.prepare(_ffi.PREC_FP32)
         ctx.det([om0], _ffi.PREC_FP32)
-        n32 = max(3, args.steps // 2)
+        n32 = min(max(3, args.steps // 2), len(oms) - args.warmup)
         c32 = 0.0
         t0 = time.perf_counter()
         d32 = []

@@ -233,8 +233,10 @@ __global__ void k_W64_to_32(const double2* __restrict__ W, int64_t n, float2* __
 // =====================================================================
 __global__ void k_weights(const double* __restrict__ DJ, const double* __restrict__ F0l,
                           const double* __restrict__ Ores, int64_t NPh, int64_t K, int64_t Kp,
-                          const double* __restrict__ om, int nb, double2* __restrict__ c)
+                          const double* __restrict__ om, int nb, double2* __restrict__ c,
+                          int* __restrict__ counters)
 {
+    if (blockIdx.x == 0 && threadIdx.x < 8) counters[threadIdx.x] = 0;
     const int64_t n = (int64_t)nb * Kp;
     for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
          x += (int64_t)gridDim.x * blockDim.x) {
@@ -253,195 +255,354 @@ __global__ void k_weights(const double* __restrict__ DJ, const double* __restric
 }
 
 // =====================================================================
-// Stage 3: the contraction M_b[beta, alpha] = sum_K W[K][beta] c_b[K]
+// Stage 3: the contraction M_w[beta, alpha] = sum_K W[K][beta] c_w[K]
 // conj(W[K][alpha]) (MatrixBuilder.jl:119-141), split over K.
 //
-// Grid: T*S workgroups of 4 waves; T = tilesN^2 output tiles of 64x64, S
-// K-splits (S % 8 == 0).  Linear block L is remapped so that the T tiles of
-// one K-split land on the same XCD (blocks go round-robin over the 8 XCDs),
-// which keeps the W rows they share in that XCD's L2.  Each wave owns a
-// 32x32 sub-tile = 2x2 MFMA blocks, for NB omegas at once (the W operand
-// loads are shared by the batch).  Complex products use four real MFMAs:
-//   Cr += Ar*Br + (-Ai)*Bi,   Ci += Ar*Bi + Ai*Br,   B = c * conj(W_alpha).
-// Partials go to P[b][s][alpha][beta] (column-major M per split).  FP64
-// accumulator lanes hold rows kk + 4v, FP32 ones rows 4kk + v.
+// Work decomposition.  A workgroup is 4 waves arranged 2x2; a wave owns an
+// MB x NB grid of 16x16 output blocks for NW omegas over one K-split, so the
+// workgroup tile is (32*MB) x (32*NB).  Waves read their MFMA operands
+// straight from global memory -- four 256-byte row segments per 16x4
+// operand, shared with the partner wave through L1 -- and prefetch the next
+// K-step into registers before issuing the current step's MFMAs, so in
+// steady state the matrix pipe never waits on a load.  A wave whose tile
+// hangs over Np runs a compile-time specialisation with fewer blocks (no
+// per-block branches in the inner loop).  Complex products use four real
+// MFMAs per block:  Cr += Ar Br + (-Ai) Bi,  Ci += Ar Bi + Ai Br,  with
+// B = c * conj(W_alpha) formed in registers.
+//
+// Grid: T*S workgroups, T output tiles x S K-splits (S % 8 == 0).  The
+// linear block index is remapped so that the T tiles of one K-split land on
+// the same XCD (blocks are dealt round-robin over the 8 XCDs), which keeps
+// the W rows they share in that XCD's L2.  Partials go to
+// P[w][s][alpha][beta] (column-major M per split).  FP64 accumulator lanes
+// hold rows kk + 4v, FP32 ones rows 4kk + v.  Kp is a multiple of 8 and W
+// and c carry kRowPad zero rows past Kp, so the loads issued for the step
+// after the last one stay in bounds.
 // =====================================================================
-__device__ __forceinline__ void tile_coords(int T, int tilesN, int& tile, int& s, int& tb, int& ta)
+
+// One K-step (4 rows of K) of the wave tile: B = c conj(X) in registers, then
+// 4*NW*MB*NB MFMAs, ordered so consecutive MFMAs hit different accumulators.
+template <int MB, int NB, int NW, typename V, typename S2, typename ACC, typename MF>
+__device__ __forceinline__ void cstep(const S2 (&A)[MB], const S2 (&X)[NB], const double2 (&C)[NW],
+                                      ACC (&cr)[NW][MB][NB], ACC (&ci)[NW][MB][NB], MF mfma)
 {
-    const int L = blockIdx.x;
-    const int xcd = L & 7, j = L >> 3;
-    tile = j % T;
-    s = (j / T) * 8 + xcd;
-    tb = tile / tilesN;
-    ta = tile % tilesN;
-}
-
-template <int NB>
-__global__ __launch_bounds__(256) void k_contract_f64(
-    const double2* __restrict__ W, const double2* __restrict__ cw, double2* __restrict__ P,
-    int64_t Kp, int Np, int64_t Ks, int T, int tilesN, int S)
-{
-    int tile, s, tb, ta;
-    tile_coords(T, tilesN, tile, s, tb, ta);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = lane & 15, kk = lane >> 4;
-    const int b0 = tb * kTile + (wave >> 1) * 32;
-    const int a0 = ta * kTile + (wave & 1) * 32;
-    if (b0 >= Np || a0 >= Np) return;
-    const bool vb1 = b0 + 16 < Np, va1 = a0 + 16 < Np;
-    const int64_t Kb = (int64_t)s * Ks;
-    const int64_t Ke = min(Kp, Kb + Ks);
-
-    v4d cr[NB][2][2], ci[NB][2][2];
+    S2 Bv[NW][NB];
 #pragma unroll
-    for (int b = 0; b < NB; ++b)
+    for (int w = 0; w < NW; ++w) {
+        const V c0 = (V)C[w].x, c1 = (V)C[w].y;
 #pragma unroll
-        for (int x = 0; x < 2; ++x)
-#pragma unroll
-            for (int y = 0; y < 2; ++y) { cr[b][x][y] = (v4d){0, 0, 0, 0}; ci[b][x][y] = cr[b][x][y]; }
-
-    const double2 zero2 = make_double2(0.0, 0.0);
-    for (int64_t K0 = Kb; K0 < Ke; K0 += kKStep) {
-        const int64_t K = K0 + kk;
-        const double2* row = W + K * Np;
-        const double2 A0 = row[b0 + i];
-        const double2 A1 = vb1 ? row[b0 + 16 + i] : zero2;
-        const double2 X0 = row[a0 + i];
-        const double2 X1 = va1 ? row[a0 + 16 + i] : zero2;
-        const double nA0 = -A0.y, nA1 = -A1.y;
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const double2 c = cw[(int64_t)b * Kp + K];
-            // B = c * conj(X)
-            const double2 B0 = make_double2(c.x * X0.x + c.y * X0.y, c.y * X0.x - c.x * X0.y);
-            const double2 B1 = make_double2(c.x * X1.x + c.y * X1.y, c.y * X1.x - c.x * X1.y);
-#define KLSP_CMAC(ACR, ACI, AR, AI, NAI, BB)                                              \
-    ACR = __builtin_amdgcn_mfma_f64_16x16x4f64(AR, BB.x, ACR, 0, 0, 0);                    \
-    ACR = __builtin_amdgcn_mfma_f64_16x16x4f64(NAI, BB.y, ACR, 0, 0, 0);                   \
-    ACI = __builtin_amdgcn_mfma_f64_16x16x4f64(AR, BB.y, ACI, 0, 0, 0);                    \
-    ACI = __builtin_amdgcn_mfma_f64_16x16x4f64(AI, BB.x, ACI, 0, 0, 0);
-            KLSP_CMAC(cr[b][0][0], ci[b][0][0], A0.x, A0.y, nA0, B0)
-            if (va1) { KLSP_CMAC(cr[b][0][1], ci[b][0][1], A0.x, A0.y, nA0, B1) }
-            if (vb1) {
-                KLSP_CMAC(cr[b][1][0], ci[b][1][0], A1.x, A1.y, nA1, B0)
-                if (va1) { KLSP_CMAC(cr[b][1][1], ci[b][1][1], A1.x, A1.y, nA1, B1) }
-            }
+        for (int n = 0; n < NB; ++n) {
+            Bv[w][n].x = c0 * X[n].x + c1 * X[n].y;
+            Bv[w][n].y = c1 * X[n].x - c0 * X[n].y;
         }
     }
-    // D lane layout (FP64): rows (beta) kk + 4v, column (alpha) i.
+    V nAy[MB];
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        double2* Pb = P + ((int64_t)b * S + s) * (int64_t)Np * Np;
+    for (int m = 0; m < MB; ++m) nAy[m] = -A[m].y;
 #pragma unroll
-        for (int x = 0; x < 2; ++x) {
-            if (x == 1 && !vb1) continue;
+    for (int w = 0; w < NW; ++w)
 #pragma unroll
-            for (int y = 0; y < 2; ++y) {
-                if (y == 1 && !va1) continue;
-                const int a = a0 + 16 * y + i;
-                double2* dst = Pb + (int64_t)a * Np + b0 + 16 * x + kk;
+        for (int m = 0; m < MB; ++m)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) dst[4 * v] = make_double2(cr[b][x][y][v], ci[b][x][y][v]);
+            for (int n = 0; n < NB; ++n) cr[w][m][n] = mfma(A[m].x, Bv[w][n].x, cr[w][m][n]);
+#pragma unroll
+    for (int w = 0; w < NW; ++w)
+#pragma unroll
+        for (int m = 0; m < MB; ++m)
+#pragma unroll
+            for (int n = 0; n < NB; ++n) ci[w][m][n] = mfma(A[m].x, Bv[w][n].y, ci[w][m][n]);
+#pragma unroll
+    for (int w = 0; w < NW; ++w)
+#pragma unroll
+        for (int m = 0; m < MB; ++m)
+#pragma unroll
+            for (int n = 0; n < NB; ++n) cr[w][m][n] = mfma(nAy[m], Bv[w][n].y, cr[w][m][n]);
+#pragma unroll
+    for (int w = 0; w < NW; ++w)
+#pragma unroll
+        for (int m = 0; m < MB; ++m)
+#pragma unroll
+            for (int n = 0; n < NB; ++n) ci[w][m][n] = mfma(A[m].y, Bv[w][n].x, ci[w][m][n]);
+}
+
+// Operand fetch of one K-step for a wave: MB row segments of the beta side,
+// NB of the alpha side, NW weights.
+template <int MB, int NB, int NW, typename S2>
+__device__ __forceinline__ void cload(const S2* pa, const S2* px, const double2* pc, int64_t Kpc,
+                                      S2 (&A)[MB], S2 (&X)[NB], double2 (&C)[NW])
+{
+#pragma unroll
+    for (int m = 0; m < MB; ++m) A[m] = pa[16 * m];
+#pragma unroll
+    for (int n = 0; n < NB; ++n) X[n] = px[16 * n];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) C[w] = pc[w * Kpc];
+}
+
+struct MfmaF64 {
+    __device__ __forceinline__ v4d operator()(double a, double b, v4d c) const
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+struct MfmaF32 {
+    __device__ __forceinline__ v4f operator()(float a, float b, v4f c) const
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+// The K loop runs two K-steps per trip on two operand sets (ping-pong): the
+// loads of step k+1 are issued before the MFMAs of step k (scheduling
+// barriers keep the order), so a step's memory latency hides behind the
+// previous step's MFMAs.  K ranges are multiples of 8 rows.
+template <int MB, int NB, int NW>
+__device__ __forceinline__ void wave_f64(const double2* __restrict__ W,
+                                         const double2* __restrict__ cw,
+                                         double2* __restrict__ P, int64_t Kpc, int Np,
+                                         int64_t Kb, int64_t Ke, int b0, int a0, int s, int S,
+                                         int lane)
+{
+    const int i = lane & 15, kk = lane >> 4;
+    v4d cr[NW][MB][NB], ci[NW][MB][NB];
+#pragma unroll
+    for (int w = 0; w < NW; ++w)
+#pragma unroll
+        for (int m = 0; m < MB; ++m)
+#pragma unroll
+            for (int n = 0; n < NB; ++n) {
+                cr[w][m][n] = (v4d){0, 0, 0, 0};
+                ci[w][m][n] = (v4d){0, 0, 0, 0};
             }
+    const int64_t rstep = (int64_t)kKStep * Np;
+    const double2* pa = W + (Kb + kk) * Np + b0 + i;
+    const double2* px = W + (Kb + kk) * Np + a0 + i;
+    const double2* pc = cw + Kb + kk;
+    double2 A0[MB], X0[NB], C0[NW], A1[MB], X1[NB], C1[NW];
+    cload<MB, NB, NW>(pa, px, pc, Kpc, A0, X0, C0);
+    const MfmaF64 mf;
+    for (int64_t K0 = Kb; K0 < Ke; K0 += 2 * kKStep) {
+        cload<MB, NB, NW>(pa + rstep, px + rstep, pc + kKStep, Kpc, A1, X1, C1);
+        __builtin_amdgcn_sched_barrier(0);
+        cstep<MB, NB, NW, double>(A0, X0, C0, cr, ci, mf);
+        __builtin_amdgcn_sched_barrier(0);
+        pa += 2 * rstep;
+        px += 2 * rstep;
+        pc += 2 * kKStep;
+        cload<MB, NB, NW>(pa, px, pc, Kpc, A0, X0, C0);
+        __builtin_amdgcn_sched_barrier(0);
+        cstep<MB, NB, NW, double>(A1, X1, C1, cr, ci, mf);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        double2* Pw = P + ((int64_t)w * S + s) * (int64_t)Np * Np;
+#pragma unroll
+        for (int m = 0; m < MB; ++m)
+#pragma unroll
+            for (int n = 0; n < NB; ++n) {
+                double2* dst = Pw + (int64_t)(a0 + 16 * n + i) * Np + b0 + 16 * m + kk;
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    dst[4 * v] = make_double2(cr[w][m][n][v], ci[w][m][n][v]);
+            }
+    }
+}
+
+// Picks the specialisation with exactly mv x nv valid blocks (1 <= mv <= MB).
+template <int MB, int NB, int NW>
+__device__ __forceinline__ void dispatch_f64(int mv, int nv, const double2* W, const double2* cw,
+                                             double2* P, int64_t Kpc, int Np, int64_t Kb,
+                                             int64_t Ke, int b0, int a0, int s, int S, int lane)
+{
+    if constexpr (MB > 1) {
+        if (mv < MB) {
+            dispatch_f64<MB - 1, NB, NW>(mv, nv, W, cw, P, Kpc, Np, Kb, Ke, b0, a0, s, S, lane);
+            return;
         }
+    }
+    if constexpr (NB > 1) {
+        if (nv < NB) {
+            dispatch_f64<MB, NB - 1, NW>(mv, nv, W, cw, P, Kpc, Np, Kb, Ke, b0, a0, s, S, lane);
+            return;
+        }
+    }
+    wave_f64<MB, NB, NW>(W, cw, P, Kpc, Np, Kb, Ke, b0, a0, s, S, lane);
+}
+
+// Persistent form: the grid is exactly the resident workgroup count; each
+// workgroup pulls (tile, split) units from the counter of its XCD (block
+// index mod 8) until the XCD's units run out.  Unit u of XCD x is split
+// x + 8*(u / T), tile u % T, so the tiles of a split run together on one XCD.
+// counters[0..7] are zeroed by k_weights, launched just before on the stream.
+template <int MB, int NB, int NW>
+__global__ __launch_bounds__(256) void k_contract_f64(
+    const double2* __restrict__ W, const double2* __restrict__ cw, double2* __restrict__ P,
+    int64_t Kp, int Np, int64_t Ks, int S, int tiles_a, int T, int* __restrict__ counters)
+{
+    const int xcd = blockIdx.x & 7;
+    const int units = (S / 8) * T;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __shared__ int s_unit;
+    for (;;) {
+        if (threadIdx.x == 0) s_unit = atomicAdd(&counters[xcd], 1);
+        __syncthreads();
+        const int u = s_unit;
+        __syncthreads();
+        if (u >= units) break;
+        const int tile = u % T, s = xcd + 8 * (u / T);
+        const int tb = tile / tiles_a, ta = tile % tiles_a;
+        const int b0 = (tb * 2 + (wave >> 1)) * MB * 16;
+        const int a0 = (ta * 2 + (wave & 1)) * NB * 16;
+        const int mv = min(MB, (Np - b0) / 16), nv = min(NB, (Np - a0) / 16);
+        if (mv <= 0 || nv <= 0) continue;
+        const int64_t Kb = min((int64_t)s * Ks, Kp);
+        const int64_t Ke = min(Kp, Kb + Ks);
+        dispatch_f64<MB, NB, NW>(mv, nv, W, cw, P, Kp + kRowPad, Np, Kb, Ke, b0, a0, s, S, lane);
     }
 }
 
 // FP32 variant: W in complex64, weights rounded to FP32 after being formed
-// in FP64, FP32 MFMA over K-tiles of kF32Tile, K-tile partials added in FP64.
+// in FP64, FP32 MFMA over K-tiles of kF32Tile, K-tile partials added into
+// FP64 accumulators.
 constexpr int kF32Tile = 256;
 
-template <int NB>
-__global__ __launch_bounds__(256) void k_contract_f32(
-    const float2* __restrict__ W, const double2* __restrict__ cw, double2* __restrict__ P,
-    int64_t Kp, int Np, int64_t Ks, int T, int tilesN, int S)
+template <int MB, int NB, int NW>
+__device__ __forceinline__ void wave_f32(const float2* __restrict__ W,
+                                         const double2* __restrict__ cw,
+                                         double2* __restrict__ P, int64_t Kpc, int Np,
+                                         int64_t Kb, int64_t Ke, int b0, int a0, int s, int S,
+                                         int lane)
 {
-    int tile, s, tb, ta;
-    tile_coords(T, tilesN, tile, s, tb, ta);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = lane & 15, kk = lane >> 4;
-    const int b0 = tb * kTile + (wave >> 1) * 32;
-    const int a0 = ta * kTile + (wave & 1) * 32;
-    if (b0 >= Np || a0 >= Np) return;
-    const bool vb1 = b0 + 16 < Np, va1 = a0 + 16 < Np;
-    const int64_t Kb = (int64_t)s * Ks;
-    const int64_t Ke = min(Kp, Kb + Ks);
-
-    v4d dr[NB][2][2], di[NB][2][2];
+    v4d dr[NW][MB][NB], di[NW][MB][NB];
 #pragma unroll
-    for (int b = 0; b < NB; ++b)
+    for (int w = 0; w < NW; ++w)
 #pragma unroll
-        for (int x = 0; x < 2; ++x)
+        for (int m = 0; m < MB; ++m)
 #pragma unroll
-            for (int y = 0; y < 2; ++y) { dr[b][x][y] = (v4d){0, 0, 0, 0}; di[b][x][y] = dr[b][x][y]; }
-
-    const float2 zero2 = make_float2(0.f, 0.f);
-    for (int64_t T0 = Kb; T0 < Ke; T0 += kF32Tile) {
-        const int64_t Te = min(Ke, T0 + kF32Tile);
-        v4f cr[NB][2][2], ci[NB][2][2];
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-#pragma unroll
-            for (int x = 0; x < 2; ++x)
-#pragma unroll
-                for (int y = 0; y < 2; ++y) { cr[b][x][y] = (v4f){0, 0, 0, 0}; ci[b][x][y] = cr[b][x][y]; }
-        for (int64_t K0 = T0; K0 < Te; K0 += kKStep) {
-            const int64_t K = K0 + kk;
-            const float2* row = W + K * Np;
-            const float2 A0 = row[b0 + i];
-            const float2 A1 = vb1 ? row[b0 + 16 + i] : zero2;
-            const float2 X0 = row[a0 + i];
-            const float2 X1 = va1 ? row[a0 + 16 + i] : zero2;
-            const float nA0 = -A0.y, nA1 = -A1.y;
-#pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                const double2 c64 = cw[(int64_t)b * Kp + K];
-                const float2 c = make_float2((float)c64.x, (float)c64.y);
-                const float2 B0 = make_float2(c.x * X0.x + c.y * X0.y, c.y * X0.x - c.x * X0.y);
-                const float2 B1 = make_float2(c.x * X1.x + c.y * X1.y, c.y * X1.x - c.x * X1.y);
-#define KLSP_CMAC32(ACR, ACI, AR, AI, NAI, BB)                                            \
-    ACR = __builtin_amdgcn_mfma_f32_16x16x4f32(AR, BB.x, ACR, 0, 0, 0);                    \
-    ACR = __builtin_amdgcn_mfma_f32_16x16x4f32(NAI, BB.y, ACR, 0, 0, 0);                   \
-    ACI = __builtin_amdgcn_mfma_f32_16x16x4f32(AR, BB.y, ACI, 0, 0, 0);                    \
-    ACI = __builtin_amdgcn_mfma_f32_16x16x4f32(AI, BB.x, ACI, 0, 0, 0);
-                KLSP_CMAC32(cr[b][0][0], ci[b][0][0], A0.x, A0.y, nA0, B0)
-                if (va1) { KLSP_CMAC32(cr[b][0][1], ci[b][0][1], A0.x, A0.y, nA0, B1) }
-                if (vb1) {
-                    KLSP_CMAC32(cr[b][1][0], ci[b][1][0], A1.x, A1.y, nA1, B0)
-                    if (va1) { KLSP_CMAC32(cr[b][1][1], ci[b][1][1], A1.x, A1.y, nA1, B1) }
-                }
+            for (int n = 0; n < NB; ++n) {
+                dr[w][m][n] = (v4d){0, 0, 0, 0};
+                di[w][m][n] = (v4d){0, 0, 0, 0};
             }
+    const int64_t rstep = (int64_t)kKStep * Np;
+    const float2* pa = W + (Kb + kk) * Np + b0 + i;
+    const float2* px = W + (Kb + kk) * Np + a0 + i;
+    const double2* pc = cw + Kb + kk;
+    float2 A0[MB], X0[NB], A1[MB], X1[NB];
+    double2 C0[NW], C1[NW];
+    cload<MB, NB, NW>(pa, px, pc, Kpc, A0, X0, C0);
+    const MfmaF32 mf;
+    for (int64_t T0 = Kb; T0 < Ke; T0 += kF32Tile) {
+        const int npairs = (int)((min(Ke, T0 + kF32Tile) - T0) / (2 * kKStep));
+        v4f cr[NW][MB][NB], ci[NW][MB][NB];
+#pragma unroll
+        for (int w = 0; w < NW; ++w)
+#pragma unroll
+            for (int m = 0; m < MB; ++m)
+#pragma unroll
+                for (int n = 0; n < NB; ++n) {
+                    cr[w][m][n] = (v4f){0, 0, 0, 0};
+                    ci[w][m][n] = (v4f){0, 0, 0, 0};
+                }
+        for (int st = 0; st < npairs; ++st) {
+            cload<MB, NB, NW>(pa + rstep, px + rstep, pc + kKStep, Kpc, A1, X1, C1);
+            __builtin_amdgcn_sched_barrier(0);
+            cstep<MB, NB, NW, float>(A0, X0, C0, cr, ci, mf);
+            __builtin_amdgcn_sched_barrier(0);
+            pa += 2 * rstep;
+            px += 2 * rstep;
+            pc += 2 * kKStep;
+            cload<MB, NB, NW>(pa, px, pc, Kpc, A0, X0, C0);
+            __builtin_amdgcn_sched_barrier(0);
+            cstep<MB, NB, NW, float>(A1, X1, C1, cr, ci, mf);
+            __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
-        for (int b = 0; b < NB; ++b)
+        for (int w = 0; w < NW; ++w)
 #pragma unroll
-            for (int x = 0; x < 2; ++x)
+            for (int m = 0; m < MB; ++m)
 #pragma unroll
-                for (int y = 0; y < 2; ++y)
+                for (int n = 0; n < NB; ++n)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        dr[b][x][y][v] += (double)cr[b][x][y][v];
-                        di[b][x][y][v] += (double)ci[b][x][y][v];
+                        dr[w][m][n][v] += (double)cr[w][m][n][v];
+                        di[w][m][n][v] += (double)ci[w][m][n][v];
                     }
     }
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        double2* Pb = P + ((int64_t)b * S + s) * (int64_t)Np * Np;
+    for (int w = 0; w < NW; ++w) {
+        double2* Pw = P + ((int64_t)w * S + s) * (int64_t)Np * Np;
 #pragma unroll
-        for (int x = 0; x < 2; ++x) {
-            if (x == 1 && !vb1) continue;
+        for (int m = 0; m < MB; ++m)
 #pragma unroll
-            for (int y = 0; y < 2; ++y) {
-                if (y == 1 && !va1) continue;
-                const int a = a0 + 16 * y + i;
-                const int bb = b0 + 16 * x + 4 * kk;
-                double2* dst = Pb + (int64_t)a * Np + bb;
+            for (int n = 0; n < NB; ++n) {
+                double2* dst = Pw + (int64_t)(a0 + 16 * n + i) * Np + b0 + 16 * m + 4 * kk;
 #pragma unroll
-                for (int v = 0; v < 4; ++v) dst[v] = make_double2(dr[b][x][y][v], di[b][x][y][v]);
+                for (int v = 0; v < 4; ++v) dst[v] = make_double2(dr[w][m][n][v], di[w][m][n][v]);
             }
-        }
     }
 }
+
+template <int MB, int NB, int NW>
+__device__ __forceinline__ void dispatch_f32(int mv, int nv, const float2* W, const double2* cw,
+                                             double2* P, int64_t Kpc, int Np, int64_t Kb,
+                                             int64_t Ke, int b0, int a0, int s, int S, int lane)
+{
+    if constexpr (MB > 1) {
+        if (mv < MB) {
+            dispatch_f32<MB - 1, NB, NW>(mv, nv, W, cw, P, Kpc, Np, Kb, Ke, b0, a0, s, S, lane);
+            return;
+        }
+    }
+    if constexpr (NB > 1) {
+        if (nv < NB) {
+            dispatch_f32<MB, NB - 1, NW>(mv, nv, W, cw, P, Kpc, Np, Kb, Ke, b0, a0, s, S, lane);
+            return;
+        }
+    }
+    wave_f32<MB, NB, NW>(W, cw, P, Kpc, Np, Kb, Ke, b0, a0, s, S, lane);
+}
+
+template <int MB, int NB, int NW>
+__global__ __launch_bounds__(256) void k_contract_f32(
+    const float2* __restrict__ W, const double2* __restrict__ cw, double2* __restrict__ P,
+    int64_t Kp, int Np, int64_t Ks, int S, int tiles_a, int T, int* __restrict__ counters)
+{
+    const int xcd = blockIdx.x & 7;
+    const int units = (S / 8) * T;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __shared__ int s_unit;
+    for (;;) {
+        if (threadIdx.x == 0) s_unit = atomicAdd(&counters[xcd], 1);
+        __syncthreads();
+        const int u = s_unit;
+        __syncthreads();
+        if (u >= units) break;
+        const int tile = u % T, s = xcd + 8 * (u / T);
+        const int tb = tile / tiles_a, ta = tile % tiles_a;
+        const int b0 = (tb * 2 + (wave >> 1)) * MB * 16;
+        const int a0 = (ta * 2 + (wave & 1)) * NB * 16;
+        const int mv = min(MB, (Np - b0) / 16), nv = min(NB, (Np - a0) / 16);
+        if (mv <= 0 || nv <= 0) continue;
+        const int64_t Kb = min((int64_t)s * Ks, Kp);
+        const int64_t Ke = min(Kp, Kb + Ks);
+        dispatch_f32<MB, NB, NW>(mv, nv, W, cw, P, Kp + kRowPad, Np, Kb, Ke, b0, a0, s, S, lane);
+    }
+}
+
+// Wave-tile shapes (MB x NB blocks) per precision and omega-batch size.
+template <int NW> struct F64Shape;
+#ifndef KLSP_F64_MB1
+#define KLSP_F64_MB1 4
+#define KLSP_F64_NB1 2
+#endif
+template <> struct F64Shape<1> { static constexpr int MB = KLSP_F64_MB1, NB = KLSP_F64_NB1; };
+template <> struct F64Shape<2> { static constexpr int MB = 2, NB = 4; };
+template <> struct F64Shape<3> { static constexpr int MB = 2, NB = 2; };
+template <int NW> struct F32Shape;
+template <> struct F32Shape<1> { static constexpr int MB = 4, NB = 2; };
+template <> struct F32Shape<2> { static constexpr int MB = 2, NB = 2; };
 
 // Sum of the S K-split partials (fixed order, FP64), then *G and *N[alpha]
 // (MatrixBuilder.jl:144-149).  M is [nb][N, N] column-major.
@@ -592,6 +753,9 @@ struct klsp_ctx {
     double2* d_M = nullptr;   size_t M_cap = 0;
     double2* d_A = nullptr;   size_t A_cap = 0;
     double2* d_det = nullptr;
+    int last_S = 0;           // K-splits of the last contraction launch
+    int* d_counters = nullptr;  // per-XCD work counters of the persistent contraction
+    std::vector<std::pair<const void*, int>> occ_cache;
     // profiling
     bool prof = false;
     hipEvent_t ev[6] = {};
@@ -637,25 +801,36 @@ static int grid_for(int64_t n, int block = 256)
 
 // K-split count and K per split for the contraction.
 struct SplitPlan {
-    int tilesN, T, S;
+    int tiles_a, T, S;
     int64_t Ks;
 };
 
-static SplitPlan plan_splits(const klsp_ctx* ctx)
+// Output tiles of (32*MB) x (32*NB) and S K-splits (S % 8 == 0, K per split
+// a multiple of kF32Tile), sized so the persistent grid sees ~6 units per
+// resident workgroup (the tail of the last units stays short).
+static SplitPlan plan_splits(const klsp_ctx* ctx, int MB, int NB)
 {
     SplitPlan sp;
-    sp.tilesN = (int)((ctx->Np + kTile - 1) / kTile);
-    sp.T = sp.tilesN * sp.tilesN;
-    const int64_t target_wg = (int64_t)ctx->n_cu * 8;
-    int64_t S = (target_wg + sp.T - 1) / sp.T;
-    // at least kF32Tile K per split
+    const int tiles_b = (int)((ctx->Np + 32 * MB - 1) / (32 * MB));
+    sp.tiles_a = (int)((ctx->Np + 32 * NB - 1) / (32 * NB));
+    sp.T = tiles_b * sp.tiles_a;
+    const int64_t target_units = (int64_t)ctx->n_cu * 2 * 6;
+    int64_t S = (target_units + sp.T - 1) / sp.T;
     S = std::min<int64_t>(S, std::max<int64_t>(1, ctx->Kp / kF32Tile));
     S = round_up(std::max<int64_t>(S, 1), 8);
     sp.Ks = round_up((ctx->Kp + S - 1) / S, kF32Tile);
-    // drop splits that would be empty
     const int64_t used = (ctx->Kp + sp.Ks - 1) / sp.Ks;
     sp.S = (int)round_up(std::max<int64_t>(used, 1), 8);
     return sp;
+}
+
+// Largest split count any shape can ask for (sizes the partials buffer).
+static int max_splits(const klsp_ctx* ctx)
+{
+    int S = 0;
+    const int shapes[][2] = {{4, 4}, {2, 4}, {2, 2}, {4, 2}};
+    for (auto& sh : shapes) S = std::max(S, plan_splits(ctx, sh[0], sh[1]).S);
+    return S;
 }
 
 static int free_problem(klsp_ctx* ctx)
@@ -701,7 +876,8 @@ int klsp_create(int device, klsp_ctx** ctx_out)
     }
     for (auto& e : ctx->ev) hipEventCreate(&e);
     if (hipMalloc((void**)&ctx->d_om, sizeof(double) * 2 * kMaxBatch) != hipSuccess ||
-        hipMalloc((void**)&ctx->d_det, sizeof(double2) * kMaxBatch) != hipSuccess) {
+        hipMalloc((void**)&ctx->d_det, sizeof(double2) * kMaxBatch) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_counters, sizeof(int) * 8) != hipSuccess) {
         delete ctx;
         return KLSP_ERR_OOM;
     }
@@ -716,7 +892,7 @@ int klsp_destroy(klsp_ctx* ctx)
     hipStreamSynchronize(ctx->stream);
     free_problem(ctx);
     hipFree(ctx->d_om); hipFree(ctx->d_c); hipFree(ctx->d_P); hipFree(ctx->d_M);
-    hipFree(ctx->d_A); hipFree(ctx->d_det);
+    hipFree(ctx->d_A); hipFree(ctx->d_det); hipFree(ctx->d_counters);
     for (auto& e : ctx->ev) hipEventDestroy(e);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -740,7 +916,7 @@ int klsp_upload_problem(klsp_ctx* ctx, const klsp_problem* pr)
     ctx->N = pr->N_alpha;
     ctx->n_l = pr->n_l;
     ctx->K = pr->NPh * pr->n_l;
-    ctx->Kp = round_up(ctx->K, kKStep);
+    ctx->Kp = round_up(ctx->K, 2 * kKStep);
     ctx->Np = (int)round_up(ctx->N, kColPad);
     ctx->m = pr->m;
     ctx->G = pr->G;
@@ -761,8 +937,10 @@ static int alloc_W64(klsp_ctx* ctx)
 {
     if (ctx->d_W32) { KLSP_HIP(ctx, hipFree(ctx->d_W32)); ctx->d_W32 = nullptr; }
     if (!ctx->d_W64)
-        KLSP_HIP(ctx, hipMalloc((void**)&ctx->d_W64, sizeof(double2) * ctx->Kp * ctx->Np));
-    KLSP_HIP(ctx, hipMemsetAsync(ctx->d_W64, 0, sizeof(double2) * ctx->Kp * ctx->Np, ctx->stream));
+        KLSP_HIP(ctx, hipMalloc((void**)&ctx->d_W64,
+                                sizeof(double2) * (ctx->Kp + kRowPad) * ctx->Np));
+    KLSP_HIP(ctx, hipMemsetAsync(ctx->d_W64, 0, sizeof(double2) * (ctx->Kp + kRowPad) * ctx->Np,
+                                 ctx->stream));
     return KLSP_OK;
 }
 
@@ -893,7 +1071,7 @@ int klsp_prepare(klsp_ctx* ctx, int precision)
     if (!ctx->have_W) return fail(ctx, KLSP_ERR_NOT_READY, "klsp_prepare: no W on device");
     if (precision == KLSP_PREC_FP32 && !ctx->d_W32) {
         KLSP_HIP(ctx, hipSetDevice(ctx->device));
-        const int64_t n = ctx->Kp * ctx->Np;
+        const int64_t n = (ctx->Kp + kRowPad) * ctx->Np;
         KLSP_HIP(ctx, hipMalloc((void**)&ctx->d_W32, sizeof(float2) * n));
         hipLaunchKernelGGL(k_W64_to_32, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
                            ctx->d_W64, n, ctx->d_W32);
@@ -904,46 +1082,74 @@ int klsp_prepare(klsp_ctx* ctx, int precision)
 
 }  // extern "C"
 
+// Resident workgroups of a 256-thread kernel on the whole device (a multiple
+// of 8, so every XCD counter has workgroups pulling from it).
+static int resident_grid(klsp_ctx* ctx, const void* kernel)
+{
+    for (auto& e : ctx->occ_cache)
+        if (e.first == kernel) return e.second;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    const int g = (int)round_up((int64_t)ctx->n_cu * per_cu, 8);
+    ctx->occ_cache.emplace_back(kernel, g);
+    return g;
+}
+
+template <int NW>
+static void launch_f64(klsp_ctx* ctx)
+{
+    constexpr int MB = F64Shape<NW>::MB, NB = F64Shape<NW>::NB;
+    const SplitPlan sp = plan_splits(ctx, MB, NB);
+    const void* kfn = (const void*)k_contract_f64<MB, NB, NW>;
+    hipLaunchKernelGGL((k_contract_f64<MB, NB, NW>), dim3((unsigned)resident_grid(ctx, kfn)),
+                       dim3(256), 0, ctx->stream, ctx->d_W64, ctx->d_c, ctx->d_P, ctx->Kp, ctx->Np,
+                       sp.Ks, sp.S, sp.tiles_a, sp.T, ctx->d_counters);
+    ctx->last_S = sp.S;
+}
+
+template <int NW>
+static void launch_f32(klsp_ctx* ctx)
+{
+    constexpr int MB = F32Shape<NW>::MB, NB = F32Shape<NW>::NB;
+    const SplitPlan sp = plan_splits(ctx, MB, NB);
+    const void* kfn = (const void*)k_contract_f32<MB, NB, NW>;
+    hipLaunchKernelGGL((k_contract_f32<MB, NB, NW>), dim3((unsigned)resident_grid(ctx, kfn)),
+                       dim3(256), 0, ctx->stream, ctx->d_W32, ctx->d_c, ctx->d_P, ctx->Kp, ctx->Np,
+                       sp.Ks, sp.S, sp.tiles_a, sp.T, ctx->d_counters);
+    ctx->last_S = sp.S;
+}
+
 // Builds M for nb <= kMaxBatch omegas into ctx->d_M ([nb][N,N]).
 static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision)
 {
-    const int64_t Kp = ctx->Kp;
-    const SplitPlan sp = plan_splits(ctx);
+    const int64_t Kpc = ctx->Kp + kRowPad;
     int rc;
-    if ((rc = ensure(ctx, ctx->d_c, ctx->c_cap, (size_t)kMaxBatch * Kp))) return rc;
-    if ((rc = ensure(ctx, ctx->d_P, ctx->P_cap, (size_t)kMaxBatch * sp.S * ctx->Np * ctx->Np)))
+    if ((rc = ensure(ctx, ctx->d_c, ctx->c_cap, (size_t)kMaxBatch * Kpc))) return rc;
+    if ((rc = ensure(ctx, ctx->d_P, ctx->P_cap,
+                     (size_t)kMaxBatch * max_splits(ctx) * ctx->Np * ctx->Np)))
         return rc;
     if ((rc = ensure(ctx, ctx->d_M, ctx->M_cap, (size_t)kMaxBatch * ctx->N * ctx->N))) return rc;
     KLSP_HIP(ctx, hipMemcpyAsync(ctx->d_om, om, sizeof(double) * 2 * nb, hipMemcpyHostToDevice,
                                  ctx->stream));
     if (ctx->prof) hipEventRecord(ctx->ev[0], ctx->stream);
-    hipLaunchKernelGGL(k_weights, dim3(grid_for((int64_t)nb * Kp)), dim3(256), 0, ctx->stream,
-                       ctx->d_DJ, ctx->d_F0l, ctx->d_Ores, ctx->NPh, ctx->K, Kp, ctx->d_om, nb,
-                       ctx->d_c);
+    hipLaunchKernelGGL(k_weights, dim3(grid_for((int64_t)nb * Kpc)), dim3(256), 0, ctx->stream,
+                       ctx->d_DJ, ctx->d_F0l, ctx->d_Ores, ctx->NPh, ctx->K, Kpc, ctx->d_om, nb,
+                       ctx->d_c, ctx->d_counters);
     if (ctx->prof) hipEventRecord(ctx->ev[1], ctx->stream);
-    const dim3 grid((unsigned)(sp.T * sp.S)), block(256);
     if (precision == KLSP_PREC_FP64) {
-        if (nb == 1)
-            hipLaunchKernelGGL(k_contract_f64<1>, grid, block, 0, ctx->stream, ctx->d_W64, ctx->d_c,
-                               ctx->d_P, Kp, ctx->Np, sp.Ks, sp.T, sp.tilesN, sp.S);
-        else if (nb == 2)
-            hipLaunchKernelGGL(k_contract_f64<2>, grid, block, 0, ctx->stream, ctx->d_W64, ctx->d_c,
-                               ctx->d_P, Kp, ctx->Np, sp.Ks, sp.T, sp.tilesN, sp.S);
-        else
-            hipLaunchKernelGGL(k_contract_f64<3>, grid, block, 0, ctx->stream, ctx->d_W64, ctx->d_c,
-                               ctx->d_P, Kp, ctx->Np, sp.Ks, sp.T, sp.tilesN, sp.S);
+        if (nb == 1) launch_f64<1>(ctx);
+        else if (nb == 2) launch_f64<2>(ctx);
+        else launch_f64<3>(ctx);
     } else {
-        if (nb == 1)
-            hipLaunchKernelGGL(k_contract_f32<1>, grid, block, 0, ctx->stream, ctx->d_W32, ctx->d_c,
-                               ctx->d_P, Kp, ctx->Np, sp.Ks, sp.T, sp.tilesN, sp.S);
-        else
-            hipLaunchKernelGGL(k_contract_f32<2>, grid, block, 0, ctx->stream, ctx->d_W32, ctx->d_c,
-                               ctx->d_P, Kp, ctx->Np, sp.Ks, sp.T, sp.tilesN, sp.S);
+        if (nb == 1) launch_f32<1>(ctx);
+        else launch_f32<2>(ctx);
     }
     if (ctx->prof) hipEventRecord(ctx->ev[2], ctx->stream);
     hipLaunchKernelGGL(k_reduce, dim3(grid_for((int64_t)nb * ctx->N * ctx->N)), dim3(256), 0,
-                       ctx->stream, ctx->d_P, sp.S, ctx->Np, (int)ctx->N, nb, ctx->G, ctx->d_Nk,
-                       ctx->d_M);
+                       ctx->stream, ctx->d_P, ctx->last_S, ctx->Np, (int)ctx->N, nb, ctx->G,
+                       ctx->d_Nk, ctx->d_M);
     if (ctx->prof) hipEventRecord(ctx->ev[3], ctx->stream);
     KLSP_HIP(ctx, hipGetLastError());
     return KLSP_OK;
