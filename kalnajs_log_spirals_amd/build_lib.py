@@ -27,9 +27,14 @@ COMMON = [
 ]
 DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
+# (source, is HIP device code, extra flags).  The FP32 contraction keeps its
+# MFMA accumulators in VGPRs (-amdgpu-mfma-vgpr-form), which fits it in 2
+# waves per SIMD; the FP64 one is best with the default AGPR form.
 SOURCES = [
-    ("klsp_host.cpp", False),
-    ("klsp_device.hip", True),
+    ("klsp_host.cpp", False, []),
+    ("klsp_device.hip", True, []),
+    ("klsp_contract_f64.hip", True, []),
+    ("klsp_contract_f32.hip", True, ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
 ]
 
 
@@ -41,9 +46,9 @@ def _run(cmd):
 def build(verbose: bool = True) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     objs = []
-    for src, is_hip in SOURCES:
+    for src, is_hip, extra in SOURCES:
         obj = os.path.join(LIBDIR, src.rsplit(".", 1)[0] + ".o")
-        cmd = [HIPCC] + COMMON + (DEVICE + ["-x", "hip"] if is_hip else ["-x", "c++"])
+        cmd = [HIPCC] + COMMON + (DEVICE + ["-x", "hip"] if is_hip else ["-x", "c++"]) + extra
         cmd += ["-c", os.path.join(CSRC, src), "-o", obj]
         _run(cmd)
         objs.append(obj)

@@ -24,12 +24,11 @@
 #include <vector>
 
 #include "klsp.h"
+#include "klsp_contract.hpp"
 #include "klsp_internal.hpp"
 
 using namespace klsp;
 
-typedef double v4d __attribute__((ext_vector_type(4)));
-typedef float v4f __attribute__((ext_vector_type(4)));
 
 #define KLSP_VERSION_STRING "klsp 0.1.0 gfx950"
 
@@ -233,10 +232,8 @@ __global__ void k_W64_to_32(const double2* __restrict__ W, int64_t n, float2* __
 // =====================================================================
 __global__ void k_weights(const double* __restrict__ DJ, const double* __restrict__ F0l,
                           const double* __restrict__ Ores, int64_t NPh, int64_t K, int64_t Kp,
-                          const double* __restrict__ om, int nb, double2* __restrict__ c,
-                          int* __restrict__ counters)
+                          const double* __restrict__ om, int nb, double2* __restrict__ c)
 {
-    if (blockIdx.x == 0 && threadIdx.x < 8) counters[threadIdx.x] = 0;
     const int64_t n = (int64_t)nb * Kp;
     for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
          x += (int64_t)gridDim.x * blockDim.x) {
@@ -255,354 +252,9 @@ __global__ void k_weights(const double* __restrict__ DJ, const double* __restric
 }
 
 // =====================================================================
-// Stage 3: the contraction M_w[beta, alpha] = sum_K W[K][beta] c_w[K]
-// conj(W[K][alpha]) (MatrixBuilder.jl:119-141), split over K.
-//
-// Work decomposition.  A workgroup is 4 waves arranged 2x2; a wave owns an
-// MB x NB grid of 16x16 output blocks for NW omegas over one K-split, so the
-// workgroup tile is (32*MB) x (32*NB).  Waves read their MFMA operands
-// straight from global memory -- four 256-byte row segments per 16x4
-// operand, shared with the partner wave through L1 -- and prefetch the next
-// K-step into registers before issuing the current step's MFMAs, so in
-// steady state the matrix pipe never waits on a load.  A wave whose tile
-// hangs over Np runs a compile-time specialisation with fewer blocks (no
-// per-block branches in the inner loop).  Complex products use four real
-// MFMAs per block:  Cr += Ar Br + (-Ai) Bi,  Ci += Ar Bi + Ai Br,  with
-// B = c * conj(W_alpha) formed in registers.
-//
-// Grid: T*S workgroups, T output tiles x S K-splits (S % 8 == 0).  The
-// linear block index is remapped so that the T tiles of one K-split land on
-// the same XCD (blocks are dealt round-robin over the 8 XCDs), which keeps
-// the W rows they share in that XCD's L2.  Partials go to
-// P[w][s][alpha][beta] (column-major M per split).  FP64 accumulator lanes
-// hold rows kk + 4v, FP32 ones rows 4kk + v.  Kp is a multiple of 8 and W
-// and c carry kRowPad zero rows past Kp, so the loads issued for the step
-// after the last one stay in bounds.
+// Stage 3: the contraction M_w = sum_K W^T diag(c_w) conj(W): see
+// klsp_contract.hpp (kernels in klsp_contract_f64.hip / klsp_contract_f32.hip).
 // =====================================================================
-
-// One K-step (4 rows of K) of the wave tile: B = c conj(X) in registers, then
-// 4*NW*MB*NB MFMAs, ordered so consecutive MFMAs hit different accumulators.
-template <int MB, int NB, int NW, typename V, typename S2, typename ACC, typename MF>
-__device__ __forceinline__ void cstep(const S2 (&A)[MB], const S2 (&X)[NB], const double2 (&C)[NW],
-                                      ACC (&cr)[NW][MB][NB], ACC (&ci)[NW][MB][NB], MF mfma)
-{
-    S2 Bv[NW][NB];
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-        const V c0 = (V)C[w].x, c1 = (V)C[w].y;
-#pragma unroll
-        for (int n = 0; n < NB; ++n) {
-            Bv[w][n].x = c0 * X[n].x + c1 * X[n].y;
-            Bv[w][n].y = c1 * X[n].x - c0 * X[n].y;
-        }
-    }
-    V nAy[MB];
-#pragma unroll
-    for (int m = 0; m < MB; ++m) nAy[m] = -A[m].y;
-#pragma unroll
-    for (int w = 0; w < NW; ++w)
-#pragma unroll
-        for (int m = 0; m < MB; ++m)
-#pragma unroll
-            for (int n = 0; n < NB; ++n) cr[w][m][n] = mfma(A[m].x, Bv[w][n].x, cr[w][m][n]);
-#pragma unroll
-    for (int w = 0; w < NW; ++w)
-#pragma unroll
-        for (int m = 0; m < MB; ++m)
-#pragma unroll
-            for (int n = 0; n < NB; ++n) ci[w][m][n] = mfma(A[m].x, Bv[w][n].y, ci[w][m][n]);
-#pragma unroll
-    for (int w = 0; w < NW; ++w)
-#pragma unroll
-        for (int m = 0; m < MB; ++m)
-#pragma unroll
-            for (int n = 0; n < NB; ++n) cr[w][m][n] = mfma(nAy[m], Bv[w][n].y, cr[w][m][n]);
-#pragma unroll
-    for (int w = 0; w < NW; ++w)
-#pragma unroll
-        for (int m = 0; m < MB; ++m)
-#pragma unroll
-            for (int n = 0; n < NB; ++n) ci[w][m][n] = mfma(A[m].y, Bv[w][n].x, ci[w][m][n]);
-}
-
-// Operand fetch of one K-step for a wave: MB row segments of the beta side,
-// NB of the alpha side, NW weights.
-template <int MB, int NB, int NW, typename S2>
-__device__ __forceinline__ void cload(const S2* pa, const S2* px, const double2* pc, int64_t Kpc,
-                                      S2 (&A)[MB], S2 (&X)[NB], double2 (&C)[NW])
-{
-#pragma unroll
-    for (int m = 0; m < MB; ++m) A[m] = pa[16 * m];
-#pragma unroll
-    for (int n = 0; n < NB; ++n) X[n] = px[16 * n];
-#pragma unroll
-    for (int w = 0; w < NW; ++w) C[w] = pc[w * Kpc];
-}
-
-struct MfmaF64 {
-    __device__ __forceinline__ v4d operator()(double a, double b, v4d c) const
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-};
-struct MfmaF32 {
-    __device__ __forceinline__ v4f operator()(float a, float b, v4f c) const
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-};
-
-// The K loop runs two K-steps per trip on two operand sets (ping-pong): the
-// loads of step k+1 are issued before the MFMAs of step k (scheduling
-// barriers keep the order), so a step's memory latency hides behind the
-// previous step's MFMAs.  K ranges are multiples of 8 rows.
-template <int MB, int NB, int NW>
-__device__ __forceinline__ void wave_f64(const double2* __restrict__ W,
-                                         const double2* __restrict__ cw,
-                                         double2* __restrict__ P, int64_t Kpc, int Np,
-                                         int64_t Kb, int64_t Ke, int b0, int a0, int s, int S,
-                                         int lane)
-{
-    const int i = lane & 15, kk = lane >> 4;
-    v4d cr[NW][MB][NB], ci[NW][MB][NB];
-#pragma unroll
-    for (int w = 0; w < NW; ++w)
-#pragma unroll
-        for (int m = 0; m < MB; ++m)
-#pragma unroll
-            for (int n = 0; n < NB; ++n) {
-                cr[w][m][n] = (v4d){0, 0, 0, 0};
-                ci[w][m][n] = (v4d){0, 0, 0, 0};
-            }
-    const int64_t rstep = (int64_t)kKStep * Np;
-    const double2* pa = W + (Kb + kk) * Np + b0 + i;
-    const double2* px = W + (Kb + kk) * Np + a0 + i;
-    const double2* pc = cw + Kb + kk;
-    double2 A0[MB], X0[NB], C0[NW], A1[MB], X1[NB], C1[NW];
-    cload<MB, NB, NW>(pa, px, pc, Kpc, A0, X0, C0);
-    const MfmaF64 mf;
-    for (int64_t K0 = Kb; K0 < Ke; K0 += 2 * kKStep) {
-        cload<MB, NB, NW>(pa + rstep, px + rstep, pc + kKStep, Kpc, A1, X1, C1);
-        __builtin_amdgcn_sched_barrier(0);
-        cstep<MB, NB, NW, double>(A0, X0, C0, cr, ci, mf);
-        __builtin_amdgcn_sched_barrier(0);
-        pa += 2 * rstep;
-        px += 2 * rstep;
-        pc += 2 * kKStep;
-        cload<MB, NB, NW>(pa, px, pc, Kpc, A0, X0, C0);
-        __builtin_amdgcn_sched_barrier(0);
-        cstep<MB, NB, NW, double>(A1, X1, C1, cr, ci, mf);
-        __builtin_amdgcn_sched_barrier(0);
-    }
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-        double2* Pw = P + ((int64_t)w * S + s) * (int64_t)Np * Np;
-#pragma unroll
-        for (int m = 0; m < MB; ++m)
-#pragma unroll
-            for (int n = 0; n < NB; ++n) {
-                double2* dst = Pw + (int64_t)(a0 + 16 * n + i) * Np + b0 + 16 * m + kk;
-#pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    dst[4 * v] = make_double2(cr[w][m][n][v], ci[w][m][n][v]);
-            }
-    }
-}
-
-// Picks the specialisation with exactly mv x nv valid blocks (1 <= mv <= MB).
-template <int MB, int NB, int NW>
-__device__ __forceinline__ void dispatch_f64(int mv, int nv, const double2* W, const double2* cw,
-                                             double2* P, int64_t Kpc, int Np, int64_t Kb,
-                                             int64_t Ke, int b0, int a0, int s, int S, int lane)
-{
-    if constexpr (MB > 1) {
-        if (mv < MB) {
-            dispatch_f64<MB - 1, NB, NW>(mv, nv, W, cw, P, Kpc, Np, Kb, Ke, b0, a0, s, S, lane);
-            return;
-        }
-    }
-    if constexpr (NB > 1) {
-        if (nv < NB) {
-            dispatch_f64<MB, NB - 1, NW>(mv, nv, W, cw, P, Kpc, Np, Kb, Ke, b0, a0, s, S, lane);
-            return;
-        }
-    }
-    wave_f64<MB, NB, NW>(W, cw, P, Kpc, Np, Kb, Ke, b0, a0, s, S, lane);
-}
-
-// Persistent form: the grid is exactly the resident workgroup count; each
-// workgroup pulls (tile, split) units from the counter of its XCD (block
-// index mod 8) until the XCD's units run out.  Unit u of XCD x is split
-// x + 8*(u / T), tile u % T, so the tiles of a split run together on one XCD.
-// counters[0..7] are zeroed by k_weights, launched just before on the stream.
-template <int MB, int NB, int NW>
-__global__ __launch_bounds__(256) void k_contract_f64(
-    const double2* __restrict__ W, const double2* __restrict__ cw, double2* __restrict__ P,
-    int64_t Kp, int Np, int64_t Ks, int S, int tiles_a, int T, int* __restrict__ counters)
-{
-    const int xcd = blockIdx.x & 7;
-    const int units = (S / 8) * T;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    __shared__ int s_unit;
-    for (;;) {
-        if (threadIdx.x == 0) s_unit = atomicAdd(&counters[xcd], 1);
-        __syncthreads();
-        const int u = s_unit;
-        __syncthreads();
-        if (u >= units) break;
-        const int tile = u % T, s = xcd + 8 * (u / T);
-        const int tb = tile / tiles_a, ta = tile % tiles_a;
-        const int b0 = (tb * 2 + (wave >> 1)) * MB * 16;
-        const int a0 = (ta * 2 + (wave & 1)) * NB * 16;
-        const int mv = min(MB, (Np - b0) / 16), nv = min(NB, (Np - a0) / 16);
-        if (mv <= 0 || nv <= 0) continue;
-        const int64_t Kb = min((int64_t)s * Ks, Kp);
-        const int64_t Ke = min(Kp, Kb + Ks);
-        dispatch_f64<MB, NB, NW>(mv, nv, W, cw, P, Kp + kRowPad, Np, Kb, Ke, b0, a0, s, S, lane);
-    }
-}
-
-// FP32 variant: W in complex64, weights rounded to FP32 after being formed
-// in FP64, FP32 MFMA over K-tiles of kF32Tile, K-tile partials added into
-// FP64 accumulators.
-constexpr int kF32Tile = 256;
-
-template <int MB, int NB, int NW>
-__device__ __forceinline__ void wave_f32(const float2* __restrict__ W,
-                                         const double2* __restrict__ cw,
-                                         double2* __restrict__ P, int64_t Kpc, int Np,
-                                         int64_t Kb, int64_t Ke, int b0, int a0, int s, int S,
-                                         int lane)
-{
-    const int i = lane & 15, kk = lane >> 4;
-    v4d dr[NW][MB][NB], di[NW][MB][NB];
-#pragma unroll
-    for (int w = 0; w < NW; ++w)
-#pragma unroll
-        for (int m = 0; m < MB; ++m)
-#pragma unroll
-            for (int n = 0; n < NB; ++n) {
-                dr[w][m][n] = (v4d){0, 0, 0, 0};
-                di[w][m][n] = (v4d){0, 0, 0, 0};
-            }
-    const int64_t rstep = (int64_t)kKStep * Np;
-    const float2* pa = W + (Kb + kk) * Np + b0 + i;
-    const float2* px = W + (Kb + kk) * Np + a0 + i;
-    const double2* pc = cw + Kb + kk;
-    float2 A0[MB], X0[NB], A1[MB], X1[NB];
-    double2 C0[NW], C1[NW];
-    cload<MB, NB, NW>(pa, px, pc, Kpc, A0, X0, C0);
-    const MfmaF32 mf;
-    for (int64_t T0 = Kb; T0 < Ke; T0 += kF32Tile) {
-        const int npairs = (int)((min(Ke, T0 + kF32Tile) - T0) / (2 * kKStep));
-        v4f cr[NW][MB][NB], ci[NW][MB][NB];
-#pragma unroll
-        for (int w = 0; w < NW; ++w)
-#pragma unroll
-            for (int m = 0; m < MB; ++m)
-#pragma unroll
-                for (int n = 0; n < NB; ++n) {
-                    cr[w][m][n] = (v4f){0, 0, 0, 0};
-                    ci[w][m][n] = (v4f){0, 0, 0, 0};
-                }
-        for (int st = 0; st < npairs; ++st) {
-            cload<MB, NB, NW>(pa + rstep, px + rstep, pc + kKStep, Kpc, A1, X1, C1);
-            __builtin_amdgcn_sched_barrier(0);
-            cstep<MB, NB, NW, float>(A0, X0, C0, cr, ci, mf);
-            __builtin_amdgcn_sched_barrier(0);
-            pa += 2 * rstep;
-            px += 2 * rstep;
-            pc += 2 * kKStep;
-            cload<MB, NB, NW>(pa, px, pc, Kpc, A0, X0, C0);
-            __builtin_amdgcn_sched_barrier(0);
-            cstep<MB, NB, NW, float>(A1, X1, C1, cr, ci, mf);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-#pragma unroll
-        for (int w = 0; w < NW; ++w)
-#pragma unroll
-            for (int m = 0; m < MB; ++m)
-#pragma unroll
-                for (int n = 0; n < NB; ++n)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        dr[w][m][n][v] += (double)cr[w][m][n][v];
-                        di[w][m][n][v] += (double)ci[w][m][n][v];
-                    }
-    }
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-        double2* Pw = P + ((int64_t)w * S + s) * (int64_t)Np * Np;
-#pragma unroll
-        for (int m = 0; m < MB; ++m)
-#pragma unroll
-            for (int n = 0; n < NB; ++n) {
-                double2* dst = Pw + (int64_t)(a0 + 16 * n + i) * Np + b0 + 16 * m + 4 * kk;
-#pragma unroll
-                for (int v = 0; v < 4; ++v) dst[v] = make_double2(dr[w][m][n][v], di[w][m][n][v]);
-            }
-    }
-}
-
-template <int MB, int NB, int NW>
-__device__ __forceinline__ void dispatch_f32(int mv, int nv, const float2* W, const double2* cw,
-                                             double2* P, int64_t Kpc, int Np, int64_t Kb,
-                                             int64_t Ke, int b0, int a0, int s, int S, int lane)
-{
-    if constexpr (MB > 1) {
-        if (mv < MB) {
-            dispatch_f32<MB - 1, NB, NW>(mv, nv, W, cw, P, Kpc, Np, Kb, Ke, b0, a0, s, S, lane);
-            return;
-        }
-    }
-    if constexpr (NB > 1) {
-        if (nv < NB) {
-            dispatch_f32<MB, NB - 1, NW>(mv, nv, W, cw, P, Kpc, Np, Kb, Ke, b0, a0, s, S, lane);
-            return;
-        }
-    }
-    wave_f32<MB, NB, NW>(W, cw, P, Kpc, Np, Kb, Ke, b0, a0, s, S, lane);
-}
-
-template <int MB, int NB, int NW>
-__global__ __launch_bounds__(256) void k_contract_f32(
-    const float2* __restrict__ W, const double2* __restrict__ cw, double2* __restrict__ P,
-    int64_t Kp, int Np, int64_t Ks, int S, int tiles_a, int T, int* __restrict__ counters)
-{
-    const int xcd = blockIdx.x & 7;
-    const int units = (S / 8) * T;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    __shared__ int s_unit;
-    for (;;) {
-        if (threadIdx.x == 0) s_unit = atomicAdd(&counters[xcd], 1);
-        __syncthreads();
-        const int u = s_unit;
-        __syncthreads();
-        if (u >= units) break;
-        const int tile = u % T, s = xcd + 8 * (u / T);
-        const int tb = tile / tiles_a, ta = tile % tiles_a;
-        const int b0 = (tb * 2 + (wave >> 1)) * MB * 16;
-        const int a0 = (ta * 2 + (wave & 1)) * NB * 16;
-        const int mv = min(MB, (Np - b0) / 16), nv = min(NB, (Np - a0) / 16);
-        if (mv <= 0 || nv <= 0) continue;
-        const int64_t Kb = min((int64_t)s * Ks, Kp);
-        const int64_t Ke = min(Kp, Kb + Ks);
-        dispatch_f32<MB, NB, NW>(mv, nv, W, cw, P, Kp + kRowPad, Np, Kb, Ke, b0, a0, s, S, lane);
-    }
-}
-
-// Wave-tile shapes (MB x NB blocks) per precision and omega-batch size.
-template <int NW> struct F64Shape;
-#ifndef KLSP_F64_MB1
-#define KLSP_F64_MB1 4
-#define KLSP_F64_NB1 2
-#endif
-template <> struct F64Shape<1> { static constexpr int MB = KLSP_F64_MB1, NB = KLSP_F64_NB1; };
-template <> struct F64Shape<2> { static constexpr int MB = 2, NB = 4; };
-template <> struct F64Shape<3> { static constexpr int MB = 2, NB = 2; };
-template <int NW> struct F32Shape;
-template <> struct F32Shape<1> { static constexpr int MB = 4, NB = 2; };
-template <> struct F32Shape<2> { static constexpr int MB = 2, NB = 2; };
 
 // Sum of the S K-split partials (fixed order, FP64), then *G and *N[alpha]
 // (MatrixBuilder.jl:144-149).  M is [nb][N, N] column-major.
@@ -754,8 +406,6 @@ struct klsp_ctx {
     double2* d_A = nullptr;   size_t A_cap = 0;
     double2* d_det = nullptr;
     int last_S = 0;           // K-splits of the last contraction launch
-    int* d_counters = nullptr;  // per-XCD work counters of the persistent contraction
-    std::vector<std::pair<const void*, int>> occ_cache;
     // profiling
     bool prof = false;
     hipEvent_t ev[6] = {};
@@ -800,37 +450,42 @@ static int grid_for(int64_t n, int block = 256)
 }
 
 // K-split count and K per split for the contraction.
-struct SplitPlan {
-    int tiles_a, T, S;
-    int64_t Ks;
+struct GangPlan {
+    int wrows, wcols, gangs_per_xcd, n_gangs;
+    int64_t Kg;
 };
 
-// Output tiles of (32*MB) x (32*NB) and S K-splits (S % 8 == 0, K per split
-// a multiple of kF32Tile), sized so the persistent grid sees ~6 units per
-// resident workgroup (the tail of the last units stays short).
-static SplitPlan plan_splits(const klsp_ctx* ctx, int MB, int NB)
+// Gang geometry for a contraction shape: wave tiles of MB x NB blocks cover
+// the Np/16 x Np/16 block grid; as many gangs per XCD as its resident wave
+// slots hold; K per gang a multiple of kF32Tile (so also of the 8-row pair).
+static GangPlan plan_gangs(const klsp_ctx* ctx, const ContractShape& sh)
 {
-    SplitPlan sp;
-    const int tiles_b = (int)((ctx->Np + 32 * MB - 1) / (32 * MB));
-    sp.tiles_a = (int)((ctx->Np + 32 * NB - 1) / (32 * NB));
-    sp.T = tiles_b * sp.tiles_a;
-    const int64_t target_units = (int64_t)ctx->n_cu * 2 * 6;
-    int64_t S = (target_units + sp.T - 1) / sp.T;
-    S = std::min<int64_t>(S, std::max<int64_t>(1, ctx->Kp / kF32Tile));
-    S = round_up(std::max<int64_t>(S, 1), 8);
-    sp.Ks = round_up((ctx->Kp + S - 1) / S, kF32Tile);
-    const int64_t used = (ctx->Kp + sp.Ks - 1) / sp.Ks;
-    sp.S = (int)round_up(std::max<int64_t>(used, 1), 8);
-    return sp;
+    GangPlan gp;
+    const int nb = ctx->Np / 16;
+    gp.wrows = (nb + sh.MB - 1) / sh.MB;
+    gp.wcols = (nb + sh.NB - 1) / sh.NB;
+    const int gw = gp.wrows * gp.wcols;
+    const int slots_per_xcd = std::max(1, ctx->n_cu / 8) * std::max(1, sh.waves_per_cu);
+    gp.gangs_per_xcd = std::max(1, slots_per_xcd / gw);
+    const int64_t max_gangs = std::max<int64_t>(1, ctx->Kp / kF32Tile);
+    while (gp.gangs_per_xcd > 1 && (int64_t)gp.gangs_per_xcd * 8 > max_gangs) --gp.gangs_per_xcd;
+    gp.n_gangs = 8 * gp.gangs_per_xcd;
+    gp.Kg = round_up((ctx->Kp + gp.n_gangs - 1) / gp.n_gangs, kF32Tile);
+    return gp;
 }
 
-// Largest split count any shape can ask for (sizes the partials buffer).
-static int max_splits(const klsp_ctx* ctx)
+static ContractShape shape_for(int precision, int nw)
 {
-    int S = 0;
-    const int shapes[][2] = {{4, 4}, {2, 4}, {2, 2}, {4, 2}};
-    for (auto& sh : shapes) S = std::max(S, plan_splits(ctx, sh[0], sh[1]).S);
-    return S;
+    return precision == KLSP_PREC_FP64 ? contract_f64_shape(nw) : contract_f32_shape(nw);
+}
+
+// Largest gang count of any launch shape (sizes the partials buffer).
+static int max_gangs(const klsp_ctx* ctx)
+{
+    int g = 8;
+    for (int prec : {KLSP_PREC_FP64, KLSP_PREC_FP32})
+        for (int nw = 1; nw <= kMaxBatch; ++nw) g = std::max(g, plan_gangs(ctx, shape_for(prec, nw)).n_gangs);
+    return g;
 }
 
 static int free_problem(klsp_ctx* ctx)
@@ -876,8 +531,7 @@ int klsp_create(int device, klsp_ctx** ctx_out)
     }
     for (auto& e : ctx->ev) hipEventCreate(&e);
     if (hipMalloc((void**)&ctx->d_om, sizeof(double) * 2 * kMaxBatch) != hipSuccess ||
-        hipMalloc((void**)&ctx->d_det, sizeof(double2) * kMaxBatch) != hipSuccess ||
-        hipMalloc((void**)&ctx->d_counters, sizeof(int) * 8) != hipSuccess) {
+        hipMalloc((void**)&ctx->d_det, sizeof(double2) * kMaxBatch) != hipSuccess) {
         delete ctx;
         return KLSP_ERR_OOM;
     }
@@ -892,7 +546,7 @@ int klsp_destroy(klsp_ctx* ctx)
     hipStreamSynchronize(ctx->stream);
     free_problem(ctx);
     hipFree(ctx->d_om); hipFree(ctx->d_c); hipFree(ctx->d_P); hipFree(ctx->d_M);
-    hipFree(ctx->d_A); hipFree(ctx->d_det); hipFree(ctx->d_counters);
+    hipFree(ctx->d_A); hipFree(ctx->d_det);
     for (auto& e : ctx->ev) hipEventDestroy(e);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1082,43 +736,26 @@ int klsp_prepare(klsp_ctx* ctx, int precision)
 
 }  // extern "C"
 
-// Resident workgroups of a 256-thread kernel on the whole device (a multiple
-// of 8, so every XCD counter has workgroups pulling from it).
-static int resident_grid(klsp_ctx* ctx, const void* kernel)
+// Launches the contraction for nb omegas (weights already in ctx->d_c).
+static int launch_contract(klsp_ctx* ctx, int precision, int nb)
 {
-    for (auto& e : ctx->occ_cache)
-        if (e.first == kernel) return e.second;
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess ||
-        per_cu < 1)
-        per_cu = 1;
-    const int g = (int)round_up((int64_t)ctx->n_cu * per_cu, 8);
-    ctx->occ_cache.emplace_back(kernel, g);
-    return g;
-}
-
-template <int NW>
-static void launch_f64(klsp_ctx* ctx)
-{
-    constexpr int MB = F64Shape<NW>::MB, NB = F64Shape<NW>::NB;
-    const SplitPlan sp = plan_splits(ctx, MB, NB);
-    const void* kfn = (const void*)k_contract_f64<MB, NB, NW>;
-    hipLaunchKernelGGL((k_contract_f64<MB, NB, NW>), dim3((unsigned)resident_grid(ctx, kfn)),
-                       dim3(256), 0, ctx->stream, ctx->d_W64, ctx->d_c, ctx->d_P, ctx->Kp, ctx->Np,
-                       sp.Ks, sp.S, sp.tiles_a, sp.T, ctx->d_counters);
-    ctx->last_S = sp.S;
-}
-
-template <int NW>
-static void launch_f32(klsp_ctx* ctx)
-{
-    constexpr int MB = F32Shape<NW>::MB, NB = F32Shape<NW>::NB;
-    const SplitPlan sp = plan_splits(ctx, MB, NB);
-    const void* kfn = (const void*)k_contract_f32<MB, NB, NW>;
-    hipLaunchKernelGGL((k_contract_f32<MB, NB, NW>), dim3((unsigned)resident_grid(ctx, kfn)),
-                       dim3(256), 0, ctx->stream, ctx->d_W32, ctx->d_c, ctx->d_P, ctx->Kp, ctx->Np,
-                       sp.Ks, sp.S, sp.tiles_a, sp.T, ctx->d_counters);
-    ctx->last_S = sp.S;
+    const GangPlan gp = plan_gangs(ctx, shape_for(precision, nb));
+    ContractArgs a;
+    a.W = precision == KLSP_PREC_FP64 ? (const void*)ctx->d_W64 : (const void*)ctx->d_W32;
+    a.cw = ctx->d_c;
+    a.P = ctx->d_P;
+    a.Kp = ctx->Kp;
+    a.Np = ctx->Np;
+    a.Kg = gp.Kg;
+    a.n_gangs = gp.n_gangs;
+    a.wrows = gp.wrows;
+    a.wcols = gp.wcols;
+    const hipError_t e = precision == KLSP_PREC_FP64 ? contract_f64_launch(nb, a, ctx->stream)
+                                                     : contract_f32_launch(nb, a, ctx->stream);
+    if (e != hipSuccess)
+        return fail(ctx, KLSP_ERR_HIP, std::string("contraction launch: ") + hipGetErrorString(e));
+    ctx->last_S = gp.n_gangs;
+    return KLSP_OK;
 }
 
 // Builds M for nb <= kMaxBatch omegas into ctx->d_M ([nb][N,N]).
@@ -1127,8 +764,8 @@ static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision)
     const int64_t Kpc = ctx->Kp + kRowPad;
     int rc;
     if ((rc = ensure(ctx, ctx->d_c, ctx->c_cap, (size_t)kMaxBatch * Kpc))) return rc;
-    if ((rc = ensure(ctx, ctx->d_P, ctx->P_cap,
-                     (size_t)kMaxBatch * max_splits(ctx) * ctx->Np * ctx->Np)))
+    const int mg = max_gangs(ctx);
+    if ((rc = ensure(ctx, ctx->d_P, ctx->P_cap, (size_t)kMaxBatch * mg * ctx->Np * ctx->Np)))
         return rc;
     if ((rc = ensure(ctx, ctx->d_M, ctx->M_cap, (size_t)kMaxBatch * ctx->N * ctx->N))) return rc;
     KLSP_HIP(ctx, hipMemcpyAsync(ctx->d_om, om, sizeof(double) * 2 * nb, hipMemcpyHostToDevice,
@@ -1136,16 +773,9 @@ static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision)
     if (ctx->prof) hipEventRecord(ctx->ev[0], ctx->stream);
     hipLaunchKernelGGL(k_weights, dim3(grid_for((int64_t)nb * Kpc)), dim3(256), 0, ctx->stream,
                        ctx->d_DJ, ctx->d_F0l, ctx->d_Ores, ctx->NPh, ctx->K, Kpc, ctx->d_om, nb,
-                       ctx->d_c, ctx->d_counters);
+                       ctx->d_c);
     if (ctx->prof) hipEventRecord(ctx->ev[1], ctx->stream);
-    if (precision == KLSP_PREC_FP64) {
-        if (nb == 1) launch_f64<1>(ctx);
-        else if (nb == 2) launch_f64<2>(ctx);
-        else launch_f64<3>(ctx);
-    } else {
-        if (nb == 1) launch_f32<1>(ctx);
-        else launch_f32<2>(ctx);
-    }
+    if ((rc = launch_contract(ctx, precision, nb))) return rc;
     if (ctx->prof) hipEventRecord(ctx->ev[2], ctx->stream);
     hipLaunchKernelGGL(k_reduce, dim3(grid_for((int64_t)nb * ctx->N * ctx->N)), dim3(256), 0,
                        ctx->stream, ctx->d_P, ctx->last_S, ctx->Np, (int)ctx->N, nb, ctx->G,
@@ -1189,7 +819,7 @@ static int det_many(klsp_ctx* ctx, const double* om, int64_t n, int precision, d
     if (rc) return rc;
     KLSP_HIP(ctx, hipSetDevice(ctx->device));
     if ((rc = ensure(ctx, ctx->d_A, ctx->A_cap, (size_t)kMaxBatch * ctx->N * ctx->N))) return rc;
-    const int maxb = precision == KLSP_PREC_FP64 ? 3 : 2;
+    const int maxb = kMaxBatch;
     for (int64_t i0 = 0; i0 < n; i0 += maxb) {
         const int nb = (int)std::min<int64_t>(maxb, n - i0);
         if ((rc = build_batch(ctx, om + 2 * i0, nb, precision))) return rc;
