@@ -819,7 +819,9 @@ static int det_many(klsp_ctx* ctx, const double* om, int64_t n, int precision, d
     if (rc) return rc;
     KLSP_HIP(ctx, hipSetDevice(ctx->device));
     if ((rc = ensure(ctx, ctx->d_A, ctx->A_cap, (size_t)kMaxBatch * ctx->N * ctx->N))) return rc;
-    const int maxb = kMaxBatch;
+    // FP64 is MFMA-bound at one omega per pass (larger batches only lower the
+    // occupancy); the FP32 pass is memory-bound, so it shares W loads.
+    const int maxb = precision == KLSP_PREC_FP64 ? 1 : kMaxBatch;
     for (int64_t i0 = 0; i0 < n; i0 += maxb) {
         const int nb = (int)std::min<int64_t>(maxb, n - i0);
         if ((rc = build_batch(ctx, om + 2 * i0, nb, precision))) return rc;
