@@ -8,8 +8,10 @@ det(I - M(omega)) evaluation: resonance weights, the K-long complex
 contraction (FP64 MFMA), the G*N epilogue and the FP64 complex LU, with W
 resident in HBM.  Also reported: Newton roots/s (FP64, from
 omega0 = 0.8794 + 0.1231i), the complex64/FP32-MFMA det rate, the W
-quadrature time, and the CPU baseline (numpy/OpenBLAS BLAS-form det on a
-bounded sample of l-slabs, extrapolated to the full l range).
+quadrature time, the mixed-precision Newton search (FP32 Taylor-model passes,
+FP64 finish) with its distance to the FP64 root, and the CPU baseline
+(numpy/OpenBLAS BLAS-form det: one det's worth of l-slab ZGEMMs over 8
+synthetic slabs plus one LU).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -31,6 +33,11 @@ METRIC = ("det(I−M(ω)) evals/s + Newton roots/s, mixed vs FP64 GPU at "
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch of the FP64 contraction at the default grid, from two
+# rocprofv3 --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, calibrated on a
+# copy kernel of known size); profiles/det_default_grid.md holds the run.
+PROFILED_KERNEL = "klsp::k_contract<true, 4, 2, 1>"
+PROFILED_TRAFFIC_BYTES = 232.374e9 + 0.059e9
 
 
 def log(*a):
@@ -43,8 +50,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="default", help="configs/<name>.toml")
-    ap.add_argument("--cpu-slabs", type=int, default=4,
-                    help="l-slabs timed for the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-slabs", type=int, default=-1,
+                    help="l-slab ZGEMMs timed for the CPU baseline (-1 = all n_l, 0 = skip)")
     ap.add_argument("--no-newton", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     args = ap.parse_args()
@@ -144,8 +151,34 @@ def main() -> int:
                       "root_gamma": r.gamma, "root_abs_det": abs(r.det_val)})
         log(f"[bench] FP64 Newton: {tn:.2f} s, {r.iterations} iterations, {r.n_det_evals} dets, "
             f"root Omega_p={r.Omega_p:.12f} gamma={r.gamma:.12f} |det|={abs(r.det_val):.2e}")
+        w64 = r.omega
+        # mixed precision (the FP32 copy of W was made by the FP32 leg or here)
+        ctx.prepare(_ffi.PREC_FP32)
+        t0 = time.perf_counter()
+        rm = K.newton_search(cfg, pre, dev, K.GPUType.AMD, np.float64, 0.4397, 0.1231,
+                             precision="mixed")
+        tm = time.perf_counter() - t0
+        tmx = ctx.last_timing()
+        dw = abs(rm.omega - w64)
+        extra.update({"newton_roots_per_s_mixed": (1.0 / tm) if dw <= 1e-10 else 0.0,
+                      "newton_s_mixed": tm, "newton_mixed_speedup": tn / tm,
+                      "newton_mixed_lp_builds": rm.n_lp_builds,
+                      "newton_mixed_fp64_builds": rm.n_fp64_builds,
+                      "newton_mixed_converged": rm.converged,
+                      "newton_mixed_abs_domega_vs_fp64": dw,
+                      "newton_mixed_dOmega_p": rm.Omega_p - r.Omega_p,
+                      "newton_mixed_dgamma": rm.gamma - r.gamma,
+                      "newton_mixed_abs_det": abs(rm.det_val),
+                      "newton_mixed_contract_ms": tmx["contract_ms"],
+                      "newton_mixed_lu_ms": tmx["lu_ms"]})
+        log(f"[bench] mixed Newton: {tm:.3f} s ({tn / tm:.2f}x FP64), {rm.n_lp_builds} FP32 "
+            f"Taylor passes + {rm.n_fp64_builds} FP64 dets, |d omega| vs FP64 root {dw:.2e}, "
+            f"|det|={abs(rm.det_val):.2e}; contractions {tmx['contract_ms']:.1f} ms, "
+            f"LUs {tmx['lu_ms']:.1f} ms")
 
     cpu = None
+    if args.cpu_slabs < 0:
+        args.cpu_slabs = n_l
     if args.cpu_slabs > 0:
         from oracle import blas_form
         try:
@@ -155,12 +188,13 @@ def main() -> int:
         except Exception:  # pragma: no cover
             cores = os.cpu_count()
         t0 = time.perf_counter()
-        s_det, s_slab, s_lu = blas_form.time_sample(NPh, N, n_l, args.cpu_slabs, repeats=2)
+        s_det, s_slab, s_lu = blas_form.time_sample(NPh, N, n_l, args.cpu_slabs)
         log(f"[bench] CPU BLAS-form: {s_slab * 1e3:.1f} ms per l-slab, LU {s_lu * 1e3:.2f} ms -> "
             f"{s_det:.2f} s per det ({time.perf_counter() - t0:.1f} s of CPU work)")
         cpu = {"value": 1.0 / s_det, "unit": "det/s", "cores": int(cores), "kind": "port",
-               "sample": f"numpy/OpenBLAS per-l ZGEMM W_l^T (c*conj W_l) on {args.cpu_slabs} of "
-                         f"{n_l} synthetic [{NPh}x{N}] l-slabs + one LU, extrapolated to {n_l} slabs"}
+               "sample": f"numpy/OpenBLAS BLAS-form det: {args.cpu_slabs} per-l ZGEMMs "
+                         f"W_l^T (c*conj W_l) over 8 synthetic [{NPh}x{N}] complex128 l-slabs + "
+                         f"one {N}x{N} LU, scaled to {n_l} slabs per det"}
 
     out = {
         "metric": METRIC, "value": det_per_s, "unit": "det/s", "n_gpus": 1, "steps": args.steps,
@@ -173,8 +207,10 @@ def main() -> int:
                    "q1": cfg.model.q1},
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS,
-                     "traffic": None, "kernel": "k_contract_f64<1>",
-                     "flop_per_launch": flop_build},
+                     "traffic": PROFILED_TRAFFIC_BYTES if args.config == "default" else None,
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE)",
+                     "kernel": PROFILED_KERNEL, "flop_per_launch": flop_build,
+                     "algorithmic_bytes_per_launch": 16.0 * Kdim * N},
         "cpu_baseline": cpu,
         "extra": extra,
     }

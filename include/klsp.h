@@ -47,7 +47,10 @@ enum klsp_status {
 /* Arithmetic of an M(omega) build.  The determinant (LU) is FP64 always. */
 enum klsp_precision {
     KLSP_PREC_FP64 = 0,  /* W complex128, FP64 MFMA contraction            */
-    KLSP_PREC_FP32 = 1   /* W complex64, FP32 MFMA with FP64 K-tile sums   */
+    KLSP_PREC_FP32 = 1,  /* W complex64, FP32 MFMA with FP64 K-tile sums   */
+    KLSP_PREC_MIXED = 2  /* klsp_newton only: FP32 Taylor-model passes until
+                            the step is below 1e-6, then FP64 dets (see
+                            klsp_newton)                                   */
 };
 
 typedef struct klsp_ctx klsp_ctx;
@@ -183,6 +186,15 @@ int klsp_prepare(klsp_ctx* ctx, int precision);
 int klsp_build_M(klsp_ctx* ctx, double omega_re, double omega_im, int precision,
                  double* M_out);
 
+/* Taylor terms of M around a centre omega_c (no counterpart in the
+ * reference; the building block of the KLSP_PREC_MIXED Newton search):
+ * T_out is [n_terms][N, N] complex, T_j = G*N[a]*sum W c/d^j conj W with
+ * c = DJ*F0l and d = omega_c - Omega_res (j = 1..n_terms, 1e-12 clamp), so
+ * M(omega_c + s) = sum_j (-s)^(j-1) T_j for |s| < Im omega_c and T_1 =
+ * M(omega_c).  1 <= n_terms <= 3. */
+int klsp_build_taylor(klsp_ctx* ctx, double omega_re, double omega_im, int n_terms,
+                      int precision, double* T_out);
+
 /* compute_determinant / compute_determinant_batched, MatrixBuilder.jl:197,
  * :215: det(I - M(omega_i)) for n values of omega (interleaved complex),
  * FP64 complex LU with partial pivoting on the device. */
@@ -207,11 +219,29 @@ typedef struct klsp_newton_result {
     double det[2];         /* det at the final (Omega_p, gamma)             */
     int iterations;
     int converged;
-    int n_det_evals;       /* number of det(I - M) evaluations performed    */
+    int n_det_evals;       /* number of M(omega) builds performed           */
+    int n_fp64_builds;     /* ... of which FP64 single-omega builds         */
+    int n_lp_builds;       /* ... of which FP32 builds (MIXED: Taylor passes) */
 } klsp_newton_result;
 
 /* history_out: [max_iter, 3] column-major (Omega_p, gamma, |det|); the first
- * `iterations` rows are filled.  May be NULL. */
+ * min(iterations, max_iter) rows are filled.  May be NULL.
+ *
+ * precision FP64 / FP32: the reference iteration, every det in that
+ * precision.  precision MIXED (no counterpart in the reference; it returns
+ * the same root to |d omega| <= 1e-10 at a fraction of the FP64 builds):
+ *   1. low-precision passes: at a centre w_c one FP32 contraction builds the
+ *      Taylor terms T_j = G N sum W c/d^j conj W (d = w_c - Omega_res,
+ *      j = 1..3), so M(w_c + s) = T_1 - s T_2 + s^2 T_3 + O(s^3); Newton on
+ *      det(I - M) of that model (FP64 LUs only) within |s| <= Im(w_c)/4
+ *      moves the centre; passes repeat until the move is below 1e-6;
+ *   2. FP64 finish: FP64 dets at the low-precision root and after each
+ *      step, the first step with the model's derivative, later ones with
+ *      the FP64 secant slope, until |det| < tol after at least one such
+ *      step (max_iter steps at most).
+ * iterations = low-precision passes + FP64 steps; history rows record each
+ * centre with its low-precision |det|, then each FP64 point with its FP64
+ * |det|.  Bounds (gamma >= 1e-4, Omega_p in [0, 1]) apply to every move. */
 int klsp_newton(klsp_ctx* ctx, const klsp_newton_params* np, klsp_newton_result* res,
                 double* history_out);
 

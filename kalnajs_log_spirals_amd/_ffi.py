@@ -23,6 +23,7 @@ KLSP_ERR_NO_DEVICE = 5
 
 PREC_FP64 = 0
 PREC_FP32 = 1
+PREC_MIXED = 2
 
 _STATUS_NAMES = {
     KLSP_ERR_INVALID_ARG: "invalid argument",
@@ -38,7 +39,8 @@ EXPORTED = [
     "klsp_trapezoid_coef", "klsp_simpson_coef", "klsp_linrange", "klsp_gamma_complex",
     "klsp_compute_N_kernel", "klsp_compute_grids", "klsp_compute_orbits",
     "klsp_precompute_arrays", "klsp_upload_problem", "klsp_compute_W", "klsp_upload_W",
-    "klsp_download_W", "klsp_prepare", "klsp_build_M", "klsp_det", "klsp_newton",
+    "klsp_download_W", "klsp_prepare", "klsp_build_M", "klsp_build_taylor", "klsp_det",
+    "klsp_newton",
     "klsp_grid_scan", "klsp_set_profiling", "klsp_last_timing", "klsp_last_W_ms",
 ]
 
@@ -78,7 +80,8 @@ class klsp_newton_params(ct.Structure):
 class klsp_newton_result(ct.Structure):
     _fields_ = [("Omega_p", ct.c_double), ("gamma", ct.c_double),
                 ("omega", ct.c_double * 2), ("det", ct.c_double * 2),
-                ("iterations", ct.c_int), ("converged", ct.c_int), ("n_det_evals", ct.c_int)]
+                ("iterations", ct.c_int), ("converged", ct.c_int), ("n_det_evals", ct.c_int),
+                ("n_fp64_builds", ct.c_int), ("n_lp_builds", ct.c_int)]
 
 
 class klsp_scan_params(ct.Structure):
@@ -129,6 +132,7 @@ def lib() -> ct.CDLL:
     L.klsp_download_W.argtypes = [P, _D]
     L.klsp_prepare.argtypes = [P, ct.c_int]
     L.klsp_build_M.argtypes = [P, ct.c_double, ct.c_double, ct.c_int, _D]
+    L.klsp_build_taylor.argtypes = [P, ct.c_double, ct.c_double, ct.c_int, ct.c_int, _D]
     L.klsp_det.argtypes = [P, _D, i64, ct.c_int, _D]
     L.klsp_newton.argtypes = [P, ct.POINTER(klsp_newton_params),
                               ct.POINTER(klsp_newton_result), _D]
@@ -255,6 +259,17 @@ class Context:
                                      precision, dptr(out)))
         return out.view(np.complex128).reshape((N, N), order="F")
 
+    def build_taylor(self, omega: complex, n_terms: int = 3,
+                     precision: int = PREC_FP64) -> np.ndarray:
+        """[n_terms, N, N] Taylor terms T_j of M around omega (klsp_build_taylor)."""
+        N = self.dims[1]
+        out = np.empty(2 * n_terms * N * N)
+        self._chk(lib().klsp_build_taylor(self.handle, float(np.real(omega)),
+                                          float(np.imag(omega)), int(n_terms), precision,
+                                          dptr(out)))
+        return np.stack([out[2 * j * N * N:2 * (j + 1) * N * N].view(np.complex128)
+                         .reshape((N, N), order="F") for j in range(n_terms)])
+
     def det(self, omegas, precision: int = PREC_FP64) -> np.ndarray:
         om = np.ascontiguousarray(np.atleast_1d(np.asarray(omegas, dtype=np.complex128)))
         out = np.empty(om.size, dtype=np.complex128)
@@ -270,7 +285,7 @@ class Context:
         res = klsp_newton_result()
         hist = np.zeros(3 * max(int(max_iter), 1))
         self._chk(lib().klsp_newton(self.handle, ct.byref(npar), ct.byref(res), dptr(hist)))
-        n = res.iterations
+        n = min(res.iterations, max(int(max_iter), 1))
         H = hist.reshape((max(int(max_iter), 1), 3), order="F")[:n].copy()
         return res, H
 

@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -251,6 +252,31 @@ __global__ void k_weights(const double* __restrict__ DJ, const double* __restric
     }
 }
 
+// Taylor weights of M around a centre w_c: c[j][K] = DJ*F0l / d^(j+1),
+// d = w_c - Omega_res (same clamp), j = 0..nt-1, so that
+// M(w_c + s) = sum_j (-s)^j M_j for |s| < min |d| (>= Im w_c).
+__global__ void k_weights_taylor(const double* __restrict__ DJ, const double* __restrict__ F0l,
+                                 const double* __restrict__ Ores, int64_t NPh, int64_t K,
+                                 int64_t Kp, double om_re, double om_im, int nt,
+                                 double2* __restrict__ c)
+{
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < Kp;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        double2 r = make_double2(0.0, 0.0), iv = make_double2(0.0, 0.0);
+        if (q < K) {
+            double dr = om_re - Ores[q], di = om_im;
+            if (hypot(dr, di) < 1e-12) { dr = 1e-12; di = 1e-12; }
+            const double xw = DJ[q % NPh] * F0l[q];
+            iv = d_inv(dr, di);
+            r = make_double2(xw * iv.x, xw * iv.y);
+        }
+        for (int j = 0; j < nt; ++j) {
+            c[(int64_t)j * Kp + q] = r;
+            r = cmul(r, iv);
+        }
+    }
+}
+
 // =====================================================================
 // Stage 3: the contraction M_w = sum_K W^T diag(c_w) conj(W): see
 // klsp_contract.hpp (kernels in klsp_contract_f64.hip / klsp_contract_f32.hip).
@@ -292,14 +318,16 @@ __device__ __forceinline__ void argmax_merge(double& v, int& idx, double ov, int
     if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
 }
 
+// With coef == nullptr matrix b is I - M[b]; otherwise it is
+// I - sum_j coef[b*nt + j] M[j] (a point of the Taylor model of M).
 __global__ __launch_bounds__(1024) void k_lu_det(const double2* __restrict__ M,
+                                                  const double2* __restrict__ coef, int nt,
                                                   double2* __restrict__ Awork, int N,
                                                   double2* __restrict__ det_out)
 {
     const int b = blockIdx.x;
     const int tid = threadIdx.x, nth = blockDim.x;
     const int64_t NN = (int64_t)N * N;
-    const double2* Mb = M + b * NN;
     double2* A = Awork + b * NN;
     __shared__ double2 colj[kMaxN];
     __shared__ double2 rowj[kMaxN];
@@ -308,7 +336,16 @@ __global__ __launch_bounds__(1024) void k_lu_det(const double2* __restrict__ M,
     __shared__ int s_piv;
 
     for (int64_t x = tid; x < NN; x += nth) {
-        const double2 m = Mb[x];
+        double2 m;
+        if (coef == nullptr) {
+            m = M[b * NN + x];
+        } else {
+            m = make_double2(0.0, 0.0);
+            for (int j = 0; j < nt; ++j) {
+                const double2 t = cmul(coef[b * nt + j], M[j * NN + x]);
+                m = make_double2(m.x + t.x, m.y + t.y);
+            }
+        }
         const int r = (int)(x % N), c = (int)(x / N);
         A[x] = make_double2((r == c ? 1.0 : 0.0) - m.x, 0.0 - m.y);
     }
@@ -400,6 +437,7 @@ struct klsp_ctx {
     bool have_W = false;
     // workspaces
     double* d_om = nullptr;
+    double2* d_coef = nullptr;  // [kMaxBatch][kMaxBatch] Taylor-model coefficients
     double2* d_c = nullptr;   size_t c_cap = 0;
     double2* d_P = nullptr;   size_t P_cap = 0;
     double2* d_M = nullptr;   size_t M_cap = 0;
@@ -531,8 +569,9 @@ int klsp_create(int device, klsp_ctx** ctx_out)
     }
     for (auto& e : ctx->ev) hipEventCreate(&e);
     if (hipMalloc((void**)&ctx->d_om, sizeof(double) * 2 * kMaxBatch) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_coef, sizeof(double2) * kMaxBatch * kMaxBatch) != hipSuccess ||
         hipMalloc((void**)&ctx->d_det, sizeof(double2) * kMaxBatch) != hipSuccess) {
-        delete ctx;
+        klsp_destroy(ctx);
         return KLSP_ERR_OOM;
     }
     *ctx_out = ctx;
@@ -546,7 +585,7 @@ int klsp_destroy(klsp_ctx* ctx)
     hipStreamSynchronize(ctx->stream);
     free_problem(ctx);
     hipFree(ctx->d_om); hipFree(ctx->d_c); hipFree(ctx->d_P); hipFree(ctx->d_M);
-    hipFree(ctx->d_A); hipFree(ctx->d_det);
+    hipFree(ctx->d_A); hipFree(ctx->d_det); hipFree(ctx->d_coef);
     for (auto& e : ctx->ev) hipEventDestroy(e);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -758,8 +797,10 @@ static int launch_contract(klsp_ctx* ctx, int precision, int nb)
     return KLSP_OK;
 }
 
-// Builds M for nb <= kMaxBatch omegas into ctx->d_M ([nb][N,N]).
-static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision)
+// Builds M for nb <= kMaxBatch omegas into ctx->d_M ([nb][N,N]).  With
+// taylor set, om is one centre and the nb matrices are its Taylor terms.
+static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
+                       bool taylor = false)
 {
     const int64_t Kpc = ctx->Kp + kRowPad;
     int rc;
@@ -768,12 +809,18 @@ static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision)
     if ((rc = ensure(ctx, ctx->d_P, ctx->P_cap, (size_t)kMaxBatch * mg * ctx->Np * ctx->Np)))
         return rc;
     if ((rc = ensure(ctx, ctx->d_M, ctx->M_cap, (size_t)kMaxBatch * ctx->N * ctx->N))) return rc;
-    KLSP_HIP(ctx, hipMemcpyAsync(ctx->d_om, om, sizeof(double) * 2 * nb, hipMemcpyHostToDevice,
-                                 ctx->stream));
+    if (!taylor)
+        KLSP_HIP(ctx, hipMemcpyAsync(ctx->d_om, om, sizeof(double) * 2 * nb,
+                                     hipMemcpyHostToDevice, ctx->stream));
     if (ctx->prof) hipEventRecord(ctx->ev[0], ctx->stream);
-    hipLaunchKernelGGL(k_weights, dim3(grid_for((int64_t)nb * Kpc)), dim3(256), 0, ctx->stream,
-                       ctx->d_DJ, ctx->d_F0l, ctx->d_Ores, ctx->NPh, ctx->K, Kpc, ctx->d_om, nb,
-                       ctx->d_c);
+    if (taylor)
+        hipLaunchKernelGGL(k_weights_taylor, dim3(grid_for(Kpc)), dim3(256), 0, ctx->stream,
+                           ctx->d_DJ, ctx->d_F0l, ctx->d_Ores, ctx->NPh, ctx->K, Kpc, om[0],
+                           om[1], nb, ctx->d_c);
+    else
+        hipLaunchKernelGGL(k_weights, dim3(grid_for((int64_t)nb * Kpc)), dim3(256), 0,
+                           ctx->stream, ctx->d_DJ, ctx->d_F0l, ctx->d_Ores, ctx->NPh, ctx->K,
+                           Kpc, ctx->d_om, nb, ctx->d_c);
     if (ctx->prof) hipEventRecord(ctx->ev[1], ctx->stream);
     if ((rc = launch_contract(ctx, precision, nb))) return rc;
     if (ctx->prof) hipEventRecord(ctx->ev[2], ctx->stream);
@@ -825,8 +872,8 @@ static int det_many(klsp_ctx* ctx, const double* om, int64_t n, int precision, d
     for (int64_t i0 = 0; i0 < n; i0 += maxb) {
         const int nb = (int)std::min<int64_t>(maxb, n - i0);
         if ((rc = build_batch(ctx, om + 2 * i0, nb, precision))) return rc;
-        hipLaunchKernelGGL(k_lu_det, dim3(nb), dim3(1024), 0, ctx->stream, ctx->d_M, ctx->d_A,
-                           (int)ctx->N, ctx->d_det);
+        hipLaunchKernelGGL(k_lu_det, dim3(nb), dim3(1024), 0, ctx->stream, ctx->d_M,
+                           (const double2*)nullptr, 0, ctx->d_A, (int)ctx->N, ctx->d_det);
         if (ctx->prof) hipEventRecord(ctx->ev[4], ctx->stream);
         KLSP_HIP(ctx, hipGetLastError());
         KLSP_HIP(ctx, hipMemcpyAsync(out + 2 * i0, ctx->d_det, sizeof(double2) * nb,
@@ -838,6 +885,162 @@ static int det_many(klsp_ctx* ctx, const double* om, int64_t n, int precision, d
 }
 
 static void reset_timing(klsp_ctx* ctx) { ctx->last = klsp_timing{}; }
+
+// Timings of the inner calls of a driver, summed.
+static void acc_timing(klsp_timing& acc, klsp_ctx* ctx)
+{
+    acc.weights_ms += ctx->last.weights_ms;
+    acc.contract_ms += ctx->last.contract_ms;
+    acc.reduce_ms += ctx->last.reduce_ms;
+    acc.lu_ms += ctx->last.lu_ms;
+    acc.n_omega += ctx->last.n_omega;
+    acc.contract_launches += ctx->last.contract_launches;
+    ctx->last = klsp_timing{};
+}
+
+// dets of I - sum_j (-s_i)^j T_j at nd points s_i of the Taylor model that
+// the last Taylor build left in ctx->d_M (nt terms).
+static int model_dets(klsp_ctx* ctx, int nt, const std::complex<double>* s, int nd,
+                      std::complex<double>* dets)
+{
+    int rc;
+    if ((rc = ensure(ctx, ctx->d_A, ctx->A_cap, (size_t)kMaxBatch * ctx->N * ctx->N))) return rc;
+    double2 coef[kMaxBatch * kMaxBatch];
+    for (int i = 0; i < nd; ++i) {
+        std::complex<double> p = 1.0;
+        for (int j = 0; j < nt; ++j) {
+            coef[i * nt + j] = make_double2(p.real(), p.imag());
+            p *= -s[i];
+        }
+    }
+    KLSP_HIP(ctx, hipMemcpyAsync(ctx->d_coef, coef, sizeof(double2) * nd * nt,
+                                 hipMemcpyHostToDevice, ctx->stream));
+    if (ctx->prof) hipEventRecord(ctx->ev[3], ctx->stream);
+    hipLaunchKernelGGL(k_lu_det, dim3(nd), dim3(1024), 0, ctx->stream, ctx->d_M,
+                       (const double2*)ctx->d_coef, nt, ctx->d_A, (int)ctx->N, ctx->d_det);
+    if (ctx->prof) hipEventRecord(ctx->ev[4], ctx->stream);
+    KLSP_HIP(ctx, hipGetLastError());
+    KLSP_HIP(ctx, hipMemcpyAsync(dets, ctx->d_det, sizeof(double2) * nd, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->prof) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]);
+        ctx->last.lu_ms += ms;
+    }
+    return KLSP_OK;
+}
+
+// Mixed-precision Newton (KLSP_PREC_MIXED; documented at klsp_newton in
+// klsp.h).  omega = m*Omega_p + i*gamma; det(I - M(omega)) is holomorphic in
+// omega, so the step is the complex Newton step -D/D'.
+static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, klsp_newton_result* res,
+                        double* history)
+{
+    typedef std::complex<double> cd;
+    const double m = (double)ctx->m;
+    const int nt = 3;                   // Taylor terms of a low-precision pass
+    const double switch_step = 1e-6;    // leave low precision below this move
+    const double h = 1e-6;              // central-difference step on the model
+    klsp_timing acc{};
+    int rows = 0, n_lp = 0, n_64 = 0, rc;
+    auto record = [&](cd w, double absdet) {
+        if (history && rows < np->max_iter) {
+            history[rows] = w.real() / m;
+            history[rows + np->max_iter] = w.imag();
+            history[rows + 2 * np->max_iter] = absdet;
+        }
+        ++rows;
+    };
+    auto bounded = [&](cd w) {
+        double Op = w.real() / m, g = w.imag();
+        if (!std::isnan(g) && g < 1e-4) g = 1e-4;
+        if (Op > 1.0) Op = 1.0; else if (Op < 0.0) Op = 0.0;
+        return cd(m * Op, g);
+    };
+    cd w(m * np->Omega_p_init, np->gamma_init);
+    w = bounded(w);
+    cd slope(NAN, NAN);
+    // 1. low-precision passes on the Taylor model
+    for (int pass = 0; pass < np->max_iter; ++pass) {
+        const double om[2] = {w.real(), w.imag()};
+        if ((rc = build_batch(ctx, om, nt, KLSP_PREC_FP32, true))) return rc;
+        KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        add_timing(ctx, false, nt);
+        acc_timing(acc, ctx);
+        ++n_lp;
+        const double rho = 0.25 * w.imag();
+        cd s = 0.0, D0 = 0.0;
+        for (int it = 0; it < 12; ++it) {
+            const cd pts[3] = {s, s + h, s - h};
+            cd d[3];
+            if ((rc = model_dets(ctx, nt, pts, 3, d))) return rc;
+            acc_timing(acc, ctx);
+            if (it == 0) D0 = d[0];
+            const cd Dp = (d[1] - d[2]) / (2.0 * h);
+            if (!(std::abs(Dp) > 0.0) || !std::isfinite(std::abs(Dp))) break;
+            slope = Dp;
+            cd sn = s - d[0] / Dp;
+            const bool clipped = std::abs(sn) > rho;
+            if (clipped) sn *= rho / std::abs(sn);
+            const double step = std::abs(sn - s);
+            s = sn;
+            if (clipped || step <= 1e-12 * std::max(1.0, std::abs(w))) break;
+        }
+        record(w, std::abs(D0));
+        const cd wn = bounded(w + s);
+        const double moved = std::abs(wn - w);
+        w = wn;
+        if (!(moved >= switch_step)) break;
+    }
+    // 2. FP64 finish
+    cd D;
+    {
+        double d[2];
+        const double om[2] = {w.real(), w.imag()};
+        if ((rc = det_many(ctx, om, 1, KLSP_PREC_FP64, d))) return rc;
+        acc_timing(acc, ctx);
+        ++n_64;
+        D = cd(d[0], d[1]);
+    }
+    // |det| < tol alone is not enough at the low-precision root: where |det'|
+    // is small (~1e-4 at the default grid) it admits |d omega| ~ 1e-8, so at
+    // least one step informed by an FP64 det is taken before testing it.
+    int conv = 0;
+    for (int k = 0; !conv && k < np->max_iter; ++k) {
+        record(w, std::abs(D));
+        if (!(std::abs(slope) > 0.0) || !std::isfinite(std::abs(slope))) break;
+        const cd wn = bounded(w - D / slope);
+        if (wn == w) {              // the step is below the spacing of omega
+            conv = k > 0 && std::abs(D) < np->tol;
+            break;
+        }
+        double d[2];
+        const double om[2] = {wn.real(), wn.imag()};
+        if ((rc = det_many(ctx, om, 1, KLSP_PREC_FP64, d))) return rc;
+        acc_timing(acc, ctx);
+        ++n_64;
+        const cd Dn(d[0], d[1]);
+        slope = (Dn - D) / (wn - w);
+        w = wn;
+        D = Dn;
+        conv = std::abs(D) < np->tol;
+    }
+    record(w, std::abs(D));
+    res->Omega_p = w.real() / m;
+    res->gamma = w.imag();
+    res->omega[0] = w.real();
+    res->omega[1] = w.imag();
+    res->det[0] = D.real();
+    res->det[1] = D.imag();
+    res->iterations = rows;
+    res->converged = conv;
+    res->n_det_evals = n_lp + n_64;
+    res->n_fp64_builds = n_64;
+    res->n_lp_builds = n_lp;
+    ctx->last = acc;
+    return KLSP_OK;
+}
 
 extern "C" {
 
@@ -855,6 +1058,25 @@ int klsp_build_M(klsp_ctx* ctx, double om_re, double om_im, int precision, doubl
                                  hipMemcpyDeviceToHost, ctx->stream));
     KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     add_timing(ctx, false, 1);
+    return KLSP_OK;
+}
+
+int klsp_build_taylor(klsp_ctx* ctx, double om_re, double om_im, int n_terms, int precision,
+                      double* T_out)
+{
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    if (!T_out || n_terms < 1 || n_terms > kMaxBatch)
+        return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_build_taylor: bad arguments");
+    int rc = check_ready(ctx, precision, "klsp_build_taylor");
+    if (rc) return rc;
+    KLSP_HIP(ctx, hipSetDevice(ctx->device));
+    reset_timing(ctx);
+    const double om[2] = {om_re, om_im};
+    if ((rc = build_batch(ctx, om, n_terms, precision, true))) return rc;
+    KLSP_HIP(ctx, hipMemcpyAsync(T_out, ctx->d_M, sizeof(double2) * n_terms * ctx->N * ctx->N,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    add_timing(ctx, false, n_terms);
     return KLSP_OK;
 }
 
@@ -876,6 +1098,12 @@ int klsp_newton(klsp_ctx* ctx, const klsp_newton_params* np, klsp_newton_result*
     if (!ctx) return KLSP_ERR_INVALID_ARG;
     if (!np || !res || np->max_iter < 0 || np->max_line_search < 0)
         return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_newton: bad arguments");
+    if (np->precision == KLSP_PREC_MIXED) {
+        int rc = check_ready(ctx, KLSP_PREC_FP32, "klsp_newton");
+        if (rc) return rc;
+        reset_timing(ctx);
+        return newton_mixed(ctx, np, res, history);
+    }
     int rc = check_ready(ctx, np->precision, "klsp_newton");
     if (rc) return rc;
     reset_timing(ctx);
@@ -889,19 +1117,13 @@ int klsp_newton(klsp_ctx* ctx, const klsp_newton_params* np, klsp_newton_result*
         const double om[2] = {wr, wi};
         ++n_eval;
         int r = det_many(ctx, om, 1, prec, d);
-        acc.weights_ms += ctx->last.weights_ms; acc.contract_ms += ctx->last.contract_ms;
-        acc.reduce_ms += ctx->last.reduce_ms; acc.lu_ms += ctx->last.lu_ms;
-        acc.n_omega += ctx->last.n_omega; acc.contract_launches += ctx->last.contract_launches;
-        ctx->last = klsp_timing{};
+        acc_timing(acc, ctx);
         return r;
     };
     auto det2 = [&](const double* om4, double* d4) -> int {
         n_eval += 2;
         int r = det_many(ctx, om4, 2, prec, d4);
-        acc.weights_ms += ctx->last.weights_ms; acc.contract_ms += ctx->last.contract_ms;
-        acc.reduce_ms += ctx->last.reduce_ms; acc.lu_ms += ctx->last.lu_ms;
-        acc.n_omega += ctx->last.n_omega; acc.contract_launches += ctx->last.contract_launches;
-        ctx->last = klsp_timing{};
+        acc_timing(acc, ctx);
         return r;
     };
     for (int iter = 1; iter <= np->max_iter; ++iter) {
@@ -958,6 +1180,8 @@ int klsp_newton(klsp_ctx* ctx, const klsp_newton_params* np, klsp_newton_result*
     res->iterations = iter_done;
     res->converged = conv;
     res->n_det_evals = n_eval;
+    res->n_fp64_builds = prec == KLSP_PREC_FP64 ? n_eval : 0;
+    res->n_lp_builds = prec == KLSP_PREC_FP32 ? n_eval : 0;
     ctx->last = acc;
     return KLSP_OK;
 }

@@ -36,30 +36,32 @@ def det(W, DJ, F0l, Ores, Nk, G, omega) -> complex:
     return complex(np.linalg.det(np.eye(M.shape[0]) - M))
 
 
-def time_sample(NPh: int, N: int, n_l: int, n_slabs: int, repeats: int = 3, seed: int = 0):
-    """Time the per-l ZGEMM accumulation on `n_slabs` synthetic l-slabs of the
-    [NPh, N] shape plus one LU of N x N; return (seconds per full det
-    extrapolated to n_l slabs, seconds per slab, seconds per LU)."""
+def time_sample(NPh: int, N: int, n_l: int, n_slabs: int, n_distinct: int = 8, seed: int = 0):
+    """Time `n_slabs` per-l ZGEMM accumulations on synthetic [NPh, N] l-slabs
+    (cycling over `n_distinct` buffers, each larger than the host caches)
+    plus the median of 5 LUs of N x N; return (seconds per full det scaled
+    to n_l slabs, seconds per slab, seconds per LU)."""
     import time
 
     rng = np.random.default_rng(seed)
+    nd = max(1, min(n_distinct, n_slabs))
     slabs = [(rng.standard_normal((NPh, N)) + 1j * rng.standard_normal((NPh, N)))
-             for _ in range(n_slabs)]
+             for _ in range(nd)]
     DJ = rng.uniform(0.1, 1.0, NPh)
     F0l = rng.standard_normal(NPh)
     Ores = rng.uniform(-3, 3, NPh)
-    best_slab = np.inf
-    for _ in range(repeats):
-        M = np.zeros((N, N), dtype=np.complex128)
-        t0 = time.perf_counter()
-        for Wl in slabs:
-            c = weights(0.88 + 0.12j, DJ, F0l, Ores)
-            M += Wl.T @ (c[:, None] * Wl.conj())
-        best_slab = min(best_slab, (time.perf_counter() - t0) / n_slabs)
+    M = np.zeros((N, N), dtype=np.complex128)
+    t0 = time.perf_counter()
+    for i in range(n_slabs):
+        Wl = slabs[i % nd]
+        c = weights(0.88 + 0.12j, DJ, F0l, Ores)
+        M += Wl.T @ (c[:, None] * Wl.conj())
+    per_slab = (time.perf_counter() - t0) / n_slabs
     A = np.eye(N) - 1e-3 * M / max(np.abs(M).max(), 1.0)
-    best_lu = np.inf
-    for _ in range(repeats):
+    lus = []
+    for _ in range(5):
         t0 = time.perf_counter()
         np.linalg.det(A)
-        best_lu = min(best_lu, time.perf_counter() - t0)
-    return best_slab * n_l + best_lu, best_slab, best_lu
+        lus.append(time.perf_counter() - t0)
+    lu = float(np.median(lus))
+    return per_slab * n_l + lu, per_slab, lu

@@ -231,3 +231,72 @@ def test_fp32_path_close_to_fp64(oracle_small):
         d64 = pre.ctx.det([om], _ffi.PREC_FP64)[0]
         d32 = pre.ctx.det([om], _ffi.PREC_FP32)[0]
         assert abs(d32 - d64) <= 1e-4 * abs(d64) + 1e-9
+
+
+# --------------------------------------------------------------------------
+# Mixed-precision Newton (KLSP_PREC_MIXED) and its Taylor building block
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("prec", [_ffi.PREC_FP64, _ffi.PREC_FP32])
+def test_taylor_terms_expand_M(oracle_small, prec):
+    """T_1 = M(w_c), and M(w_c + s) - (T_1 - s T_2 + s^2 T_3) = O(|s|^3),
+    with M from the oracle at the shifted points."""
+    pre = device_pre(oracle_small)
+    wc = 2 * 0.44 + 0.12j
+    T = pre.ctx.build_taylor(wc, 3, prec)
+    M0 = oracle.build_M(oracle_small, wc)
+    tol = 1e-12 if prec == _ffi.PREC_FP64 else 2e-6
+    assert relnorm(T[0], M0) <= tol
+    errs = []
+    for s in (4e-3, 2e-3 + 2e-3j, 1e-3j):
+        Ms = oracle.build_M(oracle_small, wc + s)
+        errs.append(relnorm(T[0] - s * T[1] + s * s * T[2], Ms))
+        # third-order remainder: |s|^3 |T_4| ~ |s / Im w_c|^3 |M|
+        assert errs[-1] <= 20 * (abs(s) / wc.imag) ** 3 + tol, (s, errs[-1])
+    if prec == _ffi.PREC_FP64:
+        # the remainder shrinks like |s|^3: halving s divides it by ~8
+        e1 = relnorm(T[0] - 2e-3 * T[1] + 4e-6 * T[2], oracle.build_M(oracle_small, wc + 2e-3))
+        e2 = relnorm(T[0] - 1e-3 * T[1] + 1e-6 * T[2], oracle.build_M(oracle_small, wc + 1e-3))
+        assert 5.0 <= e1 / e2 <= 11.0
+
+
+def test_mixed_newton_small_grid_known_answer(small_cfg_paths):
+    """Mixed search on the published grid lands on the FP64 root of the
+    reference iteration (convergence_table.txt:13) within 1e-10."""
+    cfg = K.load_model(K.load_config(small_cfg_paths[0]), small_cfg_paths[1])
+    model = K.create_toomre_zang_model(cfg)
+    pre = K.precompute_all(cfg, model, K.GPUType.AMD)
+    dev = K.to_device(pre, K.GPUType.AMD, np.float64)
+    r = K.newton_search(cfg, pre, dev, K.GPUType.AMD, np.float64, 0.44, 0.125,
+                        precision="mixed")
+    print("mixed root", r.Omega_p, r.gamma, r.n_lp_builds, r.n_fp64_builds, r.det_val, flush=True)
+    assert r.converged and abs(r.det_val) < cfg.newton.tol
+    pre_o = oracle.precompute(oracle.load_toml_params(*small_cfg_paths))
+    ro = oracle.newton(pre_o, 0.44, 0.125)
+    assert abs(complex(2 * r.Omega_p, r.gamma) - ro["omega"]) <= 1e-10
+    assert abs(r.Omega_p - 0.4397142267) <= 1e-10
+    assert abs(r.gamma - 0.1230663944) <= 1e-10
+    assert 2 <= r.n_fp64_builds <= 3 and r.n_lp_builds <= 4
+    assert r.n_det_evals == r.n_fp64_builds + r.n_lp_builds
+    assert len(r.history) == r.iterations
+
+
+@pytest.mark.parametrize("start", [(0.45, 0.11), (0.43, 0.14), (0.4397, 0.1231)])
+def test_mixed_newton_matches_fp64_search(oracle_small, start):
+    pre = device_pre(oracle_small)
+    r64, _ = pre.ctx.newton(*start, max_iter=50, tol=1e-12, precision=_ffi.PREC_FP64)
+    rmx, H = pre.ctx.newton(*start, max_iter=50, tol=1e-12, precision=_ffi.PREC_MIXED)
+    assert r64.converged and rmx.converged
+    w64 = complex(r64.omega[0], r64.omega[1])
+    wmx = complex(rmx.omega[0], rmx.omega[1])
+    assert abs(wmx - w64) <= 1e-10
+    assert rmx.n_fp64_builds < r64.n_fp64_builds
+    assert H.shape == (rmx.iterations, 3) and np.all(np.isfinite(H))
+
+
+def test_mixed_newton_respects_bounds(oracle_small):
+    """A start below the gamma bound is lifted to 1e-4 and every move stays in
+    bounds, as in the reference iteration."""
+    pre = device_pre(oracle_small)
+    res, H = pre.ctx.newton(0.44, 1e-6, max_iter=30, tol=1e-12, precision=_ffi.PREC_MIXED)
+    assert np.all(H[:, 1] >= 1e-4) and np.all((H[:, 0] >= 0) & (H[:, 0] <= 1))
+    assert res.gamma >= 1e-4
