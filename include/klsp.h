@@ -235,13 +235,17 @@ typedef struct klsp_newton_result {
  *      j = 1..3), so M(w_c + s) = T_1 - s T_2 + s^2 T_3 + O(s^3); Newton on
  *      det(I - M) of that model (FP64 LUs only) within |s| <= Im(w_c)/4
  *      moves the centre; passes repeat until the move is below 1e-6;
- *   2. FP64 finish: FP64 dets at the low-precision root and after each
- *      step, the first step with the model's derivative, later ones with
- *      the FP64 secant slope, until |det| < tol after at least one such
- *      step (max_iter steps at most).
- * iterations = low-precision passes + FP64 steps; history rows record each
- * centre with its low-precision |det|, then each FP64 point with its FP64
- * |det|.  Bounds (gamma >= 1e-4, Omega_p in [0, 1]) apply to every move. */
+ *   2. FP64 finish: one FP64 build at the low-precision root w_a anchors
+ *      the model, M(w) = M64(w_a) - (s - s_a) T_2 + (s - s_a)(s + s_a) T_3
+ *      (s = w - w_c, s_a = w_a - w_c), in which the low-precision terms
+ *      enter only multiplied by |s - s_a| <= 1e-6, so M near the root is
+ *      FP64-accurate; Newton on it (FP64 LUs) runs until the step stops
+ *      shrinking; converged if then |det| < tol.  A new anchor is built if
+ *      the root moves more than 1e-5 from the last one.
+ * iterations = low-precision passes + anchors + 1; history rows record each
+ * centre with its low-precision |det|, each anchor with its FP64 |det|, and
+ * the final point.  Bounds (gamma >= 1e-4, Omega_p in [0, 1]) apply to
+ * every move. */
 int klsp_newton(klsp_ctx* ctx, const klsp_newton_params* np, klsp_newton_result* res,
                 double* history_out);
 

@@ -898,21 +898,15 @@ static void acc_timing(klsp_timing& acc, klsp_ctx* ctx)
     ctx->last = klsp_timing{};
 }
 
-// dets of I - sum_j (-s_i)^j T_j at nd points s_i of the Taylor model that
-// the last Taylor build left in ctx->d_M (nt terms).
-static int model_dets(klsp_ctx* ctx, int nt, const std::complex<double>* s, int nd,
+// dets of I - sum_j cf[i*nt + j] T_j for nd coefficient sets over the nt
+// matrices T_j in ctx->d_M (nd, nt <= kMaxBatch).
+static int model_dets(klsp_ctx* ctx, int nt, const std::complex<double>* cf, int nd,
                       std::complex<double>* dets)
 {
     int rc;
     if ((rc = ensure(ctx, ctx->d_A, ctx->A_cap, (size_t)kMaxBatch * ctx->N * ctx->N))) return rc;
     double2 coef[kMaxBatch * kMaxBatch];
-    for (int i = 0; i < nd; ++i) {
-        std::complex<double> p = 1.0;
-        for (int j = 0; j < nt; ++j) {
-            coef[i * nt + j] = make_double2(p.real(), p.imag());
-            p *= -s[i];
-        }
-    }
+    for (int i = 0; i < nd * nt; ++i) coef[i] = make_double2(cf[i].real(), cf[i].imag());
     KLSP_HIP(ctx, hipMemcpyAsync(ctx->d_coef, coef, sizeof(double2) * nd * nt,
                                  hipMemcpyHostToDevice, ctx->stream));
     if (ctx->prof) hipEventRecord(ctx->ev[3], ctx->stream);
@@ -960,9 +954,10 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, klsp_newton
     };
     cd w(m * np->Omega_p_init, np->gamma_init);
     w = bounded(w);
-    cd slope(NAN, NAN);
+    cd wc = w;                          // centre of the last low-precision pass
     // 1. low-precision passes on the Taylor model
     for (int pass = 0; pass < np->max_iter; ++pass) {
+        wc = w;
         const double om[2] = {w.real(), w.imag()};
         if ((rc = build_batch(ctx, om, nt, KLSP_PREC_FP32, true))) return rc;
         KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -973,13 +968,16 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, klsp_newton
         cd s = 0.0, D0 = 0.0;
         for (int it = 0; it < 12; ++it) {
             const cd pts[3] = {s, s + h, s - h};
-            cd d[3];
-            if ((rc = model_dets(ctx, nt, pts, 3, d))) return rc;
+            cd cf[3 * nt], d[3];
+            for (int i = 0; i < 3; ++i) {     // M(w_c + s) = sum_j (-s)^j T_{j+1}
+                cd p = 1.0;
+                for (int j = 0; j < nt; ++j, p *= -pts[i]) cf[i * nt + j] = p;
+            }
+            if ((rc = model_dets(ctx, nt, cf, 3, d))) return rc;
             acc_timing(acc, ctx);
             if (it == 0) D0 = d[0];
             const cd Dp = (d[1] - d[2]) / (2.0 * h);
             if (!(std::abs(Dp) > 0.0) || !std::isfinite(std::abs(Dp))) break;
-            slope = Dp;
             cd sn = s - d[0] / Dp;
             const bool clipped = std::abs(sn) > rho;
             if (clipped) sn *= rho / std::abs(sn);
@@ -993,38 +991,61 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, klsp_newton
         w = wn;
         if (!(moved >= switch_step)) break;
     }
-    // 2. FP64 finish
-    cd D;
-    {
-        double d[2];
-        const double om[2] = {w.real(), w.imag()};
-        if ((rc = det_many(ctx, om, 1, KLSP_PREC_FP64, d))) return rc;
-        acc_timing(acc, ctx);
-        ++n_64;
-        D = cd(d[0], d[1]);
-    }
-    // |det| < tol alone is not enough at the low-precision root: where |det'|
-    // is small (~1e-4 at the default grid) it admits |d omega| ~ 1e-8, so at
-    // least one step informed by an FP64 det is taken before testing it.
+    // 2. FP64 finish on the anchored model.  One FP64 build at the
+    // low-precision root w_a gives M64(w_a); with s = w - w_c and s_a = w_a - w_c
+    //   M(w) = M64(w_a) + [M_lp(w) - M_lp(w_a)]
+    //        = M64(w_a) - (s - s_a) T_2 + (s - s_a)(s + s_a) T_3 + O(|s|^2 |s - s_a|),
+    // so the low-precision terms enter only multiplied by |s - s_a| (<= 1e-6
+    // after the switch): their ~1e-5 relative error then moves M by ~1e-11
+    // |s - s_a| |T_2|, below FP64 rounding at the root.  (|det| < tol alone is
+    // no test of the root: where |det'| is small, ~1e-4 at the default grid,
+    // it admits |d omega| ~ 1e-8.)  Newton on the model needs FP64 LUs only;
+    // the anchor is rebuilt if the root moves more than 1e-5 from it.
+    const double h64 = 1e-5;            // central-difference step on the model
+    cd D(NAN, NAN);
     int conv = 0;
-    for (int k = 0; !conv && k < np->max_iter; ++k) {
-        record(w, std::abs(D));
-        if (!(std::abs(slope) > 0.0) || !std::isfinite(std::abs(slope))) break;
-        const cd wn = bounded(w - D / slope);
-        if (wn == w) {              // the step is below the spacing of omega
-            conv = k > 0 && std::abs(D) < np->tol;
-            break;
+    for (int anchor = 0; anchor < std::max(1, np->max_iter); ++anchor) {
+        {
+            const double om[2] = {w.real(), w.imag()};
+            if ((rc = build_batch(ctx, om, 1, KLSP_PREC_FP64))) return rc;   // d_M[0]
+            KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            add_timing(ctx, false, 1);
+            acc_timing(acc, ctx);
+            ++n_64;
         }
-        double d[2];
-        const double om[2] = {wn.real(), wn.imag()};
-        if ((rc = det_many(ctx, om, 1, KLSP_PREC_FP64, d))) return rc;
-        acc_timing(acc, ctx);
-        ++n_64;
-        const cd Dn(d[0], d[1]);
-        slope = (Dn - D) / (wn - w);
-        w = wn;
-        D = Dn;
-        conv = std::abs(D) < np->tol;
+        const cd sa = w - wc;
+        cd s = sa;
+        double prev_step = INFINITY;
+        bool far = false;
+        for (int it = 0; it < 30; ++it) {
+            const cd pts[3] = {s, s + h64, s - h64};
+            cd cf[9], d[3];
+            for (int i = 0; i < 3; ++i) {
+                const cd ds = pts[i] - sa;
+                cf[3 * i] = 1.0;
+                cf[3 * i + 1] = -ds;
+                cf[3 * i + 2] = ds * (pts[i] + sa);
+            }
+            if ((rc = model_dets(ctx, 3, cf, 3, d))) return rc;
+            acc_timing(acc, ctx);
+            D = d[0];
+            if (it == 0) record(w, std::abs(D));
+            const cd Dp = (d[1] - d[2]) / (2.0 * h64);
+            if (!(std::abs(Dp) > 0.0) || !std::isfinite(std::abs(Dp))) break;
+            const cd wn = bounded(wc + s - D / Dp);
+            const double step = std::abs(wn - (wc + s));
+            // converged: a step below 1e-13 |w|, or no longer shrinking (the
+            // rounding floor of the FP64 dets)
+            if (step <= 1e-13 * std::max(1.0, std::abs(wn)) || (it >= 2 && step > 0.5 * prev_step)) {
+                conv = std::abs(D) < np->tol;
+                break;
+            }
+            prev_step = step;
+            s = wn - wc;
+            if (std::abs(s - sa) > 1e-5) { far = true; break; }
+        }
+        w = wc + s;
+        if (!far) break;
     }
     record(w, std::abs(D));
     res->Omega_p = w.real() / m;
