@@ -141,6 +141,27 @@ def main() -> int:
         log(f"[bench] FP32 det: {1e3 * e32 / n32:.2f} ms/det, contraction {c32 / n32:.2f} ms, "
             f"det rel err vs FP64 {rel:.2e}")
 
+    if not args.no_fp32:
+        ctx.prepare(_ffi.PREC_BF16X3)
+        ctx.det([om0], _ffi.PREC_BF16X3)
+        nb = min(max(3, args.steps // 2), len(oms) - args.warmup)
+        cb = 0.0
+        t0 = time.perf_counter()
+        db = []
+        for i in range(nb):
+            db.append(ctx.det([oms[args.warmup + i]], _ffi.PREC_BF16X3)[0])
+            cb += ctx.last_timing()["contract_ms"]
+        eb = time.perf_counter() - t0
+        relb = max(abs(db[i] - dets[i]) / abs(dets[i]) for i in range(min(nb, len(dets))))
+        T = ctx.build_taylor(om0, 3, _ffi.PREC_BF16X3)
+        tb3 = ctx.last_timing()["contract_ms"]
+        extra.update({"det_per_s_bf16x3": nb / eb, "contract_ms_bf16x3": cb / nb,
+                      "bf16x3_contract_tflops_equiv": flop_build / (cb / nb * 1e-3) / 1e12,
+                      "bf16x3_det_rel_err_vs_fp64": relb,
+                      "taylor3_contract_ms_bf16x3": tb3})
+        log(f"[bench] BF16x3 det: {1e3 * eb / nb:.2f} ms/det, contraction {cb / nb:.2f} ms, "
+            f"det rel err vs FP64 {relb:.2e}; 3-term Taylor contraction {tb3:.2f} ms")
+
     if not args.no_newton:
         t0 = time.perf_counter()
         r = K.newton_search(cfg, pre, dev, K.GPUType.AMD, np.float64, 0.4397, 0.1231)
@@ -152,8 +173,18 @@ def main() -> int:
         log(f"[bench] FP64 Newton: {tn:.2f} s, {r.iterations} iterations, {r.n_det_evals} dets, "
             f"root Omega_p={r.Omega_p:.12f} gamma={r.gamma:.12f} |det|={abs(r.det_val):.2e}")
         w64 = r.omega
-        # mixed precision (the FP32 copy of W was made by the FP32 leg or here)
+        # mixed precision with FP32 passes, for comparison
         ctx.prepare(_ffi.PREC_FP32)
+        t0 = time.perf_counter()
+        rf = K.newton_search(cfg, pre, dev, K.GPUType.AMD, np.float64, 0.4397, 0.1231,
+                             precision="mixed_fp32")
+        tf = time.perf_counter() - t0
+        extra.update({"newton_s_mixed_fp32": tf,
+                      "newton_mixed_fp32_abs_domega_vs_fp64": abs(rf.omega - w64)})
+        log(f"[bench] mixed (FP32 passes) Newton: {tf:.3f} s, {rf.n_lp_builds} passes + "
+            f"{rf.n_fp64_builds} FP64, |d omega| {abs(rf.omega - w64):.2e}")
+        # mixed precision (BF16x3 passes; the planes of W were made by the BF16 leg or here)
+        ctx.prepare(_ffi.PREC_BF16X3)
         t0 = time.perf_counter()
         rm = K.newton_search(cfg, pre, dev, K.GPUType.AMD, np.float64, 0.4397, 0.1231,
                              precision="mixed")
@@ -171,7 +202,7 @@ def main() -> int:
                       "newton_mixed_abs_det": abs(rm.det_val),
                       "newton_mixed_contract_ms": tmx["contract_ms"],
                       "newton_mixed_lu_ms": tmx["lu_ms"]})
-        log(f"[bench] mixed Newton: {tm:.3f} s ({tn / tm:.2f}x FP64), {rm.n_lp_builds} FP32 "
+        log(f"[bench] mixed Newton: {tm:.3f} s ({tn / tm:.2f}x FP64), {rm.n_lp_builds} BF16x3 "
             f"Taylor passes + {rm.n_fp64_builds} FP64 dets, |d omega| vs FP64 root {dw:.2e}, "
             f"|det|={abs(rm.det_val):.2e}; contractions {tmx['contract_ms']:.1f} ms, "
             f"LUs {tmx['lu_ms']:.1f} ms")

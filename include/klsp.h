@@ -46,11 +46,15 @@ enum klsp_status {
 
 /* Arithmetic of an M(omega) build.  The determinant (LU) is FP64 always. */
 enum klsp_precision {
-    KLSP_PREC_FP64 = 0,  /* W complex128, FP64 MFMA contraction            */
-    KLSP_PREC_FP32 = 1,  /* W complex64, FP32 MFMA with FP64 K-tile sums   */
-    KLSP_PREC_MIXED = 2  /* klsp_newton only: FP32 Taylor-model passes until
-                            the step is below 1e-6, then FP64 dets (see
-                            klsp_newton)                                   */
+    KLSP_PREC_FP64 = 0,   /* W complex128, FP64 MFMA contraction            */
+    KLSP_PREC_FP32 = 1,   /* W complex64, FP32 MFMA with FP64 K-tile sums   */
+    KLSP_PREC_MIXED = 2,  /* klsp_newton only: BF16x3 Taylor-model passes
+                             until the step is below 1e-6, then an FP64
+                             anchored finish (see klsp_newton)             */
+    KLSP_PREC_BF16X3 = 3, /* W as bf16 hi + lo, BF16 MFMA with three-term
+                             split products, FP32 accumulation, FP64 gang
+                             sums (~1e-5 relative)                         */
+    KLSP_PREC_MIXED_FP32 = 4 /* klsp_newton only: as MIXED with FP32 passes */
 };
 
 typedef struct klsp_ctx klsp_ctx;
@@ -227,11 +231,12 @@ typedef struct klsp_newton_result {
 /* history_out: [max_iter, 3] column-major (Omega_p, gamma, |det|); the first
  * min(iterations, max_iter) rows are filled.  May be NULL.
  *
- * precision FP64 / FP32: the reference iteration, every det in that
- * precision.  precision MIXED (no counterpart in the reference; it returns
- * the same root to |d omega| <= 1e-10 at a fraction of the FP64 builds):
- *   1. low-precision passes: at a centre w_c one FP32 contraction builds the
- *      Taylor terms T_j = G N sum W c/d^j conj W (d = w_c - Omega_res,
+ * precision FP64 / FP32 / BF16X3: the reference iteration, every det in
+ * that precision.  precision MIXED (no counterpart in the reference; it
+ * returns the same root to |d omega| <= 1e-10 at a fraction of the FP64
+ * builds):
+ *   1. low-precision passes: at a centre w_c one BF16x3 (MIXED_FP32: FP32)
+ *      contraction builds the Taylor terms T_j = G N sum W c/d^j conj W (d = w_c - Omega_res,
  *      j = 1..3), so M(w_c + s) = T_1 - s T_2 + s^2 T_3 + O(s^3); Newton on
  *      det(I - M) of that model (FP64 LUs only) within |s| <= Im(w_c)/4
  *      moves the centre; passes repeat until the move is below 1e-6;

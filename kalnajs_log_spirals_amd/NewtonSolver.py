@@ -6,10 +6,10 @@ libklsp's host driver, with every det(I - M) on the device and the
 difference pair of each step evaluated as one two-omega batch.
 
 ``precision="mixed"`` (no counterpart in the reference) selects libklsp's
-mixed-precision search: FP32 passes that each build the first three Taylor
-terms of M around a centre and run Newton on that model, then FP64 dets
-until |det| < tol; it returns the FP64 root to |d omega| <= 1e-10 with a
-fraction of the FP64 builds.
+mixed-precision search: BF16x3 passes ("mixed_fp32": FP32 passes) that each
+build the first three Taylor terms of M around a centre and run Newton on
+that model, then one FP64 build that anchors the model for the FP64 finish;
+it returns the FP64 root to |d omega| <= 1e-10 with one FP64 build.
 """
 from __future__ import annotations
 
@@ -40,7 +40,8 @@ class NewtonResult:
     n_lp_builds: int = 0         # FP32 builds (Taylor passes in mixed mode)
 
 
-_PRECISIONS = {"fp64": _ffi.PREC_FP64, "fp32": _ffi.PREC_FP32, "mixed": _ffi.PREC_MIXED}
+_PRECISIONS = {"fp64": _ffi.PREC_FP64, "fp32": _ffi.PREC_FP32, "bf16x3": _ffi.PREC_BF16X3,
+               "mixed": _ffi.PREC_MIXED, "mixed_fp32": _ffi.PREC_MIXED_FP32}
 
 
 def newton_search(config: KalnajsConfig, precomputed: PrecomputedData,

@@ -24,6 +24,8 @@ KLSP_ERR_NO_DEVICE = 5
 PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_MIXED = 2
+PREC_BF16X3 = 3
+PREC_MIXED_FP32 = 4
 
 _STATUS_NAMES = {
     KLSP_ERR_INVALID_ARG: "invalid argument",

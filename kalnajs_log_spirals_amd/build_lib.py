@@ -27,14 +27,16 @@ COMMON = [
 ]
 DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-# (source, is HIP device code, extra flags).  The FP32 contraction keeps its
-# MFMA accumulators in VGPRs (-amdgpu-mfma-vgpr-form), which fits it in 2
-# waves per SIMD; the FP64 one is best with the default AGPR form.
+# (source, is HIP device code, extra flags).  The FP32 and BF16 contractions
+# keep their MFMA accumulators in VGPRs (-amdgpu-mfma-vgpr-form): with the
+# default AGPR form the compiler shuffles them between the two register files
+# every loop trip.  The FP64 one is best with the default form.
 SOURCES = [
     ("klsp_host.cpp", False, []),
     ("klsp_device.hip", True, []),
     ("klsp_contract_f64.hip", True, []),
     ("klsp_contract_f32.hip", True, ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
+    ("klsp_contract_bf16.hip", True, ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
 ]
 
 

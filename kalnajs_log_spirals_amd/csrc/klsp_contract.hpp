@@ -117,8 +117,9 @@ __device__ __forceinline__ void cload(const S2* pa, const S2* px, const double2*
 
 // Arguments shared by every contraction launch.
 struct ContractArgs {
-    const void* W;         // double2 or float2 rows of K
+    const void* W;         // double2 or float2 rows of K, or paired bf16 rows
     const double2* cw;     // weights [NW][Kp + kRowPad]
+    const float2* cwf;     // the same in FP32 (BF16 kernel)
     double2* P;            // partials [NW][n_gangs][Np][Np] (alpha-major)
     int64_t Kp;
     int Np;
@@ -300,5 +301,19 @@ ContractShape contract_f64_shape(int nw);
 ContractShape contract_f32_shape(int nw);
 hipError_t contract_f64_launch(int nw, const ContractArgs& a, hipStream_t stream);
 hipError_t contract_f32_launch(int nw, const ContractArgs& a, hipStream_t stream);
+
+// BF16 three-term contraction (klsp_contract_bf16.hip): workgroups of
+// kBf16Waves waves stacked along beta, kBf16MB blocks per wave, kBf16NB alpha
+// blocks per workgroup; K ranges per gang are multiples of 16 rows.
+constexpr int kBf16Waves = 4, kBf16MB = 5, kBf16NB = 1;
+struct Bf16Geometry {
+    int wrows, wcols, wg_per_cu;   // workgroup tiles per gang, resident workgroups per CU
+};
+Bf16Geometry contract_bf16_geometry(int Np);
+hipError_t contract_bf16_launch(int nw, const ContractArgs& a, hipStream_t stream);
+// W complex128 rows of K (`rows` present) -> bf16 hi and lo planes of nq
+// rows of four K (2 x nq x Np x 16 bytes at Wb).
+hipError_t split_W_bf16(const double2* W64, int64_t rows, int Np, int64_t nq, void* Wb,
+                        hipStream_t stream);
 
 }  // namespace klsp

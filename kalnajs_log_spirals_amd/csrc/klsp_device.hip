@@ -233,7 +233,8 @@ __global__ void k_W64_to_32(const double2* __restrict__ W, int64_t n, float2* __
 // =====================================================================
 __global__ void k_weights(const double* __restrict__ DJ, const double* __restrict__ F0l,
                           const double* __restrict__ Ores, int64_t NPh, int64_t K, int64_t Kp,
-                          const double* __restrict__ om, int nb, double2* __restrict__ c)
+                          const double* __restrict__ om, int nb, double2* __restrict__ c,
+                          float2* __restrict__ cf)
 {
     const int64_t n = (int64_t)nb * Kp;
     for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
@@ -249,6 +250,7 @@ __global__ void k_weights(const double* __restrict__ DJ, const double* __restric
             r = make_double2(xw * iv.x, xw * iv.y);
         }
         c[x] = r;
+        if (cf) cf[x] = make_float2((float)r.x, (float)r.y);
     }
 }
 
@@ -258,7 +260,7 @@ __global__ void k_weights(const double* __restrict__ DJ, const double* __restric
 __global__ void k_weights_taylor(const double* __restrict__ DJ, const double* __restrict__ F0l,
                                  const double* __restrict__ Ores, int64_t NPh, int64_t K,
                                  int64_t Kp, double om_re, double om_im, int nt,
-                                 double2* __restrict__ c)
+                                 double2* __restrict__ c, float2* __restrict__ cf)
 {
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < Kp;
          q += (int64_t)gridDim.x * blockDim.x) {
@@ -272,6 +274,7 @@ __global__ void k_weights_taylor(const double* __restrict__ DJ, const double* __
         }
         for (int j = 0; j < nt; ++j) {
             c[(int64_t)j * Kp + q] = r;
+            if (cf) cf[(int64_t)j * Kp + q] = make_float2((float)r.x, (float)r.y);
             r = cmul(r, iv);
         }
     }
@@ -434,11 +437,13 @@ struct klsp_ctx {
     // W
     double2* d_W64 = nullptr;
     float2* d_W32 = nullptr;
+    void* d_Wb = nullptr;     // paired bf16 hi/lo rows (BF16x3 builds)
     bool have_W = false;
     // workspaces
     double* d_om = nullptr;
     double2* d_coef = nullptr;  // [kMaxBatch][kMaxBatch] Taylor-model coefficients
     double2* d_c = nullptr;   size_t c_cap = 0;
+    float2* d_cf = nullptr;   size_t cf_cap = 0;   // FP32 copy of the weights
     double2* d_P = nullptr;   size_t P_cap = 0;
     double2* d_M = nullptr;   size_t M_cap = 0;
     double2* d_A = nullptr;   size_t A_cap = 0;
@@ -512,6 +517,29 @@ static GangPlan plan_gangs(const klsp_ctx* ctx, const ContractShape& sh)
     return gp;
 }
 
+// Gang geometry of the BF16 contraction: workgroup tiles of 320 beta rows x
+// 32 alpha columns; gangs per XCD chosen so that the gang workgroups fill the
+// XCD's resident workgroup slots in whole rounds where possible.
+static GangPlan plan_gangs_bf16(const klsp_ctx* ctx)
+{
+    const Bf16Geometry geo = contract_bf16_geometry(ctx->Np);
+    GangPlan gp;
+    gp.wrows = geo.wrows;
+    gp.wcols = geo.wcols;
+    const int gw = gp.wrows * gp.wcols;
+    const int slots = std::max(1, ctx->n_cu / 8) * std::max(1, geo.wg_per_cu);
+    int a = gw, b = slots;
+    while (b) { const int t = a % b; a = b; b = t; }
+    int gpx = slots / a;                       // lcm(gw, slots) / gw
+    if ((int64_t)gpx * gw > 6LL * slots) gpx = std::max(1, (4 * slots + gw - 1) / gw);
+    const int64_t Kp16 = round_up(ctx->Kp, 16);
+    while (gpx > 1 && (int64_t)gpx * 8 * 256 > Kp16) --gpx;
+    gp.gangs_per_xcd = gpx;
+    gp.n_gangs = 8 * gpx;
+    gp.Kg = round_up((Kp16 + gp.n_gangs - 1) / gp.n_gangs, 16);
+    return gp;
+}
+
 static ContractShape shape_for(int precision, int nw)
 {
     return precision == KLSP_PREC_FP64 ? contract_f64_shape(nw) : contract_f32_shape(nw);
@@ -523,16 +551,17 @@ static int max_gangs(const klsp_ctx* ctx)
     int g = 8;
     for (int prec : {KLSP_PREC_FP64, KLSP_PREC_FP32})
         for (int nw = 1; nw <= kMaxBatch; ++nw) g = std::max(g, plan_gangs(ctx, shape_for(prec, nw)).n_gangs);
-    return g;
+    return std::max(g, plan_gangs_bf16(ctx).n_gangs);
 }
 
 static int free_problem(klsp_ctx* ctx)
 {
     hipFree(ctx->d_DJ); hipFree(ctx->d_F0l); hipFree(ctx->d_Ores); hipFree(ctx->d_Nk);
-    hipFree(ctx->d_W64); hipFree(ctx->d_W32);
+    hipFree(ctx->d_W64); hipFree(ctx->d_W32); hipFree(ctx->d_Wb);
     ctx->d_DJ = ctx->d_F0l = ctx->d_Ores = ctx->d_Nk = nullptr;
     ctx->d_W64 = nullptr;
     ctx->d_W32 = nullptr;
+    ctx->d_Wb = nullptr;
     ctx->have_problem = ctx->have_W = false;
     return KLSP_OK;
 }
@@ -585,7 +614,7 @@ int klsp_destroy(klsp_ctx* ctx)
     hipStreamSynchronize(ctx->stream);
     free_problem(ctx);
     hipFree(ctx->d_om); hipFree(ctx->d_c); hipFree(ctx->d_P); hipFree(ctx->d_M);
-    hipFree(ctx->d_A); hipFree(ctx->d_det); hipFree(ctx->d_coef);
+    hipFree(ctx->d_A); hipFree(ctx->d_det); hipFree(ctx->d_coef); hipFree(ctx->d_cf);
     for (auto& e : ctx->ev) hipEventDestroy(e);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -629,6 +658,7 @@ int klsp_upload_problem(klsp_ctx* ctx, const klsp_problem* pr)
 static int alloc_W64(klsp_ctx* ctx)
 {
     if (ctx->d_W32) { KLSP_HIP(ctx, hipFree(ctx->d_W32)); ctx->d_W32 = nullptr; }
+    if (ctx->d_Wb) { KLSP_HIP(ctx, hipFree(ctx->d_Wb)); ctx->d_Wb = nullptr; }
     if (!ctx->d_W64)
         KLSP_HIP(ctx, hipMalloc((void**)&ctx->d_W64,
                                 sizeof(double2) * (ctx->Kp + kRowPad) * ctx->Np));
@@ -759,7 +789,8 @@ int klsp_download_W(klsp_ctx* ctx, double* W_out)
 int klsp_prepare(klsp_ctx* ctx, int precision)
 {
     if (!ctx) return KLSP_ERR_INVALID_ARG;
-    if (precision != KLSP_PREC_FP64 && precision != KLSP_PREC_FP32)
+    if (precision != KLSP_PREC_FP64 && precision != KLSP_PREC_FP32 &&
+        precision != KLSP_PREC_BF16X3)
         return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_prepare: unknown precision");
     if (!ctx->have_W) return fail(ctx, KLSP_ERR_NOT_READY, "klsp_prepare: no W on device");
     if (precision == KLSP_PREC_FP32 && !ctx->d_W32) {
@@ -770,6 +801,14 @@ int klsp_prepare(klsp_ctx* ctx, int precision)
                            ctx->d_W64, n, ctx->d_W32);
         KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
+    if (precision == KLSP_PREC_BF16X3 && !ctx->d_Wb) {
+        KLSP_HIP(ctx, hipSetDevice(ctx->device));
+        const int64_t nq = round_up(ctx->Kp, 16) / 4;
+        KLSP_HIP(ctx, hipMalloc(&ctx->d_Wb, (size_t)2 * 16 * nq * ctx->Np));
+        KLSP_HIP(ctx, split_W_bf16(ctx->d_W64, ctx->Kp + kRowPad, ctx->Np, nq, ctx->d_Wb,
+                                   ctx->stream));
+        KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     return KLSP_OK;
 }
 
@@ -778,10 +817,14 @@ int klsp_prepare(klsp_ctx* ctx, int precision)
 // Launches the contraction for nb omegas (weights already in ctx->d_c).
 static int launch_contract(klsp_ctx* ctx, int precision, int nb)
 {
-    const GangPlan gp = plan_gangs(ctx, shape_for(precision, nb));
+    const bool bf = precision == KLSP_PREC_BF16X3;
+    const GangPlan gp = bf ? plan_gangs_bf16(ctx) : plan_gangs(ctx, shape_for(precision, nb));
     ContractArgs a;
-    a.W = precision == KLSP_PREC_FP64 ? (const void*)ctx->d_W64 : (const void*)ctx->d_W32;
+    a.W = precision == KLSP_PREC_FP64 ? (const void*)ctx->d_W64
+          : bf                        ? (const void*)ctx->d_Wb
+                                      : (const void*)ctx->d_W32;
     a.cw = ctx->d_c;
+    a.cwf = ctx->d_cf;
     a.P = ctx->d_P;
     a.Kp = ctx->Kp;
     a.Np = ctx->Np;
@@ -790,6 +833,7 @@ static int launch_contract(klsp_ctx* ctx, int precision, int nb)
     a.wrows = gp.wrows;
     a.wcols = gp.wcols;
     const hipError_t e = precision == KLSP_PREC_FP64 ? contract_f64_launch(nb, a, ctx->stream)
+                         : bf                        ? contract_bf16_launch(nb, a, ctx->stream)
                                                      : contract_f32_launch(nb, a, ctx->stream);
     if (e != hipSuccess)
         return fail(ctx, KLSP_ERR_HIP, std::string("contraction launch: ") + hipGetErrorString(e));
@@ -805,6 +849,11 @@ static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
     const int64_t Kpc = ctx->Kp + kRowPad;
     int rc;
     if ((rc = ensure(ctx, ctx->d_c, ctx->c_cap, (size_t)kMaxBatch * Kpc))) return rc;
+    float2* cf = nullptr;
+    if (precision == KLSP_PREC_BF16X3) {
+        if ((rc = ensure(ctx, ctx->d_cf, ctx->cf_cap, (size_t)kMaxBatch * Kpc))) return rc;
+        cf = ctx->d_cf;
+    }
     const int mg = max_gangs(ctx);
     if ((rc = ensure(ctx, ctx->d_P, ctx->P_cap, (size_t)kMaxBatch * mg * ctx->Np * ctx->Np)))
         return rc;
@@ -816,11 +865,11 @@ static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
     if (taylor)
         hipLaunchKernelGGL(k_weights_taylor, dim3(grid_for(Kpc)), dim3(256), 0, ctx->stream,
                            ctx->d_DJ, ctx->d_F0l, ctx->d_Ores, ctx->NPh, ctx->K, Kpc, om[0],
-                           om[1], nb, ctx->d_c);
+                           om[1], nb, ctx->d_c, cf);
     else
         hipLaunchKernelGGL(k_weights, dim3(grid_for((int64_t)nb * Kpc)), dim3(256), 0,
                            ctx->stream, ctx->d_DJ, ctx->d_F0l, ctx->d_Ores, ctx->NPh, ctx->K,
-                           Kpc, ctx->d_om, nb, ctx->d_c);
+                           Kpc, ctx->d_om, nb, ctx->d_c, cf);
     if (ctx->prof) hipEventRecord(ctx->ev[1], ctx->stream);
     if ((rc = launch_contract(ctx, precision, nb))) return rc;
     if (ctx->prof) hipEventRecord(ctx->ev[2], ctx->stream);
@@ -853,9 +902,11 @@ static int check_ready(klsp_ctx* ctx, int precision, const char* who)
 {
     if (!ctx->have_problem || !ctx->have_W)
         return fail(ctx, KLSP_ERR_NOT_READY, std::string(who) + ": problem and W must be on device");
-    if (precision != KLSP_PREC_FP64 && precision != KLSP_PREC_FP32)
+    if (precision != KLSP_PREC_FP64 && precision != KLSP_PREC_FP32 &&
+        precision != KLSP_PREC_BF16X3)
         return fail(ctx, KLSP_ERR_INVALID_ARG, std::string(who) + ": unknown precision");
     if (precision == KLSP_PREC_FP32 && !ctx->d_W32) return klsp_prepare(ctx, precision);
+    if (precision == KLSP_PREC_BF16X3 && !ctx->d_Wb) return klsp_prepare(ctx, precision);
     return KLSP_OK;
 }
 
@@ -928,8 +979,8 @@ static int model_dets(klsp_ctx* ctx, int nt, const std::complex<double>* cf, int
 // Mixed-precision Newton (KLSP_PREC_MIXED; documented at klsp_newton in
 // klsp.h).  omega = m*Omega_p + i*gamma; det(I - M(omega)) is holomorphic in
 // omega, so the step is the complex Newton step -D/D'.
-static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, klsp_newton_result* res,
-                        double* history)
+static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec,
+                        klsp_newton_result* res, double* history)
 {
     typedef std::complex<double> cd;
     const double m = (double)ctx->m;
@@ -959,7 +1010,7 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, klsp_newton
     for (int pass = 0; pass < np->max_iter; ++pass) {
         wc = w;
         const double om[2] = {w.real(), w.imag()};
-        if ((rc = build_batch(ctx, om, nt, KLSP_PREC_FP32, true))) return rc;
+        if ((rc = build_batch(ctx, om, nt, lp_prec, true))) return rc;
         KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
         add_timing(ctx, false, nt);
         acc_timing(acc, ctx);
@@ -1119,11 +1170,12 @@ int klsp_newton(klsp_ctx* ctx, const klsp_newton_params* np, klsp_newton_result*
     if (!ctx) return KLSP_ERR_INVALID_ARG;
     if (!np || !res || np->max_iter < 0 || np->max_line_search < 0)
         return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_newton: bad arguments");
-    if (np->precision == KLSP_PREC_MIXED) {
-        int rc = check_ready(ctx, KLSP_PREC_FP32, "klsp_newton");
+    if (np->precision == KLSP_PREC_MIXED || np->precision == KLSP_PREC_MIXED_FP32) {
+        const int lp = np->precision == KLSP_PREC_MIXED ? KLSP_PREC_BF16X3 : KLSP_PREC_FP32;
+        int rc = check_ready(ctx, lp, "klsp_newton");
         if (rc) return rc;
         reset_timing(ctx);
-        return newton_mixed(ctx, np, res, history);
+        return newton_mixed(ctx, np, lp, res, history);
     }
     int rc = check_ready(ctx, np->precision, "klsp_newton");
     if (rc) return rc;
@@ -1202,7 +1254,7 @@ int klsp_newton(klsp_ctx* ctx, const klsp_newton_params* np, klsp_newton_result*
     res->converged = conv;
     res->n_det_evals = n_eval;
     res->n_fp64_builds = prec == KLSP_PREC_FP64 ? n_eval : 0;
-    res->n_lp_builds = prec == KLSP_PREC_FP32 ? n_eval : 0;
+    res->n_lp_builds = prec == KLSP_PREC_FP64 ? 0 : n_eval;
     ctx->last = acc;
     return KLSP_OK;
 }

@@ -259,6 +259,38 @@ def test_taylor_terms_expand_M(oracle_small, prec):
         assert 5.0 <= e1 / e2 <= 11.0
 
 
+@pytest.mark.parametrize("om", OMEGAS)
+def test_bf16x3_build_close_to_fp64(oracle_small, om):
+    """BF16 hi/lo split products, FP32 accumulation: M within 1e-4 normwise."""
+    pre = device_pre(oracle_small)
+    M64 = pre.ctx.build_M(om, _ffi.PREC_FP64)
+    Mb = pre.ctx.build_M(om, _ffi.PREC_BF16X3)
+    assert relnorm(Mb, M64) <= 1e-4
+    d64 = pre.ctx.det([om], _ffi.PREC_FP64)[0]
+    db = pre.ctx.det([om], _ffi.PREC_BF16X3)[0]
+    assert abs(db - d64) <= 1e-3 * det_scale(oracle_small, om)
+
+
+@pytest.mark.parametrize("NPh,N,n_l", [(37, 5, 2), (61, 17, 3), (100, 130, 2), (40, 330, 1)])
+def test_bf16x3_ragged_sizes(NPh, N, n_l):
+    """Sizes off the kernel's tiles (N = 330 needs two workgroup rows of beta)."""
+    rng = np.random.default_rng(NPh * 1000 + N)
+    W = (rng.standard_normal((NPh, N, n_l)) + 1j * rng.standard_normal((NPh, N, n_l))) / np.sqrt(NPh)
+    DJ = rng.uniform(0.5, 1.5, NPh)
+    F0l = rng.standard_normal((NPh, n_l))
+    Ores = rng.uniform(-1, 1, (NPh, n_l))
+    Nk = rng.uniform(0.5, 1.0, N)
+    pre = from_arrays(2, 1.0, W, DJ, F0l, Ores, Nk)
+    om = 0.3 + 0.2j
+    M64 = pre.ctx.build_M(om, _ffi.PREC_FP64)
+    Mb = pre.ctx.build_M(om, _ffi.PREC_BF16X3)
+    assert relnorm(Mb, M64) <= 1e-4
+    T64 = pre.ctx.build_taylor(om, 3, _ffi.PREC_FP64)
+    Tb = pre.ctx.build_taylor(om, 3, _ffi.PREC_BF16X3)
+    for j in range(3):
+        assert relnorm(Tb[j], T64[j]) <= 1e-4
+
+
 def test_mixed_newton_small_grid_known_answer(small_cfg_paths):
     """Mixed search on the published grid lands on the FP64 root of the
     reference iteration (convergence_table.txt:13) within 1e-10."""
@@ -280,11 +312,12 @@ def test_mixed_newton_small_grid_known_answer(small_cfg_paths):
     assert len(r.history) == r.iterations
 
 
+@pytest.mark.parametrize("mode", [_ffi.PREC_MIXED, _ffi.PREC_MIXED_FP32])
 @pytest.mark.parametrize("start", [(0.45, 0.11), (0.43, 0.14), (0.4397, 0.1231)])
-def test_mixed_newton_matches_fp64_search(oracle_small, start):
+def test_mixed_newton_matches_fp64_search(oracle_small, start, mode):
     pre = device_pre(oracle_small)
     r64, _ = pre.ctx.newton(*start, max_iter=50, tol=1e-12, precision=_ffi.PREC_FP64)
-    rmx, H = pre.ctx.newton(*start, max_iter=50, tol=1e-12, precision=_ffi.PREC_MIXED)
+    rmx, H = pre.ctx.newton(*start, max_iter=50, tol=1e-12, precision=mode)
     assert r64.converged and rmx.converged
     w64 = complex(r64.omega[0], r64.omega[1])
     wmx = complex(rmx.omega[0], rmx.omega[1])
