@@ -984,8 +984,8 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
 {
     typedef std::complex<double> cd;
     const double m = (double)ctx->m;
-    const int nt = 3;                   // Taylor terms of a low-precision pass
     const double switch_step = 1e-6;    // leave low precision below this move
+    const double accept_step = 1e-5;    // a model Newton step below this ends a pass
     const double h = 1e-6;              // central-difference step on the model
     klsp_timing acc{};
     int rows = 0, n_lp = 0, n_64 = 0, rc;
@@ -1006,7 +1006,13 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
     cd w(m * np->Omega_p_init, np->gamma_init);
     w = bounded(w);
     cd wc = w;                          // centre of the last low-precision pass
-    // 1. low-precision passes on the Taylor model
+    // 1. low-precision passes on the Taylor model.  A pass with nt terms
+    // leaves the root off by ~(|move| / Im w)^nt |move|; when that predicts a
+    // next move below 1e-4, the next pass builds two terms only (its own
+    // truncation, ~(1e-4 / Im w)^2 1e-4, is then far below the low-precision
+    // error).  A pass ends once a model Newton step below accept_step is
+    // applied: from there the step error is ~|step|^2 / Im w.
+    int nt = 3;                         // Taylor terms of the current pass
     for (int pass = 0; pass < np->max_iter; ++pass) {
         wc = w;
         const double om[2] = {w.real(), w.imag()};
@@ -1019,7 +1025,7 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
         cd s = 0.0, D0 = 0.0;
         for (int it = 0; it < 12; ++it) {
             const cd pts[3] = {s, s + h, s - h};
-            cd cf[3 * nt], d[3];
+            cd cf[3 * kMaxBatch], d[3];
             for (int i = 0; i < 3; ++i) {     // M(w_c + s) = sum_j (-s)^j T_{j+1}
                 cd p = 1.0;
                 for (int j = 0; j < nt; ++j, p *= -pts[i]) cf[i * nt + j] = p;
@@ -1034,14 +1040,17 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
             if (clipped) sn *= rho / std::abs(sn);
             const double step = std::abs(sn - s);
             s = sn;
-            if (clipped || step <= 1e-12 * std::max(1.0, std::abs(w))) break;
+            if (clipped || step <= accept_step * std::max(1.0, std::abs(w))) break;
         }
         record(w, std::abs(D0));
         const cd wn = bounded(w + s);
         const double moved = std::abs(wn - w);
         w = wn;
         if (!(moved >= switch_step)) break;
+        const double r = moved / std::max(wn.imag(), 1e-300);
+        nt = std::pow(r, nt) * moved < 1e-4 ? 2 : 3;
     }
+    const int nt_last = nt;
     // 2. FP64 finish on the anchored model.  One FP64 build at the
     // low-precision root w_a gives M64(w_a); with s = w - w_c and s_a = w_a - w_c
     //   M(w) = M64(w_a) + [M_lp(w) - M_lp(w_a)]
@@ -1073,11 +1082,11 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
             cd cf[9], d[3];
             for (int i = 0; i < 3; ++i) {
                 const cd ds = pts[i] - sa;
-                cf[3 * i] = 1.0;
-                cf[3 * i + 1] = -ds;
-                cf[3 * i + 2] = ds * (pts[i] + sa);
+                cf[nt_last * i] = 1.0;
+                cf[nt_last * i + 1] = -ds;
+                if (nt_last == 3) cf[nt_last * i + 2] = ds * (pts[i] + sa);
             }
-            if ((rc = model_dets(ctx, 3, cf, 3, d))) return rc;
+            if ((rc = model_dets(ctx, nt_last, cf, 3, d))) return rc;
             acc_timing(acc, ctx);
             D = d[0];
             if (it == 0) record(w, std::abs(D));
