@@ -170,38 +170,37 @@ __device__ __forceinline__ void wg_walk(const ContractArgs& a, StageB* sb, int64
                 acc.r[w][m][n] = (v4f){0, 0, 0, 0};
                 acc.i[w][m][n] = (v4f){0, 0, 0, 0};
             }
-    v4u A0[MA][2], A1[MA][2];
-    v2u X0, X1;
-    v2f C0[NW], C1[NW];
-    auto load_stage = [&](int64_t K0, v4u (&A)[MA][2], v2u& X, v2f (&C)[NW]) {
+    // A operands are loaded once per stage (the other workgroup on the CU
+    // covers their latency); the B inputs of the next stage are prefetched.
+    v4u A[MA][2];
+    v2u X, Xn;
+    v2f C[NW], Cn[NW];
+    auto load_B_in = [&](int64_t K0, v2u& Xo, v2f (&Co)[NW]) {
         const int64_t q0 = K0 >> 2;
         if (cvalid) {
             const int64_t x = ((q0 + (kr >> 2)) * Np + a0 + c) * 4 + (kr & 3);
-            X = (v2u){((const unsigned*)Wh)[x], ((const unsigned*)Wl)[x]};
+            Xo = (v2u){((const unsigned*)Wh)[x], ((const unsigned*)Wl)[x]};
         } else {
-            X = (v2u){0, 0};
+            Xo = (v2u){0, 0};
         }
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
             const float2 t = a.cwf[w * Kpc + K0 + kr];
-            C[w] = (v2f){t.x, t.y};
+            Co[w] = (v2f){t.x, t.y};
         }
-        load_A<MB>(Wh, Wl, q0, Np, b0, g, i, A);
     };
-    int64_t K0 = Kb;
-    if (K0 < Ke) load_stage(K0, A0, X0, C0);
-    while (K0 < Ke) {
-        produce_B<NW>(sb[0], X0, C0, kr, c);
-        load_stage(K0 + 16 < Ke ? K0 + 16 : K0, A1, X1, C1);
+    if (Kb < Ke) load_B_in(Kb, X, C);
+    int buf = 0;
+    for (int64_t K0 = Kb; K0 < Ke; K0 += 16) {
+        load_A<MB>(Wh, Wl, K0 >> 2, Np, b0, g, i, A);
+        produce_B<NW>(sb[buf], X, C, kr, c);
+        load_B_in(K0 + 16 < Ke ? K0 + 16 : K0, Xn, Cn);
         lds_barrier();
-        mfma_stage<MB, NB, NW>(sb[0], A0, acc, g, i);
-        K0 += 16;
-        if (K0 >= Ke) break;
-        produce_B<NW>(sb[1], X1, C1, kr, c);
-        load_stage(K0 + 16 < Ke ? K0 + 16 : K0, A0, X0, C0);
-        lds_barrier();
-        mfma_stage<MB, NB, NW>(sb[1], A1, acc, g, i);
-        K0 += 16;
+        mfma_stage<MB, NB, NW>(sb[buf], A, acc, g, i);
+        X = Xn;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) C[w] = Cn[w];
+        buf ^= 1;
     }
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
@@ -220,7 +219,7 @@ __device__ __forceinline__ void wg_walk(const ContractArgs& a, StageB* sb, int64
 
 
 template <int NW>
-__global__ __launch_bounds__(kThreads) void k_contract_bf16(ContractArgs a)
+__global__ __launch_bounds__(kThreads, 2) void k_contract_bf16(ContractArgs a)
 {
     __shared__ StageB sb[2];
     const int L = blockIdx.x;

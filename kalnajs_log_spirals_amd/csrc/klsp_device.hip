@@ -518,7 +518,7 @@ static GangPlan plan_gangs(const klsp_ctx* ctx, const ContractShape& sh)
 }
 
 // Gang geometry of the BF16 contraction: workgroup tiles of 320 beta rows x
-// 32 alpha columns; gangs per XCD chosen so that the gang workgroups fill the
+// 16 alpha columns; gangs per XCD chosen so that the gang workgroups fill the
 // XCD's resident workgroup slots in whole rounds where possible.
 static GangPlan plan_gangs_bf16(const klsp_ctx* ctx)
 {
@@ -528,10 +528,18 @@ static GangPlan plan_gangs_bf16(const klsp_ctx* ctx)
     gp.wcols = geo.wcols;
     const int gw = gp.wrows * gp.wcols;
     const int slots = std::max(1, ctx->n_cu / 8) * std::max(1, geo.wg_per_cu);
-    int a = gw, b = slots;
-    while (b) { const int t = a % b; a = b; b = t; }
-    int gpx = slots / a;                       // lcm(gw, slots) / gw
-    if ((int64_t)gpx * gw > 6LL * slots) gpx = std::max(1, (4 * slots + gw - 1) / gw);
+    // gangs per XCD: the fewest from 2 rounds of the XCD's slots on whose
+    // last round leaves at most 2% of the slots idle, else the least idle
+    // within 12 rounds
+    int gpx = 0, xmin = 0;
+    double best = 2.0;
+    for (int x = std::max(1, (2 * slots) / gw); x <= std::max(1, (12 * slots) / gw); ++x) {
+        const double r = (double)x * gw / slots, rounds = std::ceil(r);
+        const double tail = (rounds - r) / rounds;
+        if (tail <= 0.02) { gpx = x; break; }
+        if (tail < best) { best = tail; xmin = x; }
+    }
+    if (gpx == 0) gpx = std::max(1, xmin);
     const int64_t Kp16 = round_up(ctx->Kp, 16);
     while (gpx > 1 && (int64_t)gpx * 8 * 256 > Kp16) --gpx;
     gp.gangs_per_xcd = gpx;
