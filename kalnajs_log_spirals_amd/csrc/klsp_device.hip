@@ -312,30 +312,47 @@ __global__ void k_reduce(const double2* __restrict__ P, int S, int Np, int N, in
 // Stage 4: det(I - M) by complex LU with partial pivoting, one workgroup
 // per matrix (MatrixBuilder.jl:182 -> LinearAlgebra det -> LAPACK zgetrf).
 // Pivot = first row maximising |re|+|im| (izamax), multipliers scaled by the
-// reciprocal pivot, rank-1 trailing update; det = prod U_jj * (-1)^swaps
-// accumulated in LinearAlgebra's order.  The working copy A = I - M lives in
-// global memory (L2-resident); the pivot column and row are cached in LDS.
+// reciprocal pivot; det = prod U_jj * (-1)^swaps in LinearAlgebra's order.
+// Blocked right-looking form, as zgetrf: a panel of up to kLuMaxNb columns is
+// loaded into LDS and factored there column by column; then every wave takes
+// kLuCols trailing columns at a time, applies the panel's row permutation (one
+// gather), solves U12 = L11^-1 A12 with lane shuffles and subtracts
+// L21 U12 (L21 read from LDS), so the trailing matrix in global memory
+// (L2-resident) is read and written once per panel.  Only what the
+// determinant needs is kept: the L and U12 parts are not written back.
 // =====================================================================
+constexpr int kLuThreads = 512;       // (1024 threads measured slower: more waves per barrier)
+constexpr int kLuMaxNb = 8;           // panel width
+constexpr int kLuCols = 4;            // trailing columns per wave pass (U12: 128 VGPRs)
+constexpr int kLuPanel = 8192;        // panel elements in LDS (N * width)
+
 __device__ __forceinline__ void argmax_merge(double& v, int& idx, double ov, int oi)
 {
     if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
 }
 
+__device__ __forceinline__ double2 shfl2(double2 v, int src)
+{
+    return make_double2(__shfl(v.x, src), __shfl(v.y, src));
+}
+
 // With coef == nullptr matrix b is I - M[b]; otherwise it is
 // I - sum_j coef[b*nt + j] M[j] (a point of the Taylor model of M).
-__global__ __launch_bounds__(1024) void k_lu_det(const double2* __restrict__ M,
-                                                  const double2* __restrict__ coef, int nt,
-                                                  double2* __restrict__ Awork, int N,
-                                                  double2* __restrict__ det_out)
+__global__ __launch_bounds__(kLuThreads) void k_lu_det(const double2* __restrict__ M,
+                                                        const double2* __restrict__ coef, int nt,
+                                                        double2* __restrict__ Awork, int N,
+                                                        double2* __restrict__ det_out)
 {
     const int b = blockIdx.x;
     const int tid = threadIdx.x, nth = blockDim.x;
+    const int lane = tid & 63, wid = tid >> 6, nwv = nth >> 6;
     const int64_t NN = (int64_t)N * N;
     double2* A = Awork + b * NN;
-    __shared__ double2 colj[kMaxN];
-    __shared__ double2 rowj[kMaxN];
-    __shared__ double redv[32];
-    __shared__ int redi[32];
+    __shared__ double2 P[kLuPanel];   // panel, column-major, leading dimension m
+    __shared__ int perm[kMaxN];
+    __shared__ int pv[kLuMaxNb];
+    __shared__ double redv[kLuThreads / 64];
+    __shared__ int redi[kLuThreads / 64];
     __shared__ int s_piv;
 
     for (int64_t x = tid; x < NN; x += nth) {
@@ -354,62 +371,121 @@ __global__ __launch_bounds__(1024) void k_lu_det(const double2* __restrict__ M,
     }
     __syncthreads();
 
+    const int nbk = max(1, min(kLuMaxNb, kLuPanel / N));
     double2 det = make_double2(1.0, 0.0);
     int swaps = 0;
     bool singular = false;
-    for (int j = 0; j < N; ++j) {
-        // pivot search over rows j..N-1 of column j
-        double bv = -1.0;
-        int bi = N;
-        for (int r = j + tid; r < N; r += nth) {
-            const double2 a = A[r + (int64_t)N * j];
-            const double v = fabs(a.x) + fabs(a.y);
-            if (v > bv) { bv = v; bi = r; }
-        }
-        for (int off = 32; off >= 1; off >>= 1) {
-            const double ov = __shfl_xor(bv, off);
-            const int oi = __shfl_xor(bi, off);
-            argmax_merge(bv, bi, ov, oi);
-        }
-        const int wid = tid >> 6;
-        if ((tid & 63) == 0) { redv[wid] = bv; redi[wid] = bi; }
-        __syncthreads();
-        if (tid == 0) {
-            double v = redv[0];
-            int idx = redi[0];
-            for (int w = 1; w < (nth + 63) / 64; ++w) argmax_merge(v, idx, redv[w], redi[w]);
-            s_piv = (v > 0.0) ? idx : -1;
+    for (int k0 = 0; k0 < N; k0 += nbk) {
+        const int kb = min(nbk, N - k0), m = N - k0;
+        for (int x = tid; x < m * kb; x += nth) {
+            const int t = x / m, r = x - t * m;
+            P[x] = A[(k0 + r) + (int64_t)N * (k0 + t)];
         }
         __syncthreads();
-        const int p = s_piv;
-        if (p < 0) { singular = true; __syncthreads(); continue; }
-        if (p != j) {
-            ++swaps;
-            for (int c = j + tid; c < N; c += nth) {
-                const double2 t = A[j + (int64_t)N * c];
-                A[j + (int64_t)N * c] = A[p + (int64_t)N * c];
-                A[p + (int64_t)N * c] = t;
+        // unblocked factorisation of the panel
+        for (int jj = 0; jj < kb; ++jj) {
+            double2* Pj = P + jj * m;
+            double bv = -1.0;
+            int bi = m;
+            for (int r = jj + tid; r < m; r += nth) {
+                const double2 a = Pj[r];
+                const double v = fabs(a.x) + fabs(a.y);
+                if (v > bv) { bv = v; bi = r; }
             }
+            for (int off = 32; off >= 1; off >>= 1) {
+                const double ov = __shfl_xor(bv, off);
+                const int oi = __shfl_xor(bi, off);
+                argmax_merge(bv, bi, ov, oi);
+            }
+            if (lane == 0) { redv[wid] = bv; redi[wid] = bi; }
+            __syncthreads();
+            if (tid == 0) {
+                double v = redv[0];
+                int idx = redi[0];
+                for (int w = 1; w < nwv; ++w) argmax_merge(v, idx, redv[w], redi[w]);
+                s_piv = (v > 0.0) ? idx : -1;
+                pv[jj] = (v > 0.0) ? idx : jj;
+            }
+            __syncthreads();
+            const int p = s_piv;
+            if (p < 0) { singular = true; continue; }
+            if (p != jj) {
+                ++swaps;
+                for (int t = tid; t < kb; t += nth) {
+                    const double2 tmp = P[t * m + jj];
+                    P[t * m + jj] = P[t * m + p];
+                    P[t * m + p] = tmp;
+                }
+            }
+            __syncthreads();
+            const double2 piv = Pj[jj];
+            det = cmul(det, piv);
+            const double2 rp = d_inv(piv.x, piv.y);
+            for (int r = jj + 1 + tid; r < m; r += nth) Pj[r] = cmul(Pj[r], rp);
+            __syncthreads();
+            const int nr = m - jj - 1, nc = kb - jj - 1;
+            for (int x = tid; x < nr * nc; x += nth) {
+                const int t = jj + 1 + x / nr, r = jj + 1 + x % nr;
+                const double2 lu = cmul(Pj[r], P[t * m + jj]);
+                P[t * m + r] = make_double2(P[t * m + r].x - lu.x, P[t * m + r].y - lu.y);
+            }
+            __syncthreads();
         }
+        // the panel's row interchanges as one permutation of rows k0..N-1
+        for (int r = tid; r < m; r += nth) perm[r] = r;
         __syncthreads();
-        const double2 piv = A[j + (int64_t)N * j];
-        det = cmul(det, piv);
-        const double2 rp = d_inv(piv.x, piv.y);
-        for (int r = j + 1 + tid; r < N; r += nth) {
-            const double2 l = cmul(A[r + (int64_t)N * j], rp);
-            A[r + (int64_t)N * j] = l;
-            colj[r] = l;
-        }
-        for (int c = j + 1 + tid; c < N; c += nth) rowj[c] = A[j + (int64_t)N * c];
+        if (tid == 0)
+            for (int jj = 0; jj < kb; ++jj) {
+                const int t = perm[jj];
+                perm[jj] = perm[pv[jj]];
+                perm[pv[jj]] = t;
+            }
         __syncthreads();
-        const int n = N - j - 1;
-        const int64_t nn = (int64_t)n * n;
-        for (int64_t x = tid; x < nn; x += nth) {
-            const int r = j + 1 + (int)(x % n), c = j + 1 + (int)(x / n);
-            const double2 lu = cmul(colj[r], rowj[c]);
-            double2* a = A + r + (int64_t)N * c;
-            const double2 av = *a;
-            *a = make_double2(av.x - lu.x, av.y - lu.y);
+        // trailing columns, kLuCols per wave at a time
+        const int c0 = k0 + kb, ncol = N - c0;
+        for (int cg = wid * kLuCols; cg < ncol; cg += nwv * kLuCols) {
+            const int nq = min(kLuCols, ncol - cg);
+            double2 u[kLuCols][kLuMaxNb];
+            for (int r0 = 0; r0 < m; r0 += 64) {
+                const int r = r0 + lane;
+                double2 x[kLuCols];
+#pragma unroll
+                for (int q = 0; q < kLuCols; ++q)
+                    x[q] = (q < nq && r < m) ? A[(k0 + perm[r]) + (int64_t)N * (c0 + cg + q)]
+                                             : make_double2(0.0, 0.0);
+                if (r0 == 0) {
+                    // U12 = L11^-1 A12 (unit lower L11): lane t holds row t
+#pragma unroll
+                    for (int t = 0; t < kLuMaxNb; ++t) {
+                        if (t >= kb) break;
+#pragma unroll
+                        for (int q = 0; q < kLuCols; ++q) u[q][t] = shfl2(x[q], t);
+                        if (lane > t && lane < kb) {
+                            const double2 l = P[t * m + lane];
+#pragma unroll
+                            for (int q = 0; q < kLuCols; ++q) {
+                                const double2 lu = cmul(l, u[q][t]);
+                                x[q] = make_double2(x[q].x - lu.x, x[q].y - lu.y);
+                            }
+                        }
+                    }
+                }
+                if (r >= kb && r < m) {
+#pragma unroll
+                    for (int t = 0; t < kLuMaxNb; ++t) {
+                        if (t >= kb) break;
+                        const double2 l = P[t * m + r];
+#pragma unroll
+                        for (int q = 0; q < kLuCols; ++q) {
+                            const double2 lu = cmul(l, u[q][t]);
+                            x[q] = make_double2(x[q].x - lu.x, x[q].y - lu.y);
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < kLuCols; ++q)
+                        if (q < nq) A[(k0 + r) + (int64_t)N * (c0 + cg + q)] = x[q];
+                }
+            }
         }
         __syncthreads();
     }
@@ -931,7 +1007,7 @@ static int det_many(klsp_ctx* ctx, const double* om, int64_t n, int precision, d
     for (int64_t i0 = 0; i0 < n; i0 += maxb) {
         const int nb = (int)std::min<int64_t>(maxb, n - i0);
         if ((rc = build_batch(ctx, om + 2 * i0, nb, precision))) return rc;
-        hipLaunchKernelGGL(k_lu_det, dim3(nb), dim3(1024), 0, ctx->stream, ctx->d_M,
+        hipLaunchKernelGGL(k_lu_det, dim3(nb), dim3(kLuThreads), 0, ctx->stream, ctx->d_M,
                            (const double2*)nullptr, 0, ctx->d_A, (int)ctx->N, ctx->d_det);
         if (ctx->prof) hipEventRecord(ctx->ev[4], ctx->stream);
         KLSP_HIP(ctx, hipGetLastError());
@@ -969,7 +1045,7 @@ static int model_dets(klsp_ctx* ctx, int nt, const std::complex<double>* cf, int
     KLSP_HIP(ctx, hipMemcpyAsync(ctx->d_coef, coef, sizeof(double2) * nd * nt,
                                  hipMemcpyHostToDevice, ctx->stream));
     if (ctx->prof) hipEventRecord(ctx->ev[3], ctx->stream);
-    hipLaunchKernelGGL(k_lu_det, dim3(nd), dim3(1024), 0, ctx->stream, ctx->d_M,
+    hipLaunchKernelGGL(k_lu_det, dim3(nd), dim3(kLuThreads), 0, ctx->stream, ctx->d_M,
                        (const double2*)ctx->d_coef, nt, ctx->d_A, (int)ctx->N, ctx->d_det);
     if (ctx->prof) hipEventRecord(ctx->ev[4], ctx->stream);
     KLSP_HIP(ctx, hipGetLastError());
