@@ -162,6 +162,24 @@ def main() -> int:
         log(f"[bench] BF16x3 det: {1e3 * eb / nb:.2f} ms/det, contraction {cb / nb:.2f} ms, "
             f"det rel err vs FP64 {relb:.2e}; 3-term Taylor contraction {tb3:.2f} ms")
 
+    if not args.no_fp32:
+        # omega-grid scan in BF16x3 (three omegas per contraction, all LUs of a
+        # batch of builds in one launch): one 64-point column of the 64 x 64
+        # window Re w/m in [0.30, 0.60], Im w in [0.02, 0.25], at Re w/m = 0.44
+        t0 = time.perf_counter()
+        col, _, _ = ctx.grid_scan(1, 64, 0.44, 0.44, 0.02, 0.25, precision=_ffi.PREC_BF16X3)
+        ts = time.perf_counter() - t0
+        gam = np.linspace(0.02, 0.25, 64)
+        sample = list(range(0, 64, 8))
+        ref = ctx.det([complex(2 * 0.44, gam[i]) for i in sample], _ffi.PREC_FP64)
+        med = float(np.median(np.abs(ref)))
+        err = max(abs(col[i, 0] - ref[k]) / max(abs(ref[k]), med) for k, i in enumerate(sample))
+        extra.update({"scan_dets_per_s_bf16x3": 64 / ts,
+                      "scan_64x64_s_bf16x3_est": 4096 * ts / 64,
+                      "scan_bf16x3_rel_err_vs_fp64": err})
+        log(f"[bench] BF16x3 scan column: {64 / ts:.1f} det/s (64 x 64 window ~{4096 * ts / 64:.1f} s), "
+            f"rel err vs FP64 at 8 points {err:.2e}")
+
     if not args.no_newton:
         t0 = time.perf_counter()
         r = K.newton_search(cfg, pre, dev, K.GPUType.AMD, np.float64, 0.4397, 0.1231)

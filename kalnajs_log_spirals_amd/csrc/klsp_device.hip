@@ -523,7 +523,7 @@ struct klsp_ctx {
     double2* d_P = nullptr;   size_t P_cap = 0;
     double2* d_M = nullptr;   size_t M_cap = 0;
     double2* d_A = nullptr;   size_t A_cap = 0;
-    double2* d_det = nullptr;
+    double2* d_det = nullptr; size_t det_cap = 0;
     int last_S = 0;           // K-splits of the last contraction launch
     // profiling
     bool prof = false;
@@ -684,9 +684,11 @@ int klsp_create(int device, klsp_ctx** ctx_out)
     if (hipMalloc((void**)&ctx->d_om, sizeof(double) * 2 * kMaxBatch) != hipSuccess ||
         hipMalloc((void**)&ctx->d_coef, sizeof(double2) * kMaxBatch * kMaxBatch) != hipSuccess ||
         hipMalloc((void**)&ctx->d_det, sizeof(double2) * kMaxBatch) != hipSuccess) {
+        ctx->det_cap = 0;
         klsp_destroy(ctx);
         return KLSP_ERR_OOM;
     }
+    ctx->det_cap = kMaxBatch;
     *ctx_out = ctx;
     return KLSP_OK;
 }
@@ -927,8 +929,11 @@ static int launch_contract(klsp_ctx* ctx, int precision, int nb)
 
 // Builds M for nb <= kMaxBatch omegas into ctx->d_M ([nb][N,N]).  With
 // taylor set, om is one centre and the nb matrices are its Taylor terms.
+static void add_build_timing(klsp_ctx* ctx, int nb);
+
+// ... into matrix slots slot..slot+nb-1 of ctx->d_M.
 static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
-                       bool taylor = false)
+                       bool taylor = false, int slot = 0)
 {
     const int64_t Kpc = ctx->Kp + kRowPad;
     int rc;
@@ -941,7 +946,9 @@ static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
     const int mg = max_gangs(ctx);
     if ((rc = ensure(ctx, ctx->d_P, ctx->P_cap, (size_t)kMaxBatch * mg * ctx->Np * ctx->Np)))
         return rc;
-    if ((rc = ensure(ctx, ctx->d_M, ctx->M_cap, (size_t)kMaxBatch * ctx->N * ctx->N))) return rc;
+    if ((rc = ensure(ctx, ctx->d_M, ctx->M_cap,
+                     (size_t)std::max(kMaxBatch, slot + nb) * ctx->N * ctx->N)))
+        return rc;
     if (!taylor)
         KLSP_HIP(ctx, hipMemcpyAsync(ctx->d_om, om, sizeof(double) * 2 * nb,
                                      hipMemcpyHostToDevice, ctx->stream));
@@ -959,27 +966,56 @@ static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
     if (ctx->prof) hipEventRecord(ctx->ev[2], ctx->stream);
     hipLaunchKernelGGL(k_reduce, dim3(grid_for((int64_t)nb * ctx->N * ctx->N)), dim3(256), 0,
                        ctx->stream, ctx->d_P, ctx->last_S, ctx->Np, (int)ctx->N, nb, ctx->G,
-                       ctx->d_Nk, ctx->d_M);
+                       ctx->d_Nk, ctx->d_M + (size_t)slot * ctx->N * ctx->N);
     if (ctx->prof) hipEventRecord(ctx->ev[3], ctx->stream);
+    KLSP_HIP(ctx, hipGetLastError());
+    add_build_timing(ctx, nb);
+    return KLSP_OK;
+}
+
+// With profiling on: waits for the build just enqueued and adds its phases.
+static void add_build_timing(klsp_ctx* ctx, int nb)
+{
+    if (!ctx->prof) return;
+    hipEventSynchronize(ctx->ev[3]);
+    float a = 0, b = 0, c = 0;
+    hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
+    hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]);
+    hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]);
+    ctx->last.weights_ms += a;
+    ctx->last.contract_ms += b;
+    ctx->last.reduce_ms += c;
+    ctx->last.n_omega += nb;
+    ctx->last.contract_launches += 1;
+}
+
+// LU launch over matrices [0, nd) of ctx->d_M (coef as k_lu_det), dets to
+// ctx->d_det, with its device time added to the profile.
+static int launch_lu(klsp_ctx* ctx, int nd, const double2* coef, int nt)
+{
+    if (ctx->prof) hipEventRecord(ctx->ev[4], ctx->stream);
+    hipLaunchKernelGGL(k_lu_det, dim3(nd), dim3(kLuThreads), 0, ctx->stream, ctx->d_M, coef, nt,
+                       ctx->d_A, (int)ctx->N, ctx->d_det);
+    if (ctx->prof) hipEventRecord(ctx->ev[5], ctx->stream);
     KLSP_HIP(ctx, hipGetLastError());
     return KLSP_OK;
 }
 
-static void add_timing(klsp_ctx* ctx, bool with_lu, int nb)
+static void add_lu_timing(klsp_ctx* ctx)
 {
     if (!ctx->prof) return;
-    hipEventSynchronize(with_lu ? ctx->ev[4] : ctx->ev[3]);
-    float a = 0, b = 0, c = 0, d = 0;
-    hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
-    hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]);
-    hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]);
-    if (with_lu) hipEventElapsedTime(&d, ctx->ev[3], ctx->ev[4]);
-    ctx->last.weights_ms += a;
-    ctx->last.contract_ms += b;
-    ctx->last.reduce_ms += c;
+    hipEventSynchronize(ctx->ev[5]);
+    float d = 0;
+    hipEventElapsedTime(&d, ctx->ev[4], ctx->ev[5]);
     ctx->last.lu_ms += d;
-    ctx->last.n_omega += nb;
-    ctx->last.contract_launches += 1;
+}
+
+// Matrices built before one LU launch (the launch runs one workgroup per
+// matrix, so up to a few dozen cost the same time as one).
+static int det_slots(const klsp_ctx* ctx)
+{
+    const int64_t fit = ((int64_t)256 << 20) / std::max<int64_t>(1, 16 * ctx->N * ctx->N);
+    return (int)std::max<int64_t>(kMaxBatch, std::min<int64_t>(64, fit));
 }
 
 static int check_ready(klsp_ctx* ctx, int precision, const char* who)
@@ -1000,21 +1036,25 @@ static int det_many(klsp_ctx* ctx, const double* om, int64_t n, int precision, d
     int rc = check_ready(ctx, precision, "klsp_det");
     if (rc) return rc;
     KLSP_HIP(ctx, hipSetDevice(ctx->device));
-    if ((rc = ensure(ctx, ctx->d_A, ctx->A_cap, (size_t)kMaxBatch * ctx->N * ctx->N))) return rc;
     // FP64 is MFMA-bound at one omega per pass (larger batches only lower the
-    // occupancy); the FP32 pass is memory-bound, so it shares W loads.
+    // occupancy); the low-precision passes share W loads among up to three.
     const int maxb = precision == KLSP_PREC_FP64 ? 1 : kMaxBatch;
-    for (int64_t i0 = 0; i0 < n; i0 += maxb) {
-        const int nb = (int)std::min<int64_t>(maxb, n - i0);
-        if ((rc = build_batch(ctx, om + 2 * i0, nb, precision))) return rc;
-        hipLaunchKernelGGL(k_lu_det, dim3(nb), dim3(kLuThreads), 0, ctx->stream, ctx->d_M,
-                           (const double2*)nullptr, 0, ctx->d_A, (int)ctx->N, ctx->d_det);
-        if (ctx->prof) hipEventRecord(ctx->ev[4], ctx->stream);
-        KLSP_HIP(ctx, hipGetLastError());
-        KLSP_HIP(ctx, hipMemcpyAsync(out + 2 * i0, ctx->d_det, sizeof(double2) * nb,
+    const int slots = (int)std::min<int64_t>(det_slots(ctx), round_up(n, maxb));
+    const size_t NN = (size_t)ctx->N * ctx->N;
+    if ((rc = ensure(ctx, ctx->d_M, ctx->M_cap, (size_t)std::max(slots, kMaxBatch) * NN))) return rc;
+    if ((rc = ensure(ctx, ctx->d_A, ctx->A_cap, (size_t)std::max(slots, kMaxBatch) * NN))) return rc;
+    if ((rc = ensure(ctx, ctx->d_det, ctx->det_cap, (size_t)std::max(slots, kMaxBatch)))) return rc;
+    for (int64_t i0 = 0; i0 < n; i0 += slots) {
+        const int nd = (int)std::min<int64_t>(slots, n - i0);
+        for (int j0 = 0; j0 < nd; j0 += maxb) {
+            const int nb = std::min(maxb, nd - j0);
+            if ((rc = build_batch(ctx, om + 2 * (i0 + j0), nb, precision, false, j0))) return rc;
+        }
+        if ((rc = launch_lu(ctx, nd, nullptr, 0))) return rc;
+        KLSP_HIP(ctx, hipMemcpyAsync(out + 2 * i0, ctx->d_det, sizeof(double2) * nd,
                                      hipMemcpyDeviceToHost, ctx->stream));
         KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        add_timing(ctx, true, nb);
+        add_lu_timing(ctx);
     }
     return KLSP_OK;
 }
@@ -1044,19 +1084,11 @@ static int model_dets(klsp_ctx* ctx, int nt, const std::complex<double>* cf, int
     for (int i = 0; i < nd * nt; ++i) coef[i] = make_double2(cf[i].real(), cf[i].imag());
     KLSP_HIP(ctx, hipMemcpyAsync(ctx->d_coef, coef, sizeof(double2) * nd * nt,
                                  hipMemcpyHostToDevice, ctx->stream));
-    if (ctx->prof) hipEventRecord(ctx->ev[3], ctx->stream);
-    hipLaunchKernelGGL(k_lu_det, dim3(nd), dim3(kLuThreads), 0, ctx->stream, ctx->d_M,
-                       (const double2*)ctx->d_coef, nt, ctx->d_A, (int)ctx->N, ctx->d_det);
-    if (ctx->prof) hipEventRecord(ctx->ev[4], ctx->stream);
-    KLSP_HIP(ctx, hipGetLastError());
+    if ((rc = launch_lu(ctx, nd, ctx->d_coef, nt))) return rc;
     KLSP_HIP(ctx, hipMemcpyAsync(dets, ctx->d_det, sizeof(double2) * nd, hipMemcpyDeviceToHost,
                                  ctx->stream));
     KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (ctx->prof) {
-        float ms = 0;
-        hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]);
-        ctx->last.lu_ms += ms;
-    }
+    add_lu_timing(ctx);
     return KLSP_OK;
 }
 
@@ -1102,7 +1134,6 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
         const double om[2] = {w.real(), w.imag()};
         if ((rc = build_batch(ctx, om, nt, lp_prec, true))) return rc;
         KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        add_timing(ctx, false, nt);
         acc_timing(acc, ctx);
         ++n_lp;
         const double rho = 0.25 * w.imag();
@@ -1153,7 +1184,6 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
             const double om[2] = {w.real(), w.imag()};
             if ((rc = build_batch(ctx, om, 1, KLSP_PREC_FP64))) return rc;   // d_M[0]
             KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            add_timing(ctx, false, 1);
             acc_timing(acc, ctx);
             ++n_64;
         }
@@ -1222,7 +1252,6 @@ int klsp_build_M(klsp_ctx* ctx, double om_re, double om_im, int precision, doubl
     KLSP_HIP(ctx, hipMemcpyAsync(M_out, ctx->d_M, sizeof(double2) * ctx->N * ctx->N,
                                  hipMemcpyDeviceToHost, ctx->stream));
     KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    add_timing(ctx, false, 1);
     return KLSP_OK;
 }
 
@@ -1241,7 +1270,6 @@ int klsp_build_taylor(klsp_ctx* ctx, double om_re, double om_im, int n_terms, in
     KLSP_HIP(ctx, hipMemcpyAsync(T_out, ctx->d_M, sizeof(double2) * n_terms * ctx->N * ctx->N,
                                  hipMemcpyDeviceToHost, ctx->stream));
     KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    add_timing(ctx, false, n_terms);
     return KLSP_OK;
 }
 

@@ -333,3 +333,31 @@ def test_mixed_newton_respects_bounds(oracle_small):
     res, H = pre.ctx.newton(0.44, 1e-6, max_iter=30, tol=1e-12, precision=_ffi.PREC_MIXED)
     assert np.all(H[:, 1] >= 1e-4) and np.all((H[:, 0] >= 0) & (H[:, 0] <= 1))
     assert res.gamma >= 1e-4
+
+
+# --------------------------------------------------------------------------
+# Batched omega-grid scan (64 x 64, Re w/m in [0.30, 0.60], Im w in
+# [0.02, 0.25]) in the low precisions against the FP64 scan
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("prec,tol", [(_ffi.PREC_FP32, 1e-4), (_ffi.PREC_BF16X3, 2e-3)])
+def test_grid_scan_64x64_low_precision(small_cfg_paths, prec, tol):
+    from tests.prng import SplitMix64
+    cfg = K.load_model(K.load_config(small_cfg_paths[0]), small_cfg_paths[1])
+    model = K.create_toomre_zang_model(cfg)
+    pre = K.precompute_all(cfg, model, K.GPUType.AMD)
+    ctx = pre.ctx
+    win = (64, 64, 0.30, 0.60, 0.02, 0.25)
+    g64, op64, ga64 = ctx.grid_scan(*win, precision=_ffi.PREC_FP64)
+    glp, oplp, galp = ctx.grid_scan(*win, precision=prec)
+    assert g64.shape == (64, 64) and np.all(np.isfinite(g64)) and np.all(np.isfinite(glp))
+    # error relative to the local |det| scale (the larger of the point and
+    # the map's median, so that near-zeros of det do not dominate)
+    scale = np.maximum(np.abs(g64), np.median(np.abs(g64)))
+    err = np.abs(glp - g64) / scale
+    assert np.max(err) <= tol, np.max(err)
+    rng = SplitMix64(0)
+    idx = [(rng.below(64), rng.below(64)) for _ in range(64)]
+    for i, j in idx:
+        assert abs(glp[i, j] - g64[i, j]) <= tol * scale[i, j]
+    # the same minimum of |det| on the grid
+    assert (oplp, galp) == (op64, ga64)

@@ -329,3 +329,16 @@ def test_grid_scan_argmin_column_major():
     dg, op, g = oracle.grid_scan(pre, 4, 3, 0.2, 0.5, 0.01, 0.2)
     assert np.all(dg == 1.0)          # M = 0 -> det(I) = 1 everywhere; first index wins
     assert (op, g) == (0.2, 0.01)
+
+
+def test_splitmix64_reference_values():
+    """First outputs of SplitMix64 from seed 0 and seed 1234567 (the values
+    published with Vigna's reference C implementation)."""
+    from tests.prng import SplitMix64
+    g = SplitMix64(0)
+    assert [g.next_u64() for _ in range(3)] == [
+        0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    g = SplitMix64(1234567)
+    assert [g.next_u64() for _ in range(5)] == [
+        6457827717110365317, 3203168211198807973, 9817491932198370423,
+        4593380528125082431, 16408922859458223821]
