@@ -53,6 +53,11 @@ def main() -> int:
     ap.add_argument("--cpu-slabs", type=int, default=-1,
                     help="l-slab ZGEMMs timed for the CPU baseline (-1 = all n_l, 0 = skip)")
     ap.add_argument("--no-newton", action="store_true")
+    ap.add_argument("--uneven", type=int, default=0,
+                    help="also run this many draws of the 48-draw uneven batch of Newton "
+                         "searches (mixed precision; minutes at full size)")
+    ap.add_argument("--uneven-fp64", action="store_true",
+                    help="run the uneven draws in FP64 too and report the root differences")
     ap.add_argument("--no-fp32", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -224,6 +229,39 @@ def main() -> int:
             f"Taylor passes + {rm.n_fp64_builds} FP64 dets, |d omega| vs FP64 root {dw:.2e}, "
             f"|det|={abs(rm.det_val):.2e}; contractions {tmx['contract_ms']:.1f} ms, "
             f"LUs {tmx['lu_ms']:.1f} ms")
+
+    if args.uneven > 0:
+        from kalnajs_log_spirals_amd.UnevenBatch import draw_batch, run_batch
+        ctx.close()
+        draws = draw_batch(48, 0)[:args.uneven]
+        t0 = time.perf_counter()
+        rm = run_batch(draws, precision="mixed", log=log)
+        wall = time.perf_counter() - t0
+        conv = [r for r in rm if r.converged]
+        tn = sum(r.newton_s for r in rm)
+        extra.update({"uneven_draws": len(draws), "uneven_converged_mixed": len(conv),
+                      "uneven_wall_s_mixed": wall, "uneven_newton_s_mixed": tn,
+                      "uneven_roots_per_s_mixed": len(conv) / tn if tn > 0 else 0.0,
+                      "uneven_W_s": sum(r.W_s for r in rm),
+                      "uneven_host_s": sum(r.host_s for r in rm)})
+        log(f"[bench] uneven batch (mixed): {len(conv)}/{len(draws)} converged, Newton {tn:.1f} s, "
+            f"wall {wall:.1f} s")
+        if args.uneven_fp64:
+            t0 = time.perf_counter()
+            r64 = run_batch(draws, precision="fp64", log=log)
+            wall64 = time.perf_counter() - t0
+            tn64 = sum(r.newton_s for r in r64)
+            both = [(a, b) for a, b in zip(rm, r64) if a.converged and b.converged]
+            dws = [abs(complex(a.draw.m * a.Omega_p, a.gamma) - complex(b.draw.m * b.Omega_p, b.gamma))
+                   for a, b in both]
+            extra.update({"uneven_converged_fp64": sum(r.converged for r in r64),
+                          "uneven_newton_s_fp64": tn64, "uneven_wall_s_fp64": wall64,
+                          "uneven_roots_per_s_fp64":
+                              sum(r.converged for r in r64) / tn64 if tn64 > 0 else 0.0,
+                          "uneven_max_abs_domega": max(dws) if dws else None,
+                          "uneven_converged_both": len(both)})
+            log(f"[bench] uneven batch (FP64): {sum(r.converged for r in r64)} converged, Newton "
+                f"{tn64:.1f} s; max |d omega| mixed vs FP64 {max(dws) if dws else float('nan'):.2e}")
 
     cpu = None
     if args.cpu_slabs < 0:

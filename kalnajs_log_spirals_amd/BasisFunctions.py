@@ -120,16 +120,11 @@ def _orbit_inputs(od: OrbitData):
             NR * Ne, nwa)
 
 
-def precompute_all(config: KalnajsConfig, model, backend=GPUBackend.GPUType.AMD,
-                   verbose: bool = False) -> PrecomputedData:
-    """BasisFunctions.jl:73-165: grids, orbits, alpha grid, W_l (device),
-    N_m(alpha), DJ / F0l / Omega_res."""
-    _require_device(backend, "precompute_all")
-    m, G = int(config.physics.m), float(config.physics.G)
+def precompute_host(config: KalnajsConfig, model, verbose: bool = False) -> dict:
+    """Host half of precompute_all (BasisFunctions.jl:73-153 without W_l):
+    grids, orbits, alpha grid, N_m(alpha) and DJ / F0l / Omega_res."""
+    m = int(config.physics.m)
     N = int(config.grid.N_alpha)
-    l_arr = np.arange(config.grid.l_min, config.grid.l_max + 1, dtype=np.int32)
-    n_l = l_arr.size
-    NPh = config.grid.NR * config.grid.Ne
     if verbose:
         print("Computing grids...")
     grids = compute_grids(config, model)
@@ -140,16 +135,36 @@ def precompute_all(config: KalnajsConfig, model, backend=GPUBackend.GPUType.AMD,
     d_alpha = trapezoid_coef(alpha)
     Nk = compute_N_kernel(m, alpha, d_alpha)
     DJ, F0l, Ores = _omega_independent(config, od)
+    return dict(od=od, alpha=alpha, d_alpha=d_alpha, Nk=Nk, DJ=DJ, F0l=F0l, Ores=Ores)
+
+
+def precompute_device(config: KalnajsConfig, host: dict, verbose: bool = False) -> PrecomputedData:
+    """Device half of precompute_all: a context with the problem arrays and
+    W_l (BasisFunctions.jl:155-165) computed on the device."""
+    m, G = int(config.physics.m), float(config.physics.G)
+    N = int(config.grid.N_alpha)
+    l_arr = np.arange(config.grid.l_min, config.grid.l_max + 1, dtype=np.int32)
+    NPh = config.grid.NR * config.grid.Ne
     if verbose:
         print("Computing W_l basis functions (complex, on device)...")
     ctx = _ffi.Context(GPUBackend.current_device())
-    ctx.upload_problem(NPh, N, n_l, m, G, DJ, F0l, Ores, Nk)
-    w1, ra, pha, o1, o2, _, _ = _orbit_inputs(od)
-    ctx.compute_W(w1, ra, pha, o1, o2, alpha, l_arr)
+    ctx.upload_problem(NPh, N, l_arr.size, m, G, host["DJ"], host["F0l"], host["Ores"],
+                       host["Nk"])
+    w1, ra, pha, o1, o2, _, _ = _orbit_inputs(host["od"])
+    ctx.compute_W(w1, ra, pha, o1, o2, host["alpha"], l_arr)
     if verbose:
         print("Precomputation complete.")
-    return PrecomputedData(m, G, N, n_l, l_arr.astype(np.int64), NPh, alpha, d_alpha, od, DJ,
-                           F0l, Ores, Nk, ctx=ctx)
+    return PrecomputedData(m, G, N, l_arr.size, l_arr.astype(np.int64), NPh, host["alpha"],
+                           host["d_alpha"], host["od"], host["DJ"], host["F0l"], host["Ores"],
+                           host["Nk"], ctx=ctx)
+
+
+def precompute_all(config: KalnajsConfig, model, backend=GPUBackend.GPUType.AMD,
+                   verbose: bool = False) -> PrecomputedData:
+    """BasisFunctions.jl:73-165: grids, orbits, alpha grid, W_l (device),
+    N_m(alpha), DJ / F0l / Omega_res."""
+    _require_device(backend, "precompute_all")
+    return precompute_device(config, precompute_host(config, model, verbose), verbose)
 
 
 def compute_W_l(config: KalnajsConfig, orbit_data: OrbitData, alpha_arr, l_arr,

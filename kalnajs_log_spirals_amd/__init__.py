@@ -19,5 +19,6 @@ from .MatrixBuilder import (compute_determinant, compute_determinant_batched,  #
                             build_M_matrix)
 from .NewtonSolver import NewtonResult, newton_search  # noqa: F401
 from .GridScan import grid_scan, find_minimum  # noqa: F401
+from . import UnevenBatch  # noqa: F401
 
 VERSION = "0.1.0"

@@ -45,7 +45,7 @@ typedef unsigned v2u __attribute__((ext_vector_type(2)));
 constexpr int kMB = kBf16MB;            // beta blocks per wave
 constexpr int kNB = kBf16NB;            // alpha blocks per workgroup
 constexpr int kTA = kNB * 16;           // alpha columns per workgroup
-constexpr int kMaxNW = 3;
+constexpr int kMaxNW = kMaxW;
 constexpr int kThreads = kBf16Waves * 64;
 static_assert(kThreads == 16 * kTA, "one B-producing thread per K row and column");
 
@@ -277,26 +277,28 @@ const void* kernel_bf16()
 
 const void* kernel_for(int nw)
 {
-    return nw <= 1 ? kernel_bf16<1>() : nw == 2 ? kernel_bf16<2>() : kernel_bf16<3>();
+    return nw <= 1 ? kernel_bf16<1>() : nw == 2 ? kernel_bf16<2>()
+         : nw == 3 ? kernel_bf16<3>() : kernel_bf16<4>();
 }
 
 }  // namespace
 
-Bf16Geometry contract_bf16_geometry(int Np)
+Bf16Geometry contract_bf16_geometry(int Np, int nw)
 {
-    static int occ = 0;
-    if (occ == 0) {
+    static int occ[kMaxNW + 1] = {};
+    const int i = nw < 1 ? 1 : (nw > kMaxNW ? kMaxNW : nw);
+    if (occ[i] == 0) {
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_for(3), kThreads, 0) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_for(i), kThreads, 0) !=
                 hipSuccess || per_cu < 1)
             per_cu = 1;
-        occ = per_cu;
+        occ[i] = per_cu;
     }
     const int nb = Np / 16;
     Bf16Geometry g;
     g.wrows = (nb + kBf16Waves * kMB - 1) / (kBf16Waves * kMB);
     g.wcols = (nb + kNB - 1) / kNB;
-    g.wg_per_cu = occ;
+    g.wg_per_cu = occ[i];
     return g;
 }
 

@@ -117,7 +117,8 @@ __global__ __launch_bounds__(256) void k_orbit_prep(
 
 // W_k = A_k B_k per orbit on FP64 MFMA: A[l, t] = wt_t cos(l w1_t + m phi_t)
 // (real, n_l x nwa), B[t, a] = exp(-i alpha_a ln r_t) (nwa x N).
-// Workgroup = (orbit k, 32 alphas); wave w owns l-blocks w, w+4, w+8, w+12.
+// Workgroup = (orbit k, 32 alphas, group z of 16 l-blocks); wave w owns the
+// group's l-blocks w, w+4, w+8, w+12.
 // Operands are generated in registers in the MFMA lane layout:
 //   A (16 l x 4 t): lane = i + 16*kk holds A[l0+i][t0+kk]
 //   B (4 t x 16 a): lane = i + 16*kk holds B[t0+kk][a0+i]
@@ -138,6 +139,7 @@ __global__ __launch_bounds__(256) void k_W_f64(
     const int i = lane & 15, kk = lane >> 4;
     const int a0 = blockIdx.y * 32;
     const int nLB = (n_l + 15) / 16;
+    const int lb0 = blockIdx.z * 4 * kWMaxLB + wave;   // first l-block of this wave
     const bool bj0 = a0 < N, bj1 = a0 + 16 < N;
     double al[2];
     for (int j = 0; j < 2; ++j) {
@@ -146,7 +148,7 @@ __global__ __launch_bounds__(256) void k_W_f64(
     }
     double lv[kWMaxLB];
     for (int q = 0; q < kWMaxLB; ++q) {
-        const int li = (wave + 4 * q) * 16 + i;
+        const int li = (lb0 + 4 * q) * 16 + i;
         lv[q] = li < n_l ? (double)l_arr[li] : 0.0;
     }
     v4d accr[kWMaxLB][2], acci[kWMaxLB][2];
@@ -167,7 +169,7 @@ __global__ __launch_bounds__(256) void k_W_f64(
         }
 #pragma unroll
         for (int q = 0; q < kWMaxLB; ++q) {
-            if (wave + 4 * q >= nLB) break;
+            if (lb0 + 4 * q >= nLB) break;
             const double a = wtt * cos(lv[q] * w1t + mp);
             if (bj0) {
                 accr[q][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, br[0], accr[q][0], 0, 0, 0);
@@ -181,7 +183,7 @@ __global__ __launch_bounds__(256) void k_W_f64(
     }
 #pragma unroll
     for (int q = 0; q < kWMaxLB; ++q) {
-        const int lb = wave + 4 * q;
+        const int lb = lb0 + 4 * q;
         if (lb >= nLB) break;
         for (int j = 0; j < 2; ++j) {
             const int a = a0 + 16 * j + i;
@@ -596,9 +598,9 @@ static GangPlan plan_gangs(const klsp_ctx* ctx, const ContractShape& sh)
 // Gang geometry of the BF16 contraction: workgroup tiles of 320 beta rows x
 // 16 alpha columns; gangs per XCD chosen so that the gang workgroups fill the
 // XCD's resident workgroup slots in whole rounds where possible.
-static GangPlan plan_gangs_bf16(const klsp_ctx* ctx)
+static GangPlan plan_gangs_bf16(const klsp_ctx* ctx, int nw)
 {
-    const Bf16Geometry geo = contract_bf16_geometry(ctx->Np);
+    const Bf16Geometry geo = contract_bf16_geometry(ctx->Np, nw);
     GangPlan gp;
     gp.wrows = geo.wrows;
     gp.wcols = geo.wcols;
@@ -635,7 +637,8 @@ static int max_gangs(const klsp_ctx* ctx)
     int g = 8;
     for (int prec : {KLSP_PREC_FP64, KLSP_PREC_FP32})
         for (int nw = 1; nw <= kMaxBatch; ++nw) g = std::max(g, plan_gangs(ctx, shape_for(prec, nw)).n_gangs);
-    return std::max(g, plan_gangs_bf16(ctx).n_gangs);
+    for (int nw = 1; nw <= kMaxW; ++nw) g = std::max(g, plan_gangs_bf16(ctx, nw).n_gangs);
+    return g;
 }
 
 static int free_problem(klsp_ctx* ctx)
@@ -682,7 +685,7 @@ int klsp_create(int device, klsp_ctx** ctx_out)
     }
     for (auto& e : ctx->ev) hipEventCreate(&e);
     if (hipMalloc((void**)&ctx->d_om, sizeof(double) * 2 * kMaxBatch) != hipSuccess ||
-        hipMalloc((void**)&ctx->d_coef, sizeof(double2) * kMaxBatch * kMaxBatch) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_coef, sizeof(double2) * 3 * kMaxW) != hipSuccess ||
         hipMalloc((void**)&ctx->d_det, sizeof(double2) * kMaxBatch) != hipSuccess) {
         ctx->det_cap = 0;
         klsp_destroy(ctx);
@@ -762,8 +765,6 @@ int klsp_compute_W(klsp_ctx* ctx, int64_t nwa, const double* w1, const double* r
         return fail(ctx, KLSP_ERR_NOT_READY, "klsp_compute_W: upload the problem first");
     if (nwa < 2 || !w1 || !ra || !pha || !Om1 || !Om2 || !alpha || !l_arr)
         return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_compute_W: bad arguments");
-    if (ctx->n_l > 16 * 4 * kWMaxLB)
-        return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_compute_W: n_l exceeds 256");
     KLSP_HIP(ctx, hipSetDevice(ctx->device));
     const int64_t NPh = ctx->NPh, nwa_p = round_up(nwa, 4);
     const size_t no = (size_t)NPh * nwa, np = (size_t)NPh * nwa_p;
@@ -808,7 +809,9 @@ int klsp_compute_W(klsp_ctx* ctx, int64_t nwa, const double* w1, const double* r
     hipLaunchKernelGGL(k_orbit_prep, dim3((unsigned)NPh), dim3(256), 0, ctx->stream, d_w1, d_ra,
                        d_pha, d_o1, d_o2, (int)nwa, (int)nwa_p, ctx->m, d_u, d_wt, d_x1, d_mp,
                        d_valid);
-    hipLaunchKernelGGL(k_W_f64, dim3((unsigned)NPh, (unsigned)((ctx->N + 31) / 32)), dim3(256), 0,
+    const unsigned lgroups = (unsigned)((ctx->n_l + 16 * 4 * kWMaxLB - 1) / (16 * 4 * kWMaxLB));
+    hipLaunchKernelGGL(k_W_f64, dim3((unsigned)NPh, (unsigned)((ctx->N + 31) / 32), lgroups),
+                       dim3(256), 0,
                        ctx->stream, d_u, d_wt, d_x1, d_mp, d_valid, d_al, d_l, NPh, (int)ctx->N,
                        ctx->Np, (int)ctx->n_l, (int)nwa_p, ctx->d_W64);
     hipEventRecord(ctx->ev[1], ctx->stream);
@@ -904,7 +907,7 @@ int klsp_prepare(klsp_ctx* ctx, int precision)
 static int launch_contract(klsp_ctx* ctx, int precision, int nb)
 {
     const bool bf = precision == KLSP_PREC_BF16X3;
-    const GangPlan gp = bf ? plan_gangs_bf16(ctx) : plan_gangs(ctx, shape_for(precision, nb));
+    const GangPlan gp = bf ? plan_gangs_bf16(ctx, nb) : plan_gangs(ctx, shape_for(precision, nb));
     ContractArgs a;
     a.W = precision == KLSP_PREC_FP64 ? (const void*)ctx->d_W64
           : bf                        ? (const void*)ctx->d_Wb
@@ -937,17 +940,17 @@ static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
 {
     const int64_t Kpc = ctx->Kp + kRowPad;
     int rc;
-    if ((rc = ensure(ctx, ctx->d_c, ctx->c_cap, (size_t)kMaxBatch * Kpc))) return rc;
+    if ((rc = ensure(ctx, ctx->d_c, ctx->c_cap, (size_t)kMaxW * Kpc))) return rc;
     float2* cf = nullptr;
     if (precision == KLSP_PREC_BF16X3) {
-        if ((rc = ensure(ctx, ctx->d_cf, ctx->cf_cap, (size_t)kMaxBatch * Kpc))) return rc;
+        if ((rc = ensure(ctx, ctx->d_cf, ctx->cf_cap, (size_t)kMaxW * Kpc))) return rc;
         cf = ctx->d_cf;
     }
     const int mg = max_gangs(ctx);
-    if ((rc = ensure(ctx, ctx->d_P, ctx->P_cap, (size_t)kMaxBatch * mg * ctx->Np * ctx->Np)))
+    if ((rc = ensure(ctx, ctx->d_P, ctx->P_cap, (size_t)kMaxW * mg * ctx->Np * ctx->Np)))
         return rc;
     if ((rc = ensure(ctx, ctx->d_M, ctx->M_cap,
-                     (size_t)std::max(kMaxBatch, slot + nb) * ctx->N * ctx->N)))
+                     (size_t)std::max(kMaxW, slot + nb) * ctx->N * ctx->N)))
         return rc;
     if (!taylor)
         KLSP_HIP(ctx, hipMemcpyAsync(ctx->d_om, om, sizeof(double) * 2 * nb,
@@ -1041,7 +1044,7 @@ static int det_many(klsp_ctx* ctx, const double* om, int64_t n, int precision, d
     const int maxb = precision == KLSP_PREC_FP64 ? 1 : kMaxBatch;
     const int slots = (int)std::min<int64_t>(det_slots(ctx), round_up(n, maxb));
     const size_t NN = (size_t)ctx->N * ctx->N;
-    if ((rc = ensure(ctx, ctx->d_M, ctx->M_cap, (size_t)std::max(slots, kMaxBatch) * NN))) return rc;
+    if ((rc = ensure(ctx, ctx->d_M, ctx->M_cap, (size_t)std::max(slots, kMaxW) * NN))) return rc;
     if ((rc = ensure(ctx, ctx->d_A, ctx->A_cap, (size_t)std::max(slots, kMaxBatch) * NN))) return rc;
     if ((rc = ensure(ctx, ctx->d_det, ctx->det_cap, (size_t)std::max(slots, kMaxBatch)))) return rc;
     for (int64_t i0 = 0; i0 < n; i0 += slots) {
@@ -1074,13 +1077,13 @@ static void acc_timing(klsp_timing& acc, klsp_ctx* ctx)
 }
 
 // dets of I - sum_j cf[i*nt + j] T_j for nd coefficient sets over the nt
-// matrices T_j in ctx->d_M (nd, nt <= kMaxBatch).
+// matrices T_j in ctx->d_M (nd <= 3, nt <= kMaxW).
 static int model_dets(klsp_ctx* ctx, int nt, const std::complex<double>* cf, int nd,
                       std::complex<double>* dets)
 {
     int rc;
     if ((rc = ensure(ctx, ctx->d_A, ctx->A_cap, (size_t)kMaxBatch * ctx->N * ctx->N))) return rc;
-    double2 coef[kMaxBatch * kMaxBatch];
+    double2 coef[3 * kMaxW];
     for (int i = 0; i < nd * nt; ++i) coef[i] = make_double2(cf[i].real(), cf[i].imag());
     KLSP_HIP(ctx, hipMemcpyAsync(ctx->d_coef, coef, sizeof(double2) * nd * nt,
                                  hipMemcpyHostToDevice, ctx->stream));
@@ -1122,13 +1125,19 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
     cd w(m * np->Omega_p_init, np->gamma_init);
     w = bounded(w);
     cd wc = w;                          // centre of the last low-precision pass
-    // 1. low-precision passes on the Taylor model.  A pass with nt terms
-    // leaves the root off by ~(|move| / Im w)^nt |move|; when that predicts a
-    // next move below 1e-4, the next pass builds two terms only (its own
-    // truncation, ~(1e-4 / Im w)^2 1e-4, is then far below the low-precision
+    // 1. low-precision passes on the Taylor model.  A pass with nt terms that
+    // moved by m leaves the root off by e ~ r^nt m and its slope off by
+    // ~nt r^(nt-1) (relative), r = m / Im w (the series converges for
+    // |s| < Im w).  The anchored finish below corrects e with that slope, so
+    // the passes stop once e < 1e-6 and e nt r^(nt-1) < 1e-12 (or m < 1e-6);
+    // from the default start (m = 4e-3, r = 0.03) one 4-term BF16x3 pass
+    // suffices.  When e < 1e-4 the next pass builds two terms only (its own
+    // truncation, ~(1e-4 / Im w)^2 1e-4, is far below the low-precision
     // error).  A pass ends once a model Newton step below accept_step is
     // applied: from there the step error is ~|step|^2 / Im w.
-    int nt = 3;                         // Taylor terms of the current pass
+    const int nt_max = lp_prec == KLSP_PREC_BF16X3 ? kMaxTerms : kMaxBatch;
+    int nt = nt_max;                    // Taylor terms of the next pass
+    int nt_last = nt;                   // ... of the last pass built
     for (int pass = 0; pass < np->max_iter; ++pass) {
         wc = w;
         const double om[2] = {w.real(), w.imag()};
@@ -1136,11 +1145,12 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
         KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
         acc_timing(acc, ctx);
         ++n_lp;
+        nt_last = nt;
         const double rho = 0.25 * w.imag();
         cd s = 0.0, D0 = 0.0;
         for (int it = 0; it < 12; ++it) {
             const cd pts[3] = {s, s + h, s - h};
-            cd cf[3 * kMaxBatch], d[3];
+            cd cf[3 * kMaxW], d[3];
             for (int i = 0; i < 3; ++i) {     // M(w_c + s) = sum_j (-s)^j T_{j+1}
                 cd p = 1.0;
                 for (int j = 0; j < nt; ++j, p *= -pts[i]) cf[i * nt + j] = p;
@@ -1163,13 +1173,15 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
         w = wn;
         if (!(moved >= switch_step)) break;
         const double r = moved / std::max(wn.imag(), 1e-300);
-        nt = std::pow(r, nt) * moved < 1e-4 ? 2 : 3;
+        const double e = std::pow(r, nt) * moved, slope_err = nt * std::pow(r, nt - 1);
+        if (e < switch_step && e * slope_err < 1e-12) break;
+        nt = e < 1e-4 ? 2 : nt_max;
     }
-    const int nt_last = nt;
     // 2. FP64 finish on the anchored model.  One FP64 build at the
     // low-precision root w_a gives M64(w_a); with s = w - w_c and s_a = w_a - w_c
     //   M(w) = M64(w_a) + [M_lp(w) - M_lp(w_a)]
-    //        = M64(w_a) - (s - s_a) T_2 + (s - s_a)(s + s_a) T_3 + O(|s|^2 |s - s_a|),
+    //        = M64(w_a) + sum_{j>=1} ((-s)^j - (-s_a)^j) T_{j+1}
+    //        = M64(w_a) - (s - s_a) T_2 + (s - s_a)(s + s_a) T_3 - ...,
     // so the low-precision terms enter only multiplied by |s - s_a| (<= 1e-6
     // after the switch): their ~1e-5 relative error then moves M by ~1e-11
     // |s - s_a| |T_2|, below FP64 rounding at the root.  (|det| < tol alone is
@@ -1193,12 +1205,17 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
         bool far = false;
         for (int it = 0; it < 30; ++it) {
             const cd pts[3] = {s, s + h64, s - h64};
-            cd cf[9], d[3];
+            cd cf[3 * kMaxW], d[3];
             for (int i = 0; i < 3; ++i) {
-                const cd ds = pts[i] - sa;
+                // (-s)^j - (-s_a)^j = (-1)^j (s - s_a) sum_{q<j} s^q s_a^(j-1-q), free
+                // of the cancellation of the difference of powers
+                const cd si = pts[i], ds = si - sa;
                 cf[nt_last * i] = 1.0;
-                cf[nt_last * i + 1] = -ds;
-                if (nt_last == 3) cf[nt_last * i + 2] = ds * (pts[i] + sa);
+                for (int j = 1; j < nt_last; ++j) {
+                    cd sum = 0.0, sp = 1.0;
+                    for (int q = 0; q < j; ++q, sp *= si) sum += sp * std::pow(sa, j - 1 - q);
+                    cf[nt_last * i + j] = ((j & 1) ? -1.0 : 1.0) * ds * sum;
+                }
             }
             if ((rc = model_dets(ctx, nt_last, cf, 3, d))) return rc;
             acc_timing(acc, ctx);
@@ -1259,7 +1276,8 @@ int klsp_build_taylor(klsp_ctx* ctx, double om_re, double om_im, int n_terms, in
                       double* T_out)
 {
     if (!ctx) return KLSP_ERR_INVALID_ARG;
-    if (!T_out || n_terms < 1 || n_terms > kMaxBatch)
+    if (!T_out || n_terms < 1 ||
+        n_terms > (precision == KLSP_PREC_BF16X3 ? kMaxTerms : kMaxBatch))
         return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_build_taylor: bad arguments");
     int rc = check_ready(ctx, precision, "klsp_build_taylor");
     if (rc) return rc;

@@ -18,6 +18,8 @@ constexpr int kColPad = 16;     // alpha padding of the device W rows
 constexpr int kKStep = 4;       // K per MFMA step (16x16x4 f64/f32 MFMA)
 constexpr int kRowPad = 16;     // zero rows past Kp in W and c (prefetch slack)
 constexpr int kMaxBatch = 3;    // omegas sharing one pass over W
+constexpr int kMaxTerms = 4;    // Taylor terms of one BF16x3 pass
+constexpr int kMaxW = kMaxTerms > kMaxBatch ? kMaxTerms : kMaxBatch;  // weight sets per pass
 
 inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 

@@ -1,6 +1,6 @@
-"""SplitMix64 (Steele, Lea & Flood 2014): the seeded sampler the scan and
-uneven-batch tests draw from, so that the draws do not depend on numpy's
-generator versions."""
+"""SplitMix64 (Steele, Lea & Flood 2014): the seeded generator the uneven
+batch of Newton searches and the scan sample points are drawn from, so that
+the draws do not depend on numpy's generator versions."""
 from __future__ import annotations
 
 MASK = (1 << 64) - 1

@@ -39,3 +39,24 @@ def test_save_load_round_trip(tmp_path, small_cfg_paths):
     d = K.load_config(str(f))
     assert d.grid == c.grid and d.scan == c.scan and d.physics == c.physics
     assert d.model == c.model
+
+
+def test_uneven_batch_draws_match_fixture():
+    """The 48 draws of the uneven batch (SplitMix64 seed 0) are the recorded
+    ones, and every draw lies in its stated ranges."""
+    import json
+    from dataclasses import asdict
+    from kalnajs_log_spirals_amd.UnevenBatch import draw_batch, config_for
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "uneven_batch_draws.json")))
+    draws = draw_batch(48, 0)
+    assert [asdict(d) for d in draws] == fx["draws"]
+    for d in draws:
+        assert d.n_zang in (2, 3, 4, 5, 6) and d.q1 in (4, 6, 7, 9, 12) and d.m in (1, 2, 3)
+        assert 201 <= d.NR <= 801 and 21 <= d.Ne <= 101 and 101 <= d.N_alpha <= 401
+        assert 50 <= d.l_max <= 150 and 0.3 <= d.Omega_p0 < 0.6 and 0.05 <= d.gamma0 < 0.2
+    c = config_for(draws[0])
+    assert (c.grid.NR, c.grid.Ne, c.grid.l_min, c.grid.nwa) == (draws[0].NR, draws[0].Ne,
+                                                               -draws[0].l_max, 2001)
+    cs = config_for(draws[0], scale=0.1)
+    assert cs.grid.NR == max(11, round(draws[0].NR * 0.1)) and cs.grid.nwa == 201

@@ -68,6 +68,22 @@ def test_W_quadrature_matches_oracle(oracle_tiny):
     assert np.array_equal(W2, W)
 
 
+def test_W_quadrature_many_harmonics():
+    """n_l = 301 (l = -150..150, the uneven batch's largest): more l-blocks than
+    one workgroup covers, so the grid's third dimension is used."""
+    import oracle
+    p = oracle.Params(NR=5, Ne=3, Rc_min=0.1, Rc_max=10.0, v_min=0.05, N_alpha=9,
+                      alpha_max=5.0, l_min=-150, l_max=150, nw=401, nwa=41)
+    po = oracle.precompute(p)
+    cfg = cfg_from_params(p)
+    pre = K.precompute_all(cfg, K.create_toomre_zang_model(cfg), K.GPUType.AMD)
+    W = pre.W_l_mat
+    assert W.shape == po["W"].shape == (15, 9, 301)
+    for il in (0, 150, 255, 256, 300):
+        assert relnorm(W[:, :, il], po["W"][:, :, il]) <= 1e-12, il
+    assert relnorm(W, po["W"]) <= 1e-12
+
+
 def test_W_quadrature_small_grid(oracle_small, small_cfg_paths):
     cfg = K.load_model(K.load_config(small_cfg_paths[0]), small_cfg_paths[1])
     pre = K.precompute_all(cfg, K.create_toomre_zang_model(cfg), K.GPUType.AMD)
@@ -341,7 +357,7 @@ def test_mixed_newton_respects_bounds(oracle_small):
 # --------------------------------------------------------------------------
 @pytest.mark.parametrize("prec,tol", [(_ffi.PREC_FP32, 1e-4), (_ffi.PREC_BF16X3, 2e-3)])
 def test_grid_scan_64x64_low_precision(small_cfg_paths, prec, tol):
-    from tests.prng import SplitMix64
+    from kalnajs_log_spirals_amd.prng import SplitMix64
     cfg = K.load_model(K.load_config(small_cfg_paths[0]), small_cfg_paths[1])
     model = K.create_toomre_zang_model(cfg)
     pre = K.precompute_all(cfg, model, K.GPUType.AMD)
@@ -361,3 +377,62 @@ def test_grid_scan_64x64_low_precision(small_cfg_paths, prec, tol):
         assert abs(glp[i, j] - g64[i, j]) <= tol * scale[i, j]
     # the same minimum of |det| on the grid
     assert (oplp, galp) == (op64, ga64)
+
+
+# --------------------------------------------------------------------------
+# Uneven batch of Newton searches (draws of the 48-draw batch, grids scaled
+# down to test size): the mixed search lands on the FP64 search's root
+# --------------------------------------------------------------------------
+def test_uneven_batch_mixed_roots_are_fp64_roots():
+    """Every converged mixed root is a root of the FP64 det to 1e-10: three
+    FP64 Newton steps with tol = 0 from it move it by at most 1e-10.  Where
+    the reference FP64 iteration from the same start reaches the same root,
+    the two agree to 1e-10 after the same polish (the reference stops at
+    |det| < tol, which on a coarse grid, |det'| ~ 1e-8, leaves ~1e-4 of
+    omega; Newton basins differ with the path, so the two may also end at
+    different roots)."""
+    from kalnajs_log_spirals_amd.UnevenBatch import config_for, draw_batch, run_batch
+    draws = draw_batch(48, 0)[:4]
+    rm = run_batch(draws, precision="mixed", scale=0.15, log=print)
+    n_same = 0
+    assert sum(a.converged for a in rm) >= 3
+    for d, a in zip(draws, rm):
+        cfg = config_for(d, scale=0.15)
+        pre = K.precompute_all(cfg, K.create_toomre_zang_model(cfg), K.GPUType.AMD)
+        wm = complex(d.m * a.Omega_p, a.gamma)
+        if a.converged:
+            pm, _ = pre.ctx.newton(a.Omega_p, a.gamma, max_iter=3, tol=0.0,
+                                   precision=_ffi.PREC_FP64)
+            assert abs(complex(pm.omega[0], pm.omega[1]) - wm) <= 1e-10
+        r64, _ = pre.ctx.newton(d.Omega_p0, d.gamma0, max_iter=50, tol=1e-12,
+                                precision=_ffi.PREC_FP64)
+        if a.converged and r64.converged:
+            pol, _ = pre.ctx.newton(r64.Omega_p, r64.gamma, max_iter=3, tol=0.0,
+                                    precision=_ffi.PREC_FP64)
+            wp = complex(pol.omega[0], pol.omega[1])
+            print("draw", d.index, "fp64", complex(r64.omega[0], r64.omega[1]), "polished", wp,
+                  "mixed", wm, flush=True)
+            if abs(wp - wm) < 1e-6:
+                n_same += 1
+                assert abs(wp - wm) <= 1e-10
+                assert a.n_fp64_builds < r64.n_det_evals
+        pre.ctx.close()
+    assert n_same >= 1
+
+
+def test_bf16x3_four_taylor_terms(oracle_small):
+    """The 4-term BF16x3 pass: T_1..T_3 as the FP64 3-term build and
+    T_4 = -dT_3/dw / 3 (d/dw (c/d^j) = -j c/d^(j+1)), the derivative taken by
+    central differences of FP64 builds (truncation ~eps^2, far below 1e-4)."""
+    pre = device_pre(oracle_small)
+    wc, eps = 2 * 0.44 + 0.12j, 1e-4
+    Tb = pre.ctx.build_taylor(wc, 4, _ffi.PREC_BF16X3)
+    T64 = pre.ctx.build_taylor(wc, 3, _ffi.PREC_FP64)
+    for j in range(3):
+        assert relnorm(Tb[j], T64[j]) <= 1e-4
+    Tp = pre.ctx.build_taylor(wc + eps, 3, _ffi.PREC_FP64)[2]
+    Tm = pre.ctx.build_taylor(wc - eps, 3, _ffi.PREC_FP64)[2]
+    T4 = -(Tp - Tm) / (2 * eps) / 3
+    assert relnorm(Tb[3], T4) <= 1e-4
+    with pytest.raises(_ffi.KlspError):
+        pre.ctx.build_taylor(wc, 4, _ffi.PREC_FP64)

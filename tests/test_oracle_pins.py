@@ -334,7 +334,7 @@ def test_grid_scan_argmin_column_major():
 def test_splitmix64_reference_values():
     """First outputs of SplitMix64 from seed 0 and seed 1234567 (the values
     published with Vigna's reference C implementation)."""
-    from tests.prng import SplitMix64
+    from kalnajs_log_spirals_amd.prng import SplitMix64
     g = SplitMix64(0)
     assert [g.next_u64() for _ in range(3)] == [
         0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
