@@ -1138,6 +1138,7 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
     const int nt_max = lp_prec == KLSP_PREC_BF16X3 ? kMaxTerms : kMaxBatch;
     int nt = nt_max;                    // Taylor terms of the next pass
     int nt_last = nt;                   // ... of the last pass built
+    double slope_last = 1.0;            // predicted relative slope error of its model
     for (int pass = 0; pass < np->max_iter; ++pass) {
         wc = w;
         const double om[2] = {w.real(), w.imag()};
@@ -1174,6 +1175,7 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
         if (!(moved >= switch_step)) break;
         const double r = moved / std::max(wn.imag(), 1e-300);
         const double e = std::pow(r, nt) * moved, slope_err = nt * std::pow(r, nt - 1);
+        slope_last = slope_err;
         if (e < switch_step && e * slope_err < 1e-12) break;
         nt = e < 1e-4 ? 2 : nt_max;
     }
@@ -1182,13 +1184,19 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
     //   M(w) = M64(w_a) + [M_lp(w) - M_lp(w_a)]
     //        = M64(w_a) + sum_{j>=1} ((-s)^j - (-s_a)^j) T_{j+1}
     //        = M64(w_a) - (s - s_a) T_2 + (s - s_a)(s + s_a) T_3 - ...,
-    // so the low-precision terms enter only multiplied by |s - s_a| (<= 1e-6
-    // after the switch): their ~1e-5 relative error then moves M by ~1e-11
-    // |s - s_a| |T_2|, below FP64 rounding at the root.  (|det| < tol alone is
-    // no test of the root: where |det'| is small, ~1e-4 at the default grid,
-    // it admits |d omega| ~ 1e-8.)  Newton on the model needs FP64 LUs only;
-    // the anchor is rebuilt if the root moves more than 1e-5 from it.
+    // so the low-precision terms enter only multiplied by |s - s_a|.  (|det| <
+    // tol alone is no test of the root: where |det'| is small, ~1e-4 at the
+    // default grid, it admits |d omega| ~ 1e-8.)  Newton on the model needs
+    // FP64 LUs only; the anchor is rebuilt where the root moves too far.
     const double h64 = 1e-5;            // central-difference step on the model
+    // The anchored model's error moves the root by ~|s - s_a| (slope error of
+    // the last pass's model + the low-precision terms' own relative error,
+    // up to ~1e-4 for BF16x3 and 1e-5 for FP32 at the det level: measured
+    // <= 4e-5 at the default grid, ~1e-4 on a coarse test grid); a move
+    // beyond anchor_tol could leave more than 5e-11 (half the 1e-10 the
+    // roots are held to), so a new anchor is built there.
+    const double eps_lp = lp_prec == KLSP_PREC_BF16X3 ? 1e-4 : 1e-5;
+    const double anchor_tol = 5e-11 / (std::min(slope_last, 1.0) + eps_lp);
     cd D(NAN, NAN);
     int conv = 0;
     for (int anchor = 0; anchor < std::max(1, np->max_iter); ++anchor) {
@@ -1233,7 +1241,7 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
             }
             prev_step = step;
             s = wn - wc;
-            if (std::abs(s - sa) > 1e-5) { far = true; break; }
+            if (std::abs(s - sa) > anchor_tol) { far = true; break; }
         }
         w = wc + s;
         if (!far) break;

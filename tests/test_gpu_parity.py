@@ -323,7 +323,7 @@ def test_mixed_newton_small_grid_known_answer(small_cfg_paths):
     assert abs(complex(2 * r.Omega_p, r.gamma) - ro["omega"]) <= 1e-10
     assert abs(r.Omega_p - 0.4397142267) <= 1e-10
     assert abs(r.gamma - 0.1230663944) <= 1e-10
-    assert r.n_fp64_builds == 1 and r.n_lp_builds <= 4
+    assert r.n_fp64_builds <= 2 and r.n_lp_builds <= 4
     assert r.n_det_evals == r.n_fp64_builds + r.n_lp_builds
     assert len(r.history) == r.iterations
 
