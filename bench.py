@@ -235,7 +235,8 @@ def main() -> int:
         ctx.close()
         draws = draw_batch(48, 0)[:args.uneven]
         t0 = time.perf_counter()
-        rm = run_batch(draws, precision="mixed", log=log)
+        pol = 2 if args.uneven_fp64 else 0
+        rm = run_batch(draws, precision="mixed", log=log, polish=pol)
         wall = time.perf_counter() - t0
         conv = [r for r in rm if r.converged]
         tn = sum(r.newton_s for r in rm)
@@ -248,20 +249,26 @@ def main() -> int:
             f"wall {wall:.1f} s")
         if args.uneven_fp64:
             t0 = time.perf_counter()
-            r64 = run_batch(draws, precision="fp64", log=log)
+            r64 = run_batch(draws, precision="fp64", log=log, polish=pol)
             wall64 = time.perf_counter() - t0
             tn64 = sum(r.newton_s for r in r64)
             both = [(a, b) for a, b in zip(rm, r64) if a.converged and b.converged]
-            dws = [abs(complex(a.draw.m * a.Omega_p, a.gamma) - complex(b.draw.m * b.Omega_p, b.gamma))
-                   for a, b in both]
+            # the same root: the polished FP64 root within 1e-6 of the mixed one
+            same = [(a, b) for a, b in both
+                    if abs(complex(a.draw.m * a.Omega_p, a.gamma) - b.polished) < 1e-6]
+            dws = [abs(complex(a.draw.m * a.Omega_p, a.gamma) - b.polished) for a, b in same]
+            moves = [a.polish_move for a in rm if a.converged]
             extra.update({"uneven_converged_fp64": sum(r.converged for r in r64),
                           "uneven_newton_s_fp64": tn64, "uneven_wall_s_fp64": wall64,
                           "uneven_roots_per_s_fp64":
                               sum(r.converged for r in r64) / tn64 if tn64 > 0 else 0.0,
-                          "uneven_max_abs_domega": max(dws) if dws else None,
-                          "uneven_converged_both": len(both)})
+                          "uneven_converged_both": len(both), "uneven_same_root": len(same),
+                          "uneven_max_abs_domega_same_root": max(dws) if dws else None,
+                          "uneven_max_fp64_polish_move_mixed": max(moves) if moves else None})
             log(f"[bench] uneven batch (FP64): {sum(r.converged for r in r64)} converged, Newton "
-                f"{tn64:.1f} s; max |d omega| mixed vs FP64 {max(dws) if dws else float('nan'):.2e}")
+                f"{tn64:.1f} s; {len(same)} of {len(both)} on the same root, max |d omega| there "
+                f"{max(dws) if dws else float('nan'):.2e}; largest FP64 polish move of a mixed "
+                f"root {max(moves) if moves else float('nan'):.2e}")
 
     cpu = None
     if args.cpu_slabs < 0:

@@ -22,6 +22,7 @@ from typing import List, Optional
 
 import numpy as np
 
+from . import _ffi
 from .BasisFunctions import precompute_device, precompute_host, to_device
 from .Configuration import KalnajsConfig, load_config
 from .GPUBackend import GPUType
@@ -64,6 +65,8 @@ class DrawResult:
     host_s: float        # grids + orbits (worker thread, overlapped)
     W_s: float           # device W quadrature (wall, incl. orbit upload)
     newton_s: float
+    polished: complex = complex("nan")   # root after `polish` FP64 Newton steps, tol = 0
+    polish_move: float = float("nan")    # |polished - omega|
 
 
 def draw_batch(n: int = 48, seed: int = 0) -> List[Draw]:
@@ -105,9 +108,11 @@ def config_for(d: Draw, scale: float = 1.0, base: Optional[KalnajsConfig] = None
 
 
 def run_batch(draws: List[Draw], precision: str = "mixed", scale: float = 1.0,
-              base: Optional[KalnajsConfig] = None, log=None) -> List[DrawResult]:
+              base: Optional[KalnajsConfig] = None, log=None, polish: int = 0) -> List[DrawResult]:
     """Newton search (NewtonSolver.jl:36 semantics, `precision` as in
-    newton_search) from each draw's starting omega."""
+    newton_search) from each draw's starting omega.  With polish > 0 every
+    converged root is then refined by that many FP64 Newton steps with
+    tol = 0 (an a-posteriori check: the move is the root's FP64 error)."""
     cfgs = [config_for(d, scale, base) for d in draws]
 
     def host(i):
@@ -130,17 +135,23 @@ def run_batch(draws: List[Draw], precision: str = "mixed", scale: float = 1.0,
             r = newton_search(cfgs[i], pre, dev, GPUType.AMD, np.float64, d.Omega_p0, d.gamma0,
                               precision=precision)
             t_n = time.perf_counter() - t0
-            pre.ctx.close()
             res = DrawResult(d, precision, bool(r.converged), float(r.Omega_p), float(r.gamma),
                              float(abs(r.det_val)), int(r.iterations), int(r.n_fp64_builds),
                              int(r.n_lp_builds), t_host, t_W, t_n)
+            if polish > 0 and r.converged:
+                pr, _ = pre.ctx.newton(r.Omega_p, r.gamma, max_iter=polish, tol=0.0,
+                                       precision=_ffi.PREC_FP64)
+                res.polished = complex(pr.omega[0], pr.omega[1])
+                res.polish_move = abs(res.polished - complex(d.m * r.Omega_p, r.gamma))
+            pre.ctx.close()
             results.append(res)
             if log:
                 log(f"[uneven] {i:2d} m={d.m} n={d.n_zang} q1={d.q1} NR={cfgs[i].grid.NR} "
                     f"Ne={cfgs[i].grid.Ne} N={cfgs[i].grid.N_alpha} l={cfgs[i].grid.l_max}: "
                     f"{'converged' if res.converged else 'not converged'} "
-                    f"Op={res.Omega_p:.10f} g={res.gamma:.10f} host {t_host:.1f}s "
-                    f"W {t_W:.2f}s newton {t_n:.2f}s")
+                    f"Op={res.Omega_p:.13f} g={res.gamma:.13f} host {t_host:.1f}s "
+                    f"W {t_W:.2f}s newton {t_n:.2f}s"
+                    + (f" polish move {res.polish_move:.2e}" if polish > 0 else ""))
     return results
 
 

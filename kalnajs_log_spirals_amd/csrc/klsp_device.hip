@@ -316,7 +316,8 @@ __global__ void k_reduce(const double2* __restrict__ P, int S, int Np, int N, in
 // Pivot = first row maximising |re|+|im| (izamax), multipliers scaled by the
 // reciprocal pivot; det = prod U_jj * (-1)^swaps in LinearAlgebra's order.
 // Blocked right-looking form, as zgetrf: a panel of up to kLuMaxNb columns is
-// loaded into LDS and factored there column by column; then every wave takes
+// loaded into LDS and factored there column by column by one wave (no
+// workgroup barrier per column); then every wave takes
 // kLuCols trailing columns at a time, applies the panel's row permutation (one
 // gather), solves U12 = L11^-1 A12 with lane shuffles and subtracts
 // L21 U12 (L21 read from LDS), so the trailing matrix in global memory
@@ -353,9 +354,6 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_det(const double2* __restrict
     __shared__ double2 P[kLuPanel];   // panel, column-major, leading dimension m
     __shared__ int perm[kMaxN];
     __shared__ int pv[kLuMaxNb];
-    __shared__ double redv[kLuThreads / 64];
-    __shared__ int redi[kLuThreads / 64];
-    __shared__ int s_piv;
 
     for (int64_t x = tid; x < NN; x += nth) {
         double2 m;
@@ -384,55 +382,52 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_det(const double2* __restrict
             P[x] = A[(k0 + r) + (int64_t)N * (k0 + t)];
         }
         __syncthreads();
-        // unblocked factorisation of the panel
-        for (int jj = 0; jj < kb; ++jj) {
-            double2* Pj = P + jj * m;
-            double bv = -1.0;
-            int bi = m;
-            for (int r = jj + tid; r < m; r += nth) {
-                const double2 a = Pj[r];
-                const double v = fabs(a.x) + fabs(a.y);
-                if (v > bv) { bv = v; bi = r; }
-            }
-            for (int off = 32; off >= 1; off >>= 1) {
-                const double ov = __shfl_xor(bv, off);
-                const int oi = __shfl_xor(bi, off);
-                argmax_merge(bv, bi, ov, oi);
-            }
-            if (lane == 0) { redv[wid] = bv; redi[wid] = bi; }
-            __syncthreads();
-            if (tid == 0) {
-                double v = redv[0];
-                int idx = redi[0];
-                for (int w = 1; w < nwv; ++w) argmax_merge(v, idx, redv[w], redi[w]);
-                s_piv = (v > 0.0) ? idx : -1;
-                pv[jj] = (v > 0.0) ? idx : jj;
-            }
-            __syncthreads();
-            const int p = s_piv;
-            if (p < 0) { singular = true; continue; }
-            if (p != jj) {
-                ++swaps;
-                for (int t = tid; t < kb; t += nth) {
-                    const double2 tmp = P[t * m + jj];
-                    P[t * m + jj] = P[t * m + p];
-                    P[t * m + p] = tmp;
+        // unblocked factorisation of the panel by wave 0 alone: no workgroup
+        // barrier per column (a wave's LDS accesses execute in program order)
+        if (wid == 0) {
+            for (int jj = 0; jj < kb; ++jj) {
+                double2* Pj = P + jj * m;
+                double bv = -1.0;
+                int bi = m;
+                for (int r = jj + lane; r < m; r += 64) {
+                    const double2 a = Pj[r];
+                    const double v = fabs(a.x) + fabs(a.y);
+                    if (v > bv) { bv = v; bi = r; }
+                }
+                for (int off = 32; off >= 1; off >>= 1) {
+                    const double ov = __shfl_xor(bv, off);
+                    const int oi = __shfl_xor(bi, off);
+                    argmax_merge(bv, bi, ov, oi);
+                }
+                if (!(bv > 0.0)) {
+                    singular = true;
+                    if (lane == 0) pv[jj] = jj;
+                    continue;
+                }
+                const int p = bi;
+                if (lane == 0) pv[jj] = p;
+                if (p != jj) {
+                    ++swaps;
+                    if (lane < kb) {
+                        const double2 tmp = P[lane * m + jj];
+                        P[lane * m + jj] = P[lane * m + p];
+                        P[lane * m + p] = tmp;
+                    }
+                }
+                const double2 piv = Pj[jj];
+                det = cmul(det, piv);
+                const double2 rp = d_inv(piv.x, piv.y);
+                for (int r = jj + 1 + lane; r < m; r += 64) {
+                    const double2 l = cmul(Pj[r], rp);
+                    Pj[r] = l;
+                    for (int t = jj + 1; t < kb; ++t) {
+                        const double2 lu = cmul(l, P[t * m + jj]);
+                        P[t * m + r] = make_double2(P[t * m + r].x - lu.x, P[t * m + r].y - lu.y);
+                    }
                 }
             }
-            __syncthreads();
-            const double2 piv = Pj[jj];
-            det = cmul(det, piv);
-            const double2 rp = d_inv(piv.x, piv.y);
-            for (int r = jj + 1 + tid; r < m; r += nth) Pj[r] = cmul(Pj[r], rp);
-            __syncthreads();
-            const int nr = m - jj - 1, nc = kb - jj - 1;
-            for (int x = tid; x < nr * nc; x += nth) {
-                const int t = jj + 1 + x / nr, r = jj + 1 + x % nr;
-                const double2 lu = cmul(Pj[r], P[t * m + jj]);
-                P[t * m + r] = make_double2(P[t * m + r].x - lu.x, P[t * m + r].y - lu.y);
-            }
-            __syncthreads();
         }
+        __syncthreads();
         // the panel's row interchanges as one permutation of rows k0..N-1
         for (int r = tid; r < m; r += nth) perm[r] = r;
         __syncthreads();
@@ -448,13 +443,19 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_det(const double2* __restrict
         for (int cg = wid * kLuCols; cg < ncol; cg += nwv * kLuCols) {
             const int nq = min(kLuCols, ncol - cg);
             double2 u[kLuCols][kLuMaxNb];
-            for (int r0 = 0; r0 < m; r0 += 64) {
-                const int r = r0 + lane;
-                double2 x[kLuCols];
+            // chunks of 64 rows; the next chunk's gathers are issued before
+            // the current one is computed, so their L2 latency overlaps it
+            auto gather = [&](int r, double2 (&v)[kLuCols]) {
 #pragma unroll
                 for (int q = 0; q < kLuCols; ++q)
-                    x[q] = (q < nq && r < m) ? A[(k0 + perm[r]) + (int64_t)N * (c0 + cg + q)]
+                    v[q] = (q < nq && r < m) ? A[(k0 + perm[r]) + (int64_t)N * (c0 + cg + q)]
                                              : make_double2(0.0, 0.0);
+            };
+            double2 x[kLuCols], xn[kLuCols];
+            gather(lane, x);
+            for (int r0 = 0; r0 < m; r0 += 64) {
+                const int r = r0 + lane;
+                if (r0 + 64 < m) gather(r + 64, xn);
                 if (r0 == 0) {
                     // U12 = L11^-1 A12 (unit lower L11): lane t holds row t
 #pragma unroll
@@ -487,6 +488,8 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_det(const double2* __restrict
                     for (int q = 0; q < kLuCols; ++q)
                         if (q < nq) A[(k0 + r) + (int64_t)N * (c0 + cg + q)] = x[q];
                 }
+#pragma unroll
+                for (int q = 0; q < kLuCols; ++q) x[q] = xn[q];
             }
         }
         __syncthreads();
@@ -578,7 +581,9 @@ struct GangPlan {
 
 // Gang geometry for a contraction shape: wave tiles of MB x NB blocks cover
 // the Np/16 x Np/16 block grid; as many gangs per XCD as its resident wave
-// slots hold; K per gang a multiple of kF32Tile (so also of the 8-row pair).
+// slots hold (four rounds of gangs, letting freed slots take up the next
+// gang's waves, measured 7% slower: the gangs lose L2 locality); K per gang a
+// multiple of kF32Tile (so also of the 8-row pair).
 static GangPlan plan_gangs(const klsp_ctx* ctx, const ContractShape& sh)
 {
     GangPlan gp;

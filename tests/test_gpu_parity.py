@@ -144,6 +144,35 @@ def test_random_problem_ragged_sizes(NPh, N, n_l):
         assert abs(d - do) <= 1e-11 * max(abs(do), 1e-300)
 
 
+@pytest.mark.parametrize("N", [100, 300, 520, 1000])
+def test_lu_dense_pivoting(N):
+    """Full-rank M (K = 2N) scaled so that I - M is a general dense complex
+    matrix: the blocked LU pivots in most columns; det vs LAPACK on the same
+    device-built M and vs the BLAS-form build."""
+    from oracle import blas_form
+    rng = np.random.default_rng(N)
+    NPh, n_l = N, 2
+    W = 2.0 * (rng.standard_normal((NPh, N, n_l)) + 1j * rng.standard_normal((NPh, N, n_l)))
+    W /= np.sqrt(NPh * n_l)
+    DJ, F0l = rng.uniform(0.1, 1, NPh), rng.standard_normal((NPh, n_l))
+    Ores, Nk = rng.uniform(-2, 2, (NPh, n_l)), rng.uniform(0.5, 1.5, N)
+    pre = from_arrays(2, 0.7, W, DJ, F0l, Ores, Nk)
+    om = 0.3 + 0.4j
+    M = pre.ctx.build_M(om, _ffi.PREC_FP64)
+    Mb = blas_form.build_M(W, DJ, F0l, Ores, Nk, 0.7, om)
+    assert relnorm(M, Mb) <= 1e-12
+    A = np.eye(N) - M
+    assert np.linalg.cond(A) < 1e8
+    d = pre.ctx.det([om], _ffi.PREC_FP64)[0]
+    ref = np.linalg.det(A)
+    # det is a product of N pivots: relative agreement ~ N eps cond
+    assert abs(d - ref) <= 1e-9 * abs(ref)
+    # the pivoting is real: LAPACK interchanges rows in a third of the columns or more
+    import scipy.linalg
+    piv = scipy.linalg.lu_factor(A)[1]
+    assert np.count_nonzero(piv != np.arange(N)) >= N // 3
+
+
 def test_resonance_clamp():
     """|omega - Omega_res| < 1e-12 -> denominator 1e-12 + 1e-12 i (MatrixBuilder.jl:47)."""
     rng = np.random.default_rng(7)
