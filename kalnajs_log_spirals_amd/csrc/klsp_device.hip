@@ -334,6 +334,13 @@ __device__ __forceinline__ void argmax_merge(double& v, int& idx, double ov, int
     if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
 }
 
+// x - a b for complex values with four fused multiply-adds (the LU's
+// multiply-subtract; zgetrf's BLAS kernels fuse the same way)
+__device__ __forceinline__ double2 cfms(double2 x, double2 a, double2 b)
+{
+    return make_double2(fma(-a.x, b.x, fma(a.y, b.y, x.x)), fma(-a.x, b.y, fma(-a.y, b.x, x.y)));
+}
+
 __device__ __forceinline__ double2 shfl2(double2 v, int src)
 {
     return make_double2(__shfl(v.x, src), __shfl(v.y, src));
@@ -420,10 +427,7 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_det(const double2* __restrict
                 for (int r = jj + 1 + lane; r < m; r += 64) {
                     const double2 l = cmul(Pj[r], rp);
                     Pj[r] = l;
-                    for (int t = jj + 1; t < kb; ++t) {
-                        const double2 lu = cmul(l, P[t * m + jj]);
-                        P[t * m + r] = make_double2(P[t * m + r].x - lu.x, P[t * m + r].y - lu.y);
-                    }
+                    for (int t = jj + 1; t < kb; ++t) P[t * m + r] = cfms(P[t * m + r], l, P[t * m + jj]);
                 }
             }
         }
@@ -466,10 +470,7 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_det(const double2* __restrict
                         if (lane > t && lane < kb) {
                             const double2 l = P[t * m + lane];
 #pragma unroll
-                            for (int q = 0; q < kLuCols; ++q) {
-                                const double2 lu = cmul(l, u[q][t]);
-                                x[q] = make_double2(x[q].x - lu.x, x[q].y - lu.y);
-                            }
+                            for (int q = 0; q < kLuCols; ++q) x[q] = cfms(x[q], l, u[q][t]);
                         }
                     }
                 }
@@ -479,10 +480,7 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_det(const double2* __restrict
                         if (t >= kb) break;
                         const double2 l = P[t * m + r];
 #pragma unroll
-                        for (int q = 0; q < kLuCols; ++q) {
-                            const double2 lu = cmul(l, u[q][t]);
-                            x[q] = make_double2(x[q].x - lu.x, x[q].y - lu.y);
-                        }
+                        for (int q = 0; q < kLuCols; ++q) x[q] = cfms(x[q], l, u[q][t]);
                     }
 #pragma unroll
                     for (int q = 0; q < kLuCols; ++q)
