@@ -58,6 +58,8 @@ def main() -> int:
                          "searches (mixed precision; minutes at full size)")
     ap.add_argument("--uneven-fp64", action="store_true",
                     help="run the uneven draws in FP64 too and report the root differences")
+    ap.add_argument("--uneven-orbits", choices=("host", "device"), default="device",
+                    help="where the uneven draws' orbits are integrated")
     ap.add_argument("--no-fp32", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -91,8 +93,29 @@ def main() -> int:
     t_pre = time.perf_counter() - t0
     W_ms = pre.ctx.last_W_ms()
     log(f"[bench] precompute_all: {t_pre:.2f} s (W quadrature on device {W_ms:.1f} ms)")
+    # the same precompute with the orbits integrated on the device
+    t0 = time.perf_counter()
+    pre_d = K.precompute_device_orbits(cfg, model)
+    t_pre_d = time.perf_counter() - t0
+    orbit_ms, W_ms_d = pre_d.ctx.last_orbit_ms(), pre_d.ctx.last_W_ms()
+    od_h = pre.orbit_data
+    d_om = max(float(np.max(np.abs(pre_d.orbit_data.Omega_1 - od_h.Omega_1)
+                            / np.abs(od_h.Omega_1))),
+               float(np.max(np.abs(pre_d.orbit_data.Omega_2 - od_h.Omega_2)
+                            / np.maximum(np.abs(od_h.Omega_2), 1e-300))))
+    d_samples = max(float(np.max(np.abs(getattr(pre_d.orbit_data, k) - getattr(od_h, k))))
+                    for k in ("w1", "ra", "pha"))
+    log(f"[bench] device orbits: precompute {t_pre_d:.2f} s (orbit kernel {orbit_ms:.1f} ms, "
+        f"W quadrature {W_ms_d:.1f} ms); max rel |d Omega| vs host {d_om:.2e}, max |d sample| "
+        f"{d_samples:.2e}")
     dev = K.to_device(pre, K.GPUType.AMD, np.float64)
     ctx = dev.ctx
+    det_h, det_d = ctx.det([complex(2 * 0.4397, 0.1231)])[0], pre_d.ctx.det(
+        [complex(2 * 0.4397, 0.1231)])[0]
+    d_det = abs(det_d - det_h) / abs(det_h)
+    log(f"[bench] det(I - M) with device vs host orbits: relative difference {d_det:.2e}")
+    pre_d.ctx.close()
+    del pre_d
     ctx.set_profiling(True)
 
     om0 = complex(2 * 0.4397, 0.1231)
@@ -126,6 +149,9 @@ def main() -> int:
              "lu_ms": lu_ms / args.steps, "weights_ms": weights_ms / args.steps,
              "reduce_ms": reduce_ms / args.steps, "W_quadrature_ms": W_ms,
              "host_orbits_s": t_orb, "precompute_s": t_pre,
+             "device_orbits_ms": orbit_ms, "precompute_device_orbits_s": t_pre_d,
+             "device_orbits_max_rel_domega": d_om, "device_orbits_max_dsample": d_samples,
+             "device_orbits_det_rel_diff": d_det,
              "W_quadrature_tflops": 4.0 * n_l * g.nwa * N * NPh / (W_ms * 1e-3) / 1e12}
 
     if not args.no_fp32:
@@ -236,7 +262,7 @@ def main() -> int:
         draws = draw_batch(48, 0)[:args.uneven]
         t0 = time.perf_counter()
         pol = 2 if args.uneven_fp64 else 0
-        rm = run_batch(draws, precision="mixed", log=log, polish=pol)
+        rm = run_batch(draws, precision="mixed", log=log, polish=pol, orbits=args.uneven_orbits)
         wall = time.perf_counter() - t0
         conv = [r for r in rm if r.converged]
         tn = sum(r.newton_s for r in rm)
@@ -244,12 +270,14 @@ def main() -> int:
                       "uneven_wall_s_mixed": wall, "uneven_newton_s_mixed": tn,
                       "uneven_roots_per_s_mixed": len(conv) / tn if tn > 0 else 0.0,
                       "uneven_W_s": sum(r.W_s for r in rm),
-                      "uneven_host_s": sum(r.host_s for r in rm)})
+                      "uneven_host_s": sum(r.host_s for r in rm),
+                      "uneven_orbits": args.uneven_orbits})
         log(f"[bench] uneven batch (mixed): {len(conv)}/{len(draws)} converged, Newton {tn:.1f} s, "
             f"wall {wall:.1f} s")
         if args.uneven_fp64:
             t0 = time.perf_counter()
-            r64 = run_batch(draws, precision="fp64", log=log, polish=pol)
+            r64 = run_batch(draws, precision="fp64", log=log, polish=pol,
+                            orbits=args.uneven_orbits)
             wall64 = time.perf_counter() - t0
             tn64 = sum(r.newton_s for r in r64)
             both = [(a, b) for a, b in zip(rm, r64) if a.converged and b.converged]

@@ -131,6 +131,18 @@ int klsp_compute_orbits(const klsp_params* p, const double* Rc, const double* e,
                         double* pha, double* jacobian, double* F0, double* FE,
                         double* FL, int n_threads);
 
+/* The Jacobian |d(E,L)/d(Rc,v)| / Omega_1 and the DF on the grid, the tail of
+ * compute_orbits (compute_jacobian!, OrbitIntegration.jl:341-382; DF fill
+ * :317-333), for orbits integrated on the device (klsp_compute_orbits_W). */
+int klsp_compute_jacobian_df(const klsp_params* p, const double* Rc, const double* R1,
+                             const double* R2, const double* E, const double* L,
+                             const double* Omega_1, double* jacobian, double* F0, double* FE,
+                             double* FL);
+
+/* The radial-phase samples of compute_orbits (OrbitIntegration.jl:190-193):
+ * w[nw] = range(1e-8, pi - 1e-8, length = nw), cw = cos(w), sw = sin(w). */
+int klsp_orbit_phase_grid(const klsp_params* p, double* w, double* cw, double* sw);
+
 /* omega-independent arrays of precompute_all, BasisFunctions.jl:127-153:
  * DJ[NPh], F0l[NPh, n_l], Omega_res[NPh, n_l]. */
 int klsp_precompute_arrays(const klsp_params* p, const double* S_RC, const double* S_e,
@@ -151,13 +163,33 @@ typedef struct klsp_problem {
     int64_t n_l;            /* number of radial harmonics                   */
     int m;
     double G;
-    const double* DJ;       /* [NPh]                                        */
-    const double* F0l;      /* [NPh, n_l]                                   */
-    const double* Omega_res;/* [NPh, n_l]                                   */
+    const double* DJ;       /* [NPh]  (DJ, F0l, Omega_res may all be NULL:   */
+    const double* F0l;      /* [NPh, n_l]   set them later with             */
+    const double* Omega_res;/* [NPh, n_l]   klsp_set_problem_arrays)        */
     const double* N_kernel; /* [N]                                          */
 } klsp_problem;
 
 int klsp_upload_problem(klsp_ctx* ctx, const klsp_problem* prob);
+
+/* DJ [NPh], F0l and Omega_res [NPh, n_l] of an uploaded problem (dimensions
+ * unchanged; W is kept).  Needed before any build when the problem was
+ * uploaded without them. */
+int klsp_set_problem_arrays(klsp_ctx* ctx, const double* DJ, const double* F0l,
+                            const double* Omega_res);
+
+/* compute_orbits (OrbitIntegration.jl:181-302) followed by compute_W_l
+ * (BasisFunctions.jl:199), both on the device: one workgroup per orbit
+ * integrates its nw phase samples (the cumulative trapezoid sums as block
+ * scans, so the sums are grouped differently from the sequential host loop
+ * at the 1e-16 level); the nwa stored samples feed the W quadrature without
+ * leaving the device.  Grid inputs as from klsp_compute_grids; the problem
+ * (its dimensions at least) must be uploaded.  Omega_1, Omega_2 [NPh] are
+ * returned; w1, ra, pha [nwa, NPh] are copied back if non-NULL. */
+int klsp_compute_orbits_W(klsp_ctx* ctx, const klsp_params* p, const double* Rc,
+                          const double* e, const double* R1, const double* R2, const double* E,
+                          const double* L, const double* L2, const double* SGNL,
+                          const double* alpha, const int32_t* l_arr, double* Omega_1,
+                          double* Omega_2, double* w1, double* ra, double* pha);
 
 /* compute_W_l, BasisFunctions.jl:199 (W_l(alpha) orbit quadrature, Eq. 41),
  * evaluated on the device as one GEMM per orbit and kept device-resident in
@@ -289,8 +321,13 @@ typedef struct klsp_timing {
 int klsp_set_profiling(klsp_ctx* ctx, int on);
 int klsp_last_timing(klsp_ctx* ctx, klsp_timing* t_out);
 
-/* Device time of the last klsp_compute_W (orbit preprocessing + GEMM). */
+/* Device time of the last W quadrature (klsp_compute_W, klsp_compute_orbits_W):
+ * orbit preprocessing + GEMM. */
 int klsp_last_W_ms(klsp_ctx* ctx, double* ms_out);
+
+/* Device time of the orbit integration kernel of the last
+ * klsp_compute_orbits_W (the W quadrature after it is in klsp_last_W_ms). */
+int klsp_last_orbit_ms(klsp_ctx* ctx, double* ms_out);
 
 #ifdef __cplusplus
 }

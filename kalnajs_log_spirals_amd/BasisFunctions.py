@@ -15,10 +15,11 @@ import numpy as np
 from . import _ffi
 from . import GPUBackend
 from .Configuration import KalnajsConfig
-from .OrbitIntegration import OrbitData, compute_grids, compute_orbits, params_from_config
+from .OrbitIntegration import (OrbitData, compute_grids, compute_orbits, jacobian_df,
+                               params_from_config)
 
-__all__ = ["PrecomputedData", "precompute_all", "trapezoid_coef", "compute_W_l",
-           "compute_N_kernel", "gamma_complex", "DevicePrecomputed", "to_device"]
+__all__ = ["PrecomputedData", "precompute_all", "precompute_device_orbits", "trapezoid_coef",
+           "compute_W_l", "compute_N_kernel", "gamma_complex", "DevicePrecomputed", "to_device"]
 
 
 def _require_device(backend, who: str):
@@ -159,11 +160,61 @@ def precompute_device(config: KalnajsConfig, host: dict, verbose: bool = False) 
                            host["Nk"], ctx=ctx)
 
 
+def precompute_device_orbits(config: KalnajsConfig, model, verbose: bool = False,
+                             download_orbits: bool = True) -> PrecomputedData:
+    """precompute_all with the orbits integrated on the device
+    (OrbitIntegration.jl:181-302 as one workgroup per orbit) and fed to the W
+    quadrature without leaving it.  Grids, N_m(alpha), the Jacobian / DF and
+    DJ / F0l / Omega_res stay on the host (O(NPh) work).  The orbit samples
+    are copied back into orbit_data unless download_orbits is False (then
+    w1 / ra / pha are None)."""
+    m, G = int(config.physics.m), float(config.physics.G)
+    N = int(config.grid.N_alpha)
+    l_arr = np.arange(config.grid.l_min, config.grid.l_max + 1, dtype=np.int32)
+    p = params_from_config(config)
+    NPh = p.NR * p.Ne
+    if verbose:
+        print("Computing grids...")
+    grids = compute_grids(config, model)
+    alpha = _linrange(-config.grid.alpha_max, config.grid.alpha_max, N)
+    d_alpha = trapezoid_coef(alpha)
+    Nk = compute_N_kernel(m, alpha, d_alpha)
+    if verbose:
+        print("Computing orbits and W_l basis functions (on device)...")
+    ctx = _ffi.Context(GPUBackend.current_device())
+    try:
+        ctx.upload_problem(NPh, N, l_arr.size, m, G, None, None, None, Nk)
+        g = dict(Rc=grids.Rc, e=grids.e, R1=grids.R1, R2=grids.R2, E=grids.E, L=grids.L_m,
+                 L2=grids.L2_m, SGNL=grids.SGNL)
+        o1, o2, w1, ra, pha = ctx.compute_orbits_W(p, g, alpha, l_arr, download_orbits)
+        shp = (p.NR, p.Ne)
+        Om1, Om2 = o1.reshape(shp, order="F"), o2.reshape(shp, order="F")
+        jac, F0, FE, FL = jacobian_df(config, grids, Om1)
+        shp3 = (p.nwa, p.NR, p.Ne)
+        orb = [x.reshape(shp3, order="F") if x is not None else None for x in (w1, ra, pha)]
+        od = OrbitData(grids, Om1, Om2, *orb, jac, F0, FE, FL)
+        DJ, F0l, Ores = _omega_independent(config, od)
+        ctx.set_problem_arrays(DJ, F0l, Ores)
+    except BaseException:
+        ctx.close()
+        raise
+    if verbose:
+        print("Precomputation complete.")
+    return PrecomputedData(m, G, N, l_arr.size, l_arr.astype(np.int64), NPh, alpha, d_alpha,
+                           od, DJ, F0l, Ores, Nk, ctx=ctx)
+
+
 def precompute_all(config: KalnajsConfig, model, backend=GPUBackend.GPUType.AMD,
-                   verbose: bool = False) -> PrecomputedData:
+                   verbose: bool = False, orbits: str = "host") -> PrecomputedData:
     """BasisFunctions.jl:73-165: grids, orbits, alpha grid, W_l (device),
-    N_m(alpha), DJ / F0l / Omega_res."""
+    N_m(alpha), DJ / F0l / Omega_res.  orbits = "host" integrates the orbits
+    with the host library (threads), "device" on the GPU
+    (precompute_device_orbits)."""
     _require_device(backend, "precompute_all")
+    if orbits == "device":
+        return precompute_device_orbits(config, model, verbose)
+    if orbits != "host":
+        raise ValueError(f"precompute_all: orbits must be 'host' or 'device', not {orbits!r}")
     return precompute_device(config, precompute_host(config, model, verbose), verbose)
 
 

@@ -14,7 +14,7 @@ from . import _ffi
 from .Configuration import KalnajsConfig
 
 __all__ = ["GridData", "OrbitData", "compute_grids", "compute_orbits", "trapezoid_coef",
-           "simpson_coef", "params_from_config"]
+           "simpson_coef", "params_from_config", "jacobian_df", "orbit_phase_grid"]
 
 
 @dataclass
@@ -116,3 +116,31 @@ def compute_orbits(config: KalnajsConfig, model, grids: GridData, verbose: bool 
                      w1.reshape(shp3, order="F"), ra.reshape(shp3, order="F"),
                      pha.reshape(shp3, order="F"), _grid(o["jac"], NR, Ne),
                      _grid(o["F0"], NR, Ne), _grid(o["FE"], NR, Ne), _grid(o["FL"], NR, Ne))
+
+
+def orbit_phase_grid(config: KalnajsConfig):
+    """The radial-phase samples w, cos w, sin w (OrbitIntegration.jl:190-193)."""
+    import ctypes as ct
+    p = params_from_config(config)
+    w, cw, sw = np.empty(p.nw), np.empty(p.nw), np.empty(p.nw)
+    _ffi.check(_ffi.lib().klsp_orbit_phase_grid(ct.byref(p), _ffi.dptr(w), _ffi.dptr(cw),
+                                                _ffi.dptr(sw)), what="orbit_phase_grid")
+    return w, cw, sw
+
+
+def jacobian_df(config: KalnajsConfig, grids: GridData, Omega_1):
+    """compute_jacobian! (OrbitIntegration.jl:341-382) and the DF on the grid
+    (:317-333) for given radial frequencies: (jacobian, F0, FE, FL), each
+    (NR, Ne)."""
+    import ctypes as ct
+    p = params_from_config(config)
+    NR, Ne = p.NR, p.Ne
+    flat = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1, order="F"))  # noqa: E731
+    ins = [flat(getattr(grids, k)) for k in ("Rc", "R1", "R2", "E", "L_m")] + [flat(Omega_1)]
+    if any(x.size != NR * Ne for x in ins):
+        raise ValueError("jacobian_df: arrays must have NR * Ne entries")
+    out = [np.empty(NR * Ne) for _ in range(4)]
+    _ffi.check(_ffi.lib().klsp_compute_jacobian_df(ct.byref(p), *[_ffi.dptr(x) for x in ins],
+                                                   *[_ffi.dptr(x) for x in out]),
+               what="jacobian_df: invalid parameters")
+    return tuple(_grid(x, NR, Ne) for x in out)

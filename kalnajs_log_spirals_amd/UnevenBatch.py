@@ -9,8 +9,10 @@ not set take the values of configs/default.toml.
 
 The batch runs one draw at a time on the device, each with its own context
 (freed before the next one is built, so the largest draw alone bounds device
-memory), while a worker thread computes the next draw's host half (grids and
-orbits); the host and device stages of consecutive draws overlap.
+memory).  With orbits = "host" a worker thread computes the next draw's host
+half (grids and orbits) while the device works on the current one; with
+orbits = "device" the orbits are integrated on the device too and only the
+O(NR * Ne) grid and distribution-function work stays on the host.
 """
 from __future__ import annotations
 
@@ -23,7 +25,7 @@ from typing import List, Optional
 import numpy as np
 
 from . import _ffi
-from .BasisFunctions import precompute_device, precompute_host, to_device
+from .BasisFunctions import precompute_device, precompute_device_orbits, precompute_host, to_device
 from .Configuration import KalnajsConfig, load_config
 from .GPUBackend import GPUType
 from .Models import create_toomre_zang_model
@@ -62,8 +64,9 @@ class DrawResult:
     iterations: int
     n_fp64_builds: int
     n_lp_builds: int
-    host_s: float        # grids + orbits (worker thread, overlapped)
-    W_s: float           # device W quadrature (wall, incl. orbit upload)
+    host_s: float        # grids + orbits (worker thread, overlapped); 0 with device orbits
+    W_s: float           # device W quadrature (wall, incl. orbit upload); with device
+                         # orbits: the whole precompute (grids, orbits, W, DF arrays)
     newton_s: float
     polished: complex = complex("nan")   # root after `polish` FP64 Newton steps, tol = 0
     polish_move: float = float("nan")    # |polished - omega|
@@ -108,14 +111,20 @@ def config_for(d: Draw, scale: float = 1.0, base: Optional[KalnajsConfig] = None
 
 
 def run_batch(draws: List[Draw], precision: str = "mixed", scale: float = 1.0,
-              base: Optional[KalnajsConfig] = None, log=None, polish: int = 0) -> List[DrawResult]:
+              base: Optional[KalnajsConfig] = None, log=None, polish: int = 0,
+              orbits: str = "host") -> List[DrawResult]:
     """Newton search (NewtonSolver.jl:36 semantics, `precision` as in
     newton_search) from each draw's starting omega.  With polish > 0 every
     converged root is then refined by that many FP64 Newton steps with
-    tol = 0 (an a-posteriori check: the move is the root's FP64 error)."""
+    tol = 0 (an a-posteriori check: the move is the root's FP64 error).
+    orbits = "host" | "device" chooses where the orbits are integrated."""
+    if orbits not in ("host", "device"):
+        raise ValueError(f"run_batch: orbits must be 'host' or 'device', not {orbits!r}")
     cfgs = [config_for(d, scale, base) for d in draws]
 
     def host(i):
+        if orbits == "device":
+            return None, 0.0
         t0 = time.perf_counter()
         h = precompute_host(cfgs[i], create_toomre_zang_model(cfgs[i]))
         return h, time.perf_counter() - t0
@@ -127,7 +136,11 @@ def run_batch(draws: List[Draw], precision: str = "mixed", scale: float = 1.0,
             h, t_host = nxt.result()
             nxt = pool.submit(host, i + 1) if i + 1 < len(draws) else None
             t0 = time.perf_counter()
-            pre = precompute_device(cfgs[i], h)
+            if orbits == "device":
+                pre = precompute_device_orbits(cfgs[i], create_toomre_zang_model(cfgs[i]),
+                                               download_orbits=False)
+            else:
+                pre = precompute_device(cfgs[i], h)
             t_W = time.perf_counter() - t0
             del h
             t0 = time.perf_counter()

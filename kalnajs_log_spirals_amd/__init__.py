@@ -14,7 +14,8 @@ from .GPUBackend import GPUType, detect_gpu_type  # noqa: F401
 from .Models import ToomreZangModel, create_toomre_zang_model  # noqa: F401
 from .OrbitIntegration import GridData, OrbitData, compute_grids, compute_orbits  # noqa: F401
 from .BasisFunctions import (PrecomputedData, DevicePrecomputed, precompute_all,  # noqa: F401
-                             compute_W_l, compute_N_kernel, gamma_complex, to_device)
+                             precompute_device_orbits, compute_W_l, compute_N_kernel,
+                             gamma_complex, to_device)
 from .MatrixBuilder import (compute_determinant, compute_determinant_batched,  # noqa: F401
                             build_M_matrix)
 from .NewtonSolver import NewtonResult, newton_search  # noqa: F401

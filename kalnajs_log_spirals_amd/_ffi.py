@@ -40,10 +40,11 @@ EXPORTED = [
     "klsp_version", "klsp_device_count", "klsp_create", "klsp_destroy", "klsp_last_error",
     "klsp_trapezoid_coef", "klsp_simpson_coef", "klsp_linrange", "klsp_gamma_complex",
     "klsp_compute_N_kernel", "klsp_compute_grids", "klsp_compute_orbits",
-    "klsp_precompute_arrays", "klsp_upload_problem", "klsp_compute_W", "klsp_upload_W",
-    "klsp_download_W", "klsp_prepare", "klsp_build_M", "klsp_build_taylor", "klsp_det",
-    "klsp_newton",
-    "klsp_grid_scan", "klsp_set_profiling", "klsp_last_timing", "klsp_last_W_ms",
+    "klsp_compute_jacobian_df", "klsp_orbit_phase_grid", "klsp_precompute_arrays",
+    "klsp_upload_problem", "klsp_set_problem_arrays", "klsp_compute_orbits_W",
+    "klsp_compute_W", "klsp_upload_W", "klsp_download_W", "klsp_prepare", "klsp_build_M",
+    "klsp_build_taylor", "klsp_det", "klsp_newton", "klsp_grid_scan", "klsp_set_profiling",
+    "klsp_last_timing", "klsp_last_W_ms", "klsp_last_orbit_ms",
 ]
 
 
@@ -127,8 +128,13 @@ def lib() -> ct.CDLL:
     L.klsp_compute_N_kernel.argtypes = [ct.c_int, _D, _D, i64, _D]
     L.klsp_compute_grids.argtypes = [ct.POINTER(klsp_params)] + [_D] * 11
     L.klsp_compute_orbits.argtypes = [ct.POINTER(klsp_params)] + [_D] * 17 + [ct.c_int]
+    L.klsp_compute_jacobian_df.argtypes = [ct.POINTER(klsp_params)] + [_D] * 10
+    L.klsp_orbit_phase_grid.argtypes = [ct.POINTER(klsp_params)] + [_D] * 3
     L.klsp_precompute_arrays.argtypes = [ct.POINTER(klsp_params)] + [_D] * 10
     L.klsp_upload_problem.argtypes = [P, ct.POINTER(klsp_problem)]
+    L.klsp_set_problem_arrays.argtypes = [P, _D, _D, _D]
+    L.klsp_compute_orbits_W.argtypes = ([P, ct.POINTER(klsp_params)] + [_D] * 9
+                                        + [ct.POINTER(ct.c_int32)] + [_D] * 5)
     L.klsp_compute_W.argtypes = [P, i64, _D, _D, _D, _D, _D, _D, ct.POINTER(ct.c_int32), _D]
     L.klsp_upload_W.argtypes = [P, _D]
     L.klsp_download_W.argtypes = [P, _D]
@@ -142,6 +148,7 @@ def lib() -> ct.CDLL:
     L.klsp_set_profiling.argtypes = [P, ct.c_int]
     L.klsp_last_timing.argtypes = [P, ct.POINTER(klsp_timing)]
     L.klsp_last_W_ms.argtypes = [P, _D]
+    L.klsp_last_orbit_ms.argtypes = [P, _D]
     for name in EXPORTED:
         if name not in ("klsp_version", "klsp_last_error"):
             getattr(L, name).restype = ct.c_int
@@ -205,18 +212,60 @@ class Context:
         check(st, self.handle, what)
 
     # -- residency -----------------------------------------------------------
+    @staticmethod
+    def _problem_arrays(NPh, n_l, DJ, F0l, Omega_res):
+        DJ = f64c(DJ)
+        F0l = f64c(np.asarray(F0l, dtype=np.float64).reshape(-1, order="F"))
+        Ores = f64c(np.asarray(Omega_res, dtype=np.float64).reshape(-1, order="F"))
+        if DJ.size != NPh or F0l.size != NPh * n_l or Ores.size != NPh * n_l:
+            raise ValueError("problem arrays: sizes do not match (NPh, n_l)")
+        return DJ, F0l, Ores
+
     def upload_problem(self, NPh: int, N: int, n_l: int, m: int, G: float, DJ, F0l, Omega_res,
                        N_kernel):
-        DJ = f64c(DJ)
-        F0l = np.asfortranarray(np.asarray(F0l, dtype=np.float64)).reshape(-1, order="F")
-        Ores = np.asfortranarray(np.asarray(Omega_res, dtype=np.float64)).reshape(-1, order="F")
+        """DJ, F0l and Omega_res may all be None (set them later with
+        set_problem_arrays, e.g. after compute_orbits_W)."""
         Nk = f64c(N_kernel)
-        if DJ.size != NPh or F0l.size != NPh * n_l or Ores.size != NPh * n_l or Nk.size != N:
-            raise ValueError("upload_problem: array sizes do not match (NPh, N, n_l)")
-        F0l, Ores = f64c(F0l), f64c(Ores)
-        pr = klsp_problem(NPh, N, n_l, int(m), float(G), dptr(DJ), dptr(F0l), dptr(Ores), dptr(Nk))
+        if Nk.size != N:
+            raise ValueError("upload_problem: N_kernel must have N entries")
+        if DJ is None and F0l is None and Omega_res is None:
+            arrs = (None, None, None)
+            ptrs = (None, None, None)
+        else:
+            arrs = self._problem_arrays(NPh, n_l, DJ, F0l, Omega_res)
+            ptrs = tuple(dptr(a) for a in arrs)
+        pr = klsp_problem(NPh, N, n_l, int(m), float(G), *ptrs, dptr(Nk))
         self._chk(lib().klsp_upload_problem(self.handle, ct.byref(pr)))
         self.dims = (int(NPh), int(N), int(n_l))
+
+    def set_problem_arrays(self, DJ, F0l, Omega_res):
+        NPh, _, n_l = self.dims
+        DJ, F0l, Ores = self._problem_arrays(NPh, n_l, DJ, F0l, Omega_res)
+        self._chk(lib().klsp_set_problem_arrays(self.handle, dptr(DJ), dptr(F0l), dptr(Ores)))
+
+    def compute_orbits_W(self, params: klsp_params, grids: dict, alpha, l_arr,
+                         download_orbits: bool = False):
+        """Orbits and W on the device (klsp_compute_orbits_W).  grids holds the
+        [NPh] arrays Rc, e, R1, R2, E, L, L2, SGNL (column-major order of the
+        (NR, Ne) grid).  Returns (Omega_1, Omega_2, w1, ra, pha) with the
+        orbit samples [nwa, NPh] (None unless download_orbits)."""
+        NPh, N, n_l = self.dims
+        ins = [f64c(np.asarray(grids[k]).reshape(-1, order="F"))
+               for k in ("Rc", "e", "R1", "R2", "E", "L", "L2", "SGNL")]
+        if any(x.size != NPh for x in ins):
+            raise ValueError("compute_orbits_W: grid arrays must have NPh entries")
+        al = f64c(alpha)
+        la = np.ascontiguousarray(l_arr, dtype=np.int32)
+        if al.size != N or la.size != n_l:
+            raise ValueError("compute_orbits_W: sizes do not match the uploaded problem")
+        o1, o2 = np.empty(NPh), np.empty(NPh)
+        nwa = int(params.nwa)
+        orb = [np.empty(NPh * nwa) for _ in range(3)] if download_orbits else [None] * 3
+        self._chk(lib().klsp_compute_orbits_W(
+            self.handle, ct.byref(params), *[dptr(x) for x in ins], dptr(al),
+            la.ctypes.data_as(ct.POINTER(ct.c_int32)), dptr(o1), dptr(o2),
+            *[dptr(x) if x is not None else None for x in orb]))
+        return (o1, o2, *orb)
 
     def compute_W(self, w1, ra, pha, Omega_1, Omega_2, alpha, l_arr, download: bool = False):
         NPh, N, n_l = self.dims
@@ -312,4 +361,9 @@ class Context:
     def last_W_ms(self) -> float:
         v = ct.c_double()
         self._chk(lib().klsp_last_W_ms(self.handle, ct.byref(v)))
+        return v.value
+
+    def last_orbit_ms(self) -> float:
+        v = ct.c_double()
+        self._chk(lib().klsp_last_orbit_ms(self.handle, ct.byref(v)))
         return v.value

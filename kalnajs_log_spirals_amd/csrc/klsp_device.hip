@@ -115,6 +115,141 @@ __global__ __launch_bounds__(256) void k_orbit_prep(
     if (threadIdx.x == 0) valid[k] = ok ? 1 : 0;
 }
 
+// compute_orbits (OrbitIntegration.jl:209-302), one workgroup per orbit.
+// Circular (e < 1e-10) and epicyclic (e < 0.01) orbits are closed forms;
+// general orbits need the cumulative trapezoid sums of drs dw sv over the nw
+// phase samples (t = cumsum, :277-281; phase, :287-295).  They run as block
+// scans: each thread sums its contiguous share of the terms, the shares are
+// prefix-summed, and a second pass re-walks each share from its offset and
+// stores every stride-th sample.  The endpoint extrapolations of sv
+// (2 sv[1] - sv[2], ...) are taken from the recomputed neighbours.
+__device__ __forceinline__ double orbit_rv(double x, double E, double l2, bool nonradial)
+{
+    const double q = nonradial ? 2.0 * (E - log(x)) * (x * x) - l2 : 2.0 * (E - log(x));
+    return sqrt(fmax(0.0, q));
+}
+
+__global__ __launch_bounds__(256) void k_orbits(
+    const double* __restrict__ Rc, const double* __restrict__ ecc_a, const double* __restrict__ R1a,
+    const double* __restrict__ R2a, const double* __restrict__ Ea, const double* __restrict__ La,
+    const double* __restrict__ L2a, const double* __restrict__ SGa, const double* __restrict__ wg,
+    const double* __restrict__ cw, const double* __restrict__ sw, int nw, int nwa,
+    double* __restrict__ w1, double* __restrict__ ra, double* __restrict__ pha,
+    double* __restrict__ Om1, double* __restrict__ Om2)
+{
+    const int64_t k = blockIdx.x;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int stride = (nw - 1) / (nwa - 1);
+    const double pi = 3.14159265358979323846;
+    const double Ri = Rc[k], ecc = ecc_a[k], r1 = R1a[k], r2 = R2a[k], Ek = Ea[k], Lk = La[k];
+    const double l2 = L2a[k], sg = SGa[k];
+    double* W1 = w1 + k * nwa;
+    double* RA = ra + k * nwa;
+    double* PH = pha + k * nwa;
+    if (fabs(ecc) < 1e-10 || ecc < 0.01) {
+        const bool circ = fabs(ecc) < 1e-10;
+        const double kap = sqrt(2.0) / Ri, Om = 1.0 / Ri, aj = (r2 - r1) / 2;
+        const double o1 = kap, o2 = Om * sg, c2 = 2 * Om / kap * aj / Ri;
+        for (int a = tid; a < nwa; a += nth) {
+            const int i = a * stride;
+            W1[a] = wg[i];
+            RA[a] = circ ? Ri : Ri - aj * cw[i];
+            PH[a] = wg[i] * o2 / o1 + (circ ? 0.0 : c2 * sw[i] * sg);
+        }
+        if (tid == 0) { Om1[k] = o1; Om2[k] = o2; }
+        return;
+    }
+    const double rs = (r1 + r2) / 2, drs = (r2 - r1) / 2, dw = wg[1] - wg[0];
+    const bool nonradial = l2 > 1e-12;
+    const bool phase = fabs(1.0 - ecc) > 1e-10 && nonradial;
+    auto xs = [&](int i) { return rs - drs * cw[i]; };
+    auto sv1_raw = [&](int i) {
+        const double x = xs(i), rv = orbit_rv(x, Ek, l2, nonradial);
+        return nonradial ? sw[i] * x / rv : sw[i] / rv;
+    };
+    auto sv2_raw = [&](int i) { return sw[i] / orbit_rv(xs(i), Ek, l2, nonradial); };
+    // The first sample is extrapolated before the last one, so at nw = 3 the
+    // last extrapolation reads the extrapolated first sample (:262-263).
+    auto sv1_first = [&]() { return nonradial ? 2.0 * sv1_raw(1) - sv1_raw(2) : sv1_raw(0); };
+    auto sv2_first = [&]() { return 2.0 * sv2_raw(1) - sv2_raw(2); };
+    auto d1 = [&](int i) {
+        double v;
+        if (i == 0) v = sv1_first();
+        else if (i == nw - 1) v = 2.0 * sv1_raw(nw - 2) - (nw == 3 ? sv1_first() : sv1_raw(nw - 3));
+        else v = sv1_raw(i);
+        return drs * dw * v;
+    };
+    auto d2 = [&](int i) {
+        double v;
+        if (i == 0) v = sv2_first();
+        else if (i == nw - 1) v = 2.0 * sv2_raw(nw - 2) - (nw == 3 ? sv2_first() : sv2_raw(nw - 3));
+        else v = sv2_raw(i);
+        return drs * dw * v / xs(i);
+    };
+    // terms j = 1..nw-1; thread t owns [j0, j1)
+    const int chunk = (nw - 1 + nth - 1) / nth;
+    const int j0 = 1 + tid * chunk, j1 = min(nw, j0 + chunk);
+    double s1 = 0.0, s2 = 0.0;
+    if (j0 < j1) {
+        double p1 = d1(j0 - 1), p2 = phase ? d2(j0 - 1) : 0.0;
+        for (int j = j0; j < j1; ++j) {
+            const double c1 = d1(j);
+            s1 += (p1 + c1) / 2;
+            p1 = c1;
+            if (phase) {
+                const double c2v = d2(j);
+                s2 += (p2 + c2v) / 2;
+                p2 = c2v;
+            }
+        }
+    }
+    __shared__ double sh1[256], sh2[256], tot[2];
+    sh1[tid] = s1;
+    sh2[tid] = s2;
+    __syncthreads();
+    if (tid == 0) {
+        double a1 = 0.0, a2 = 0.0;
+        for (int t = 0; t < nth; ++t) {
+            const double v1 = sh1[t], v2 = sh2[t];
+            sh1[t] = a1;
+            sh2[t] = a2;
+            a1 += v1;
+            a2 += v2;
+        }
+        tot[0] = a1;
+        tot[1] = a2;
+    }
+    __syncthreads();
+    const double o1 = pi / tot[0];
+    if (tid == 0) {
+        W1[0] = 0.0 * o1;
+        RA[0] = xs(0);
+        PH[0] = phase ? Lk * 0.0 : 0.0;
+        Om1[k] = o1;
+        Om2[k] = o1 * (phase ? Lk * tot[1] : pi / 2 * 1.0 * sg) / pi;
+    }
+    if (j0 < j1) {
+        double acc1 = sh1[tid], acc2 = sh2[tid];
+        double p1 = d1(j0 - 1), p2 = phase ? d2(j0 - 1) : 0.0;
+        for (int j = j0; j < j1; ++j) {
+            const double c1 = d1(j);
+            acc1 += (p1 + c1) / 2;
+            p1 = c1;
+            if (phase) {
+                const double c2v = d2(j);
+                acc2 += (p2 + c2v) / 2;
+                p2 = c2v;
+            }
+            if (j % stride == 0) {
+                const int a = j / stride;
+                W1[a] = acc1 * o1;
+                RA[a] = xs(j);
+                PH[a] = phase ? Lk * acc2 : pi / 2 * 1.0 * sg;
+            }
+        }
+    }
+}
+
 // W_k = A_k B_k per orbit on FP64 MFMA: A[l, t] = wt_t cos(l w1_t + m phi_t)
 // (real, n_l x nwa), B[t, a] = exp(-i alpha_a ln r_t) (nwa x N).
 // Workgroup = (orbit k, 32 alphas, group z of 16 l-blocks); wave w owns the
@@ -509,6 +644,7 @@ struct klsp_ctx {
     int n_cu = 256;
     // problem
     bool have_problem = false;
+    bool have_arrays = false;  // DJ, F0l, Omega_res set
     int64_t NPh = 0, N = 0, n_l = 0, K = 0, Kp = 0;
     int Np = 0, m = 0;
     double G = 0.0;
@@ -533,6 +669,7 @@ struct klsp_ctx {
     hipEvent_t ev[6] = {};
     klsp_timing last{};
     double last_W_ms = 0.0;
+    double last_orbit_ms = 0.0;
 };
 
 static const char* g_noctx_err = "klsp: null context";
@@ -652,7 +789,7 @@ static int free_problem(klsp_ctx* ctx)
     ctx->d_W64 = nullptr;
     ctx->d_W32 = nullptr;
     ctx->d_Wb = nullptr;
-    ctx->have_problem = ctx->have_W = false;
+    ctx->have_problem = ctx->have_W = ctx->have_arrays = false;
     return KLSP_OK;
 }
 
@@ -718,8 +855,10 @@ const char* klsp_last_error(const klsp_ctx* ctx) { return ctx ? ctx->err.c_str()
 int klsp_upload_problem(klsp_ctx* ctx, const klsp_problem* pr)
 {
     if (!ctx) return KLSP_ERR_INVALID_ARG;
-    if (!pr || pr->NPh < 1 || pr->N_alpha < 1 || pr->n_l < 1 || !pr->DJ || !pr->F0l ||
-        !pr->Omega_res || !pr->N_kernel)
+    const bool arrays = pr && pr->DJ && pr->F0l && pr->Omega_res;
+    const bool none = pr && !pr->DJ && !pr->F0l && !pr->Omega_res;
+    if (!pr || pr->NPh < 1 || pr->N_alpha < 1 || pr->n_l < 1 || !(arrays || none) ||
+        !pr->N_kernel)
         return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_upload_problem: bad problem description");
     if (pr->N_alpha > kMaxN)
         return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_upload_problem: N_alpha exceeds 1024");
@@ -739,11 +878,27 @@ int klsp_upload_problem(klsp_ctx* ctx, const klsp_problem* pr)
     KLSP_HIP(ctx, hipMalloc((void**)&ctx->d_F0l, sizeof(double) * nK));
     KLSP_HIP(ctx, hipMalloc((void**)&ctx->d_Ores, sizeof(double) * nK));
     KLSP_HIP(ctx, hipMalloc((void**)&ctx->d_Nk, sizeof(double) * ctx->N));
-    KLSP_HIP(ctx, hipMemcpy(ctx->d_DJ, pr->DJ, sizeof(double) * ctx->NPh, hipMemcpyHostToDevice));
-    KLSP_HIP(ctx, hipMemcpy(ctx->d_F0l, pr->F0l, sizeof(double) * nK, hipMemcpyHostToDevice));
-    KLSP_HIP(ctx, hipMemcpy(ctx->d_Ores, pr->Omega_res, sizeof(double) * nK, hipMemcpyHostToDevice));
     KLSP_HIP(ctx, hipMemcpy(ctx->d_Nk, pr->N_kernel, sizeof(double) * ctx->N, hipMemcpyHostToDevice));
     ctx->have_problem = true;
+    if (arrays) return klsp_set_problem_arrays(ctx, pr->DJ, pr->F0l, pr->Omega_res);
+    ctx->have_arrays = false;
+    return KLSP_OK;
+}
+
+int klsp_set_problem_arrays(klsp_ctx* ctx, const double* DJ, const double* F0l,
+                            const double* Omega_res)
+{
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    if (!ctx->have_problem)
+        return fail(ctx, KLSP_ERR_NOT_READY, "klsp_set_problem_arrays: upload the problem first");
+    if (!DJ || !F0l || !Omega_res)
+        return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_set_problem_arrays: null array");
+    KLSP_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t nK = (size_t)ctx->K;
+    KLSP_HIP(ctx, hipMemcpy(ctx->d_DJ, DJ, sizeof(double) * ctx->NPh, hipMemcpyHostToDevice));
+    KLSP_HIP(ctx, hipMemcpy(ctx->d_F0l, F0l, sizeof(double) * nK, hipMemcpyHostToDevice));
+    KLSP_HIP(ctx, hipMemcpy(ctx->d_Ores, Omega_res, sizeof(double) * nK, hipMemcpyHostToDevice));
+    ctx->have_arrays = true;
     return KLSP_OK;
 }
 
@@ -759,33 +914,39 @@ static int alloc_W64(klsp_ctx* ctx)
     return KLSP_OK;
 }
 
-int klsp_compute_W(klsp_ctx* ctx, int64_t nwa, const double* w1, const double* ra,
-                   const double* pha, const double* Om1, const double* Om2,
-                   const double* alpha, const int32_t* l_arr, double* W_out)
+}  // extern "C"
+
+// Device orbit buffers of one W quadrature ([NPh][nwa] samples, [NPh] frequencies).
+struct OrbitBufs {
+    double *w1 = nullptr, *ra = nullptr, *pha = nullptr, *o1 = nullptr, *o2 = nullptr;
+    ~OrbitBufs() { hipFree(w1); hipFree(ra); hipFree(pha); hipFree(o1); hipFree(o2); }
+};
+
+static hipError_t alloc_orbits(OrbitBufs& ob, int64_t NPh, int64_t nwa)
 {
-    if (!ctx) return KLSP_ERR_INVALID_ARG;
-    if (!ctx->have_problem)
-        return fail(ctx, KLSP_ERR_NOT_READY, "klsp_compute_W: upload the problem first");
-    if (nwa < 2 || !w1 || !ra || !pha || !Om1 || !Om2 || !alpha || !l_arr)
-        return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_compute_W: bad arguments");
-    KLSP_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t no = (size_t)NPh * nwa;
+    hipError_t e = hipMalloc((void**)&ob.w1, no * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&ob.ra, no * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&ob.pha, no * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&ob.o1, NPh * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&ob.o2, NPh * 8);
+    return e;
+}
+
+// compute_W_l from orbits already on the device (k_orbit_prep + k_W_f64).
+static int W_from_orbits(klsp_ctx* ctx, int64_t nwa, const OrbitBufs& ob, const double* alpha,
+                         const int32_t* l_arr, const char* who)
+{
     const int64_t NPh = ctx->NPh, nwa_p = round_up(nwa, 4);
-    const size_t no = (size_t)NPh * nwa, np = (size_t)NPh * nwa_p;
-    double *d_w1 = nullptr, *d_ra = nullptr, *d_pha = nullptr, *d_o1 = nullptr, *d_o2 = nullptr;
+    const size_t np = (size_t)NPh * nwa_p;
     double *d_u = nullptr, *d_wt = nullptr, *d_x1 = nullptr, *d_mp = nullptr, *d_al = nullptr;
     int *d_valid = nullptr, *d_l = nullptr;
     auto cleanup = [&]() {
-        hipFree(d_w1); hipFree(d_ra); hipFree(d_pha); hipFree(d_o1); hipFree(d_o2);
         hipFree(d_u); hipFree(d_wt); hipFree(d_x1); hipFree(d_mp); hipFree(d_al);
         hipFree(d_valid); hipFree(d_l);
     };
     hipError_t e = hipSuccess;
 #define KLSP_TRY(call) if (e == hipSuccess) e = (call)
-    KLSP_TRY(hipMalloc((void**)&d_w1, no * 8));
-    KLSP_TRY(hipMalloc((void**)&d_ra, no * 8));
-    KLSP_TRY(hipMalloc((void**)&d_pha, no * 8));
-    KLSP_TRY(hipMalloc((void**)&d_o1, NPh * 8));
-    KLSP_TRY(hipMalloc((void**)&d_o2, NPh * 8));
     KLSP_TRY(hipMalloc((void**)&d_u, np * 8));
     KLSP_TRY(hipMalloc((void**)&d_wt, np * 8));
     KLSP_TRY(hipMalloc((void**)&d_x1, np * 8));
@@ -793,24 +954,19 @@ int klsp_compute_W(klsp_ctx* ctx, int64_t nwa, const double* w1, const double* r
     KLSP_TRY(hipMalloc((void**)&d_al, ctx->N * 8));
     KLSP_TRY(hipMalloc((void**)&d_valid, NPh * 4));
     KLSP_TRY(hipMalloc((void**)&d_l, ctx->n_l * 4));
-    KLSP_TRY(hipMemcpy(d_w1, w1, no * 8, hipMemcpyHostToDevice));
-    KLSP_TRY(hipMemcpy(d_ra, ra, no * 8, hipMemcpyHostToDevice));
-    KLSP_TRY(hipMemcpy(d_pha, pha, no * 8, hipMemcpyHostToDevice));
-    KLSP_TRY(hipMemcpy(d_o1, Om1, NPh * 8, hipMemcpyHostToDevice));
-    KLSP_TRY(hipMemcpy(d_o2, Om2, NPh * 8, hipMemcpyHostToDevice));
     KLSP_TRY(hipMemcpy(d_al, alpha, ctx->N * 8, hipMemcpyHostToDevice));
     KLSP_TRY(hipMemcpy(d_l, l_arr, ctx->n_l * 4, hipMemcpyHostToDevice));
 #undef KLSP_TRY
     if (e != hipSuccess) {
         cleanup();
         return fail(ctx, e == hipErrorOutOfMemory ? KLSP_ERR_OOM : KLSP_ERR_HIP,
-                    std::string("klsp_compute_W: ") + hipGetErrorString(e));
+                    std::string(who) + ": " + hipGetErrorString(e));
     }
     int rc = alloc_W64(ctx);
     if (rc != KLSP_OK) { cleanup(); return rc; }
     hipEventRecord(ctx->ev[0], ctx->stream);
-    hipLaunchKernelGGL(k_orbit_prep, dim3((unsigned)NPh), dim3(256), 0, ctx->stream, d_w1, d_ra,
-                       d_pha, d_o1, d_o2, (int)nwa, (int)nwa_p, ctx->m, d_u, d_wt, d_x1, d_mp,
+    hipLaunchKernelGGL(k_orbit_prep, dim3((unsigned)NPh), dim3(256), 0, ctx->stream, ob.w1, ob.ra,
+                       ob.pha, ob.o1, ob.o2, (int)nwa, (int)nwa_p, ctx->m, d_u, d_wt, d_x1, d_mp,
                        d_valid);
     const unsigned lgroups = (unsigned)((ctx->n_l + 16 * 4 * kWMaxLB - 1) / (16 * 4 * kWMaxLB));
     hipLaunchKernelGGL(k_W_f64, dim3((unsigned)NPh, (unsigned)((ctx->N + 31) / 32), lgroups),
@@ -824,10 +980,98 @@ int klsp_compute_W(klsp_ctx* ctx, int64_t nwa, const double* w1, const double* r
     hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
     ctx->last_W_ms = ms;
     cleanup();
-    if (e != hipSuccess)
-        return fail(ctx, KLSP_ERR_HIP, std::string("klsp_compute_W: ") + hipGetErrorString(e));
+    if (e != hipSuccess) return fail(ctx, KLSP_ERR_HIP, std::string(who) + ": " + hipGetErrorString(e));
     ctx->have_W = true;
+    return KLSP_OK;
+}
+
+extern "C" {
+
+int klsp_compute_W(klsp_ctx* ctx, int64_t nwa, const double* w1, const double* ra,
+                   const double* pha, const double* Om1, const double* Om2,
+                   const double* alpha, const int32_t* l_arr, double* W_out)
+{
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    if (!ctx->have_problem)
+        return fail(ctx, KLSP_ERR_NOT_READY, "klsp_compute_W: upload the problem first");
+    if (nwa < 2 || !w1 || !ra || !pha || !Om1 || !Om2 || !alpha || !l_arr)
+        return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_compute_W: bad arguments");
+    KLSP_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t NPh = ctx->NPh;
+    const size_t no = (size_t)NPh * nwa;
+    OrbitBufs ob;
+    hipError_t e = alloc_orbits(ob, NPh, nwa);
+    if (e == hipSuccess) e = hipMemcpy(ob.w1, w1, no * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ob.ra, ra, no * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ob.pha, pha, no * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ob.o1, Om1, NPh * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ob.o2, Om2, NPh * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+        return fail(ctx, e == hipErrorOutOfMemory ? KLSP_ERR_OOM : KLSP_ERR_HIP,
+                    std::string("klsp_compute_W: ") + hipGetErrorString(e));
+    int rc = W_from_orbits(ctx, nwa, ob, alpha, l_arr, "klsp_compute_W");
+    if (rc) return rc;
     if (W_out) return klsp_download_W(ctx, W_out);
+    return KLSP_OK;
+}
+
+int klsp_compute_orbits_W(klsp_ctx* ctx, const klsp_params* p, const double* Rc,
+                          const double* e, const double* R1, const double* R2, const double* E,
+                          const double* L, const double* L2, const double* SGNL,
+                          const double* alpha, const int32_t* l_arr, double* Om1, double* Om2,
+                          double* w1, double* ra, double* pha)
+{
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    if (!ctx->have_problem)
+        return fail(ctx, KLSP_ERR_NOT_READY, "klsp_compute_orbits_W: upload the problem first");
+    if (!p || !Rc || !e || !R1 || !R2 || !E || !L || !L2 || !SGNL || !alpha || !l_arr || !Om1 ||
+        !Om2 || p->nw < 3 || p->nwa < 2 || (p->nw - 1) % (p->nwa - 1) != 0 ||
+        (int64_t)p->NR * p->Ne != ctx->NPh || p->N_alpha != ctx->N ||
+        p->l_max - p->l_min + 1 != ctx->n_l)
+        return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_compute_orbits_W: bad arguments");
+    KLSP_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t NPh = ctx->NPh, nwa = p->nwa;
+    const int nw = p->nw;
+    std::vector<double> wg(nw), cw(nw), sw(nw);
+    klsp_orbit_phase_grid(p, wg.data(), cw.data(), sw.data());
+    OrbitBufs ob;
+    double* d_in = nullptr;     // Rc, e, R1, R2, E, L, L2, SGNL, then wg, cw, sw
+    hipError_t st = alloc_orbits(ob, NPh, nwa);
+    if (st == hipSuccess) st = hipMalloc((void**)&d_in, sizeof(double) * (8 * NPh + 3 * nw));
+    const double* ins[8] = {Rc, e, R1, R2, E, L, L2, SGNL};
+    for (int i = 0; i < 8 && st == hipSuccess; ++i)
+        st = hipMemcpy(d_in + i * NPh, ins[i], sizeof(double) * NPh, hipMemcpyHostToDevice);
+    if (st == hipSuccess) st = hipMemcpy(d_in + 8 * NPh, wg.data(), 8 * nw, hipMemcpyHostToDevice);
+    if (st == hipSuccess) st = hipMemcpy(d_in + 8 * NPh + nw, cw.data(), 8 * nw, hipMemcpyHostToDevice);
+    if (st == hipSuccess) st = hipMemcpy(d_in + 8 * NPh + 2 * nw, sw.data(), 8 * nw, hipMemcpyHostToDevice);
+    if (st == hipSuccess) {
+        hipEventRecord(ctx->ev[0], ctx->stream);
+        hipLaunchKernelGGL(k_orbits, dim3((unsigned)NPh), dim3(256), 0, ctx->stream, d_in,
+                           d_in + NPh, d_in + 2 * NPh, d_in + 3 * NPh, d_in + 4 * NPh,
+                           d_in + 5 * NPh, d_in + 6 * NPh, d_in + 7 * NPh, d_in + 8 * NPh,
+                           d_in + 8 * NPh + nw, d_in + 8 * NPh + 2 * nw, nw, (int)nwa, ob.w1,
+                           ob.ra, ob.pha, ob.o1, ob.o2);
+        hipEventRecord(ctx->ev[1], ctx->stream);
+        st = hipGetLastError();
+    }
+    if (st == hipSuccess) st = hipStreamSynchronize(ctx->stream);
+    if (st == hipSuccess) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+        ctx->last_orbit_ms = ms;
+    }
+    hipFree(d_in);
+    if (st != hipSuccess)
+        return fail(ctx, st == hipErrorOutOfMemory ? KLSP_ERR_OOM : KLSP_ERR_HIP,
+                    std::string("klsp_compute_orbits_W: ") + hipGetErrorString(st));
+    int rc = W_from_orbits(ctx, nwa, ob, alpha, l_arr, "klsp_compute_orbits_W");
+    if (rc) return rc;
+    const size_t no = (size_t)NPh * nwa;
+    KLSP_HIP(ctx, hipMemcpy(Om1, ob.o1, NPh * 8, hipMemcpyDeviceToHost));
+    KLSP_HIP(ctx, hipMemcpy(Om2, ob.o2, NPh * 8, hipMemcpyDeviceToHost));
+    if (w1) KLSP_HIP(ctx, hipMemcpy(w1, ob.w1, no * 8, hipMemcpyDeviceToHost));
+    if (ra) KLSP_HIP(ctx, hipMemcpy(ra, ob.ra, no * 8, hipMemcpyDeviceToHost));
+    if (pha) KLSP_HIP(ctx, hipMemcpy(pha, ob.pha, no * 8, hipMemcpyDeviceToHost));
     return KLSP_OK;
 }
 
@@ -1026,8 +1270,9 @@ static int det_slots(const klsp_ctx* ctx)
 
 static int check_ready(klsp_ctx* ctx, int precision, const char* who)
 {
-    if (!ctx->have_problem || !ctx->have_W)
-        return fail(ctx, KLSP_ERR_NOT_READY, std::string(who) + ": problem and W must be on device");
+    if (!ctx->have_problem || !ctx->have_W || !ctx->have_arrays)
+        return fail(ctx, KLSP_ERR_NOT_READY,
+                    std::string(who) + ": problem arrays and W must be on device");
     if (precision != KLSP_PREC_FP64 && precision != KLSP_PREC_FP32 &&
         precision != KLSP_PREC_BF16X3)
         return fail(ctx, KLSP_ERR_INVALID_ARG, std::string(who) + ": unknown precision");
@@ -1459,6 +1704,13 @@ int klsp_last_W_ms(klsp_ctx* ctx, double* ms)
 {
     if (!ctx || !ms) return KLSP_ERR_INVALID_ARG;
     *ms = ctx->last_W_ms;
+    return KLSP_OK;
+}
+
+int klsp_last_orbit_ms(klsp_ctx* ctx, double* ms)
+{
+    if (!ctx || !ms) return KLSP_ERR_INVALID_ARG;
+    *ms = ctx->last_orbit_ms;
     return KLSP_OK;
 }
 

@@ -379,6 +379,28 @@ int klsp_compute_orbits(const klsp_params* p, const double* Rc, const double* e,
     worker();
     for (auto& th : pool) th.join();
 
+    return klsp_compute_jacobian_df(p, Rc, R1, R2, E, L, Om1, jac, F0, FE, FL);
+}
+
+int klsp_orbit_phase_grid(const klsp_params* p, double* w, double* cw, double* sw)
+{
+    if (!params_ok(p) || !w || !cw || !sw) return KLSP_ERR_INVALID_ARG;
+    for (int i = 0; i < p->nw; ++i) {
+        w[i] = range_at(1e-8, kPi - 1e-8, p->nw, i);
+        cw[i] = std::cos(w[i]);
+        sw[i] = std::sin(w[i]);
+    }
+    return KLSP_OK;
+}
+
+int klsp_compute_jacobian_df(const klsp_params* p, const double* Rc, const double* R1,
+                             const double* R2, const double* E, const double* L,
+                             const double* Om1, double* jac, double* F0, double* FE, double* FL)
+{
+    if (!params_ok(p) || !Rc || !R1 || !R2 || !E || !L || !Om1 || !jac || !F0 || !FE || !FL)
+        return KLSP_ERR_INVALID_ARG;
+    const int64_t NPh = (int64_t)p->NR * p->Ne;
+    const Model md(*p);
     // Jacobian (:341-382) and DF on the grid (:317-333).
     for (int64_t k = 0; k < NPh; ++k) {
         const double r1 = R1[k], r2 = R2[k], rc = Rc[k], Ek = E[k], Lk = L[k], o1 = Om1[k];
