@@ -60,6 +60,47 @@ __device__ __forceinline__ double2 d_inv(double c, double d)
     return make_double2(p * s, q * s);
 }
 
+// sin x and cos x for |x| < 2^20, without branches: Cody-Waite reduction by
+// pi/2 with pi/2 split into three doubles (x - q P1 - q P2 - q P3, each step
+// an FMA), the Taylor series of sin (through x^17) and cos (through x^18) on
+// [-pi/4, pi/4] -- the first omitted terms are below 1e-19 there -- and the
+// quadrant q mod 4.  Absolute error <= 1.2e-16 (checked against glibc over
+// |x| < 2^20 and at multiples of pi/2).
+__device__ __forceinline__ void sincos_small(double x, double* sn, double* cs)
+{
+    const double q = rint(x * 0x1.45f306dc9c883p-1);          // x * 2/pi
+    double r = fma(-q, 0x1.921fb54442d18p+0, x);
+    r = fma(-q, 0x1.1a62633145c07p-54, r);
+    r = fma(-q, -0x1.f1976b7ed8fbcp-110, r);
+    const double z = r * r;
+    double ps = fma(z, 1.0 / 355687428096000.0, -1.0 / 1307674368000.0);
+    ps = fma(z, ps, 1.0 / 6227020800.0);
+    ps = fma(z, ps, -1.0 / 39916800.0);
+    ps = fma(z, ps, 1.0 / 362880.0);
+    ps = fma(z, ps, -1.0 / 5040.0);
+    ps = fma(z, ps, 1.0 / 120.0);
+    ps = fma(z, ps, -1.0 / 6.0);
+    const double S = fma(r * z, ps, r);
+    double pc = fma(z, -1.0 / 6402373705728000.0, 1.0 / 20922789888000.0);
+    pc = fma(z, pc, -1.0 / 87178291200.0);
+    pc = fma(z, pc, 1.0 / 479001600.0);
+    pc = fma(z, pc, -1.0 / 3628800.0);
+    pc = fma(z, pc, 1.0 / 40320.0);
+    pc = fma(z, pc, -1.0 / 720.0);
+    pc = fma(z, pc, 1.0 / 24.0);
+    pc = fma(z, pc, -0.5);
+    const double C = fma(z, pc, 1.0);
+    const int n = (int)q & 3;
+    double ss = (n & 1) ? C : S, cc = (n & 1) ? S : C;
+    if (n & 2) ss = -ss;
+    if ((n + 1) & 2) cc = -cc;
+    *sn = ss;
+    *cs = cc;
+}
+
+// Largest |argument| given to sincos_small by the W quadrature.
+constexpr double kSmallTrigArg = 1048576.0;   // 2^20
+
 __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -74,19 +115,22 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 // non-uniform w1 samples (:229), phi = (Om2/Om1) w1 - pha (:232-235).  Emits
 // per sample: u = ln r, wt = Sw1 / (pi sqrt r) (0 where the integrand is
 // non-finite, :248), w1 and m*phi, padded to nwa_p samples with wt = 0.
+// valid[k] = 0 skips the orbit; bit 1 marks that every argument of the
+// quadrature's sines and cosines (l w1 + m phi with |l| <= lmax, 16 w1,
+// alpha ln r with |alpha| <= amax) stays below kSmallTrigArg.
 __global__ __launch_bounds__(256) void k_orbit_prep(
     const double* __restrict__ w1, const double* __restrict__ ra,
     const double* __restrict__ pha, const double* __restrict__ Om1,
-    const double* __restrict__ Om2, int nwa, int nwa_p, int m, double* __restrict__ u,
-    double* __restrict__ wt, double* __restrict__ w1o, double* __restrict__ mph,
-    int* __restrict__ valid)
+    const double* __restrict__ Om2, int nwa, int nwa_p, int m, double lmax, double amax,
+    double* __restrict__ u, double* __restrict__ wt, double* __restrict__ w1o,
+    double* __restrict__ mph, int* __restrict__ valid)
 {
     const int64_t k = blockIdx.x;
     const double* W1 = w1 + k * nwa;
     const double* RA = ra + k * nwa;
     const double* PH = pha + k * nwa;
-    __shared__ int bad;
-    if (threadIdx.x == 0) bad = 0;
+    __shared__ int bad, big;
+    if (threadIdx.x == 0) { bad = 0; big = 0; }
     __syncthreads();
     for (int t = threadIdx.x; t < nwa; t += blockDim.x)
         if (!isfinite(W1[t]) || !isfinite(RA[t]) || !isfinite(PH[t])) atomicOr(&bad, 1);
@@ -108,11 +152,15 @@ __global__ __launch_bounds__(256) void k_orbit_prep(
             lu = log(r);
             w = sw / sqrt(r) * inv_pi;
             if (!(r > 0.0) || !isfinite(lu) || !isfinite(w)) { lu = 0.0; w = 0.0; }
+            if (!(lmax * fabs(x1) + fabs(mp) < kSmallTrigArg) ||
+                !(16.0 * fabs(x1) < kSmallTrigArg) || !(amax * fabs(lu) < kSmallTrigArg))
+                atomicOr(&big, 1);
         }
         const int64_t o = k * nwa_p + t;
         u[o] = lu; wt[o] = w; w1o[o] = x1; mph[o] = mp;
     }
-    if (threadIdx.x == 0) valid[k] = ok ? 1 : 0;
+    __syncthreads();
+    if (threadIdx.x == 0) valid[k] = ok ? (big ? 1 : 3) : 0;
 }
 
 // compute_orbits (OrbitIntegration.jl:209-302), one workgroup per orbit.
@@ -252,83 +300,221 @@ __global__ __launch_bounds__(256) void k_orbits(
 
 // W_k = A_k B_k per orbit on FP64 MFMA: A[l, t] = wt_t cos(l w1_t + m phi_t)
 // (real, n_l x nwa), B[t, a] = exp(-i alpha_a ln r_t) (nwa x N).
-// Workgroup = (orbit k, 32 alphas, group z of 16 l-blocks); wave w owns the
-// group's l-blocks w, w+4, w+8, w+12.
-// Operands are generated in registers in the MFMA lane layout:
+// Workgroup = (orbit k, 64 alphas, 128 l) of twelve waves: waves 8-11
+// generate the operands of stage s (16 samples t) into one LDS buffer while
+// waves 0-7 run the MFMAs of stage s-1 from the other; one barrier per
+// stage.  MFMA wave w owns alpha-block w % 4 (re and im) and l-blocks
+// 4 (w / 4) .. +3.  On gfx950 FP64 VALU instructions and FP64 MFMAs share
+// one pipeline per SIMD (measured: a wave of each on one SIMD takes the sum
+// of their times), so the split does not hide operand generation; the time
+// is the MFMAs plus the generation's FP64 instructions, which is why the
+// generation uses rotations and a branch-free short sincos.  Lane layouts:
 //   A (16 l x 4 t): lane = i + 16*kk holds A[l0+i][t0+kk]
 //   B (4 t x 16 a): lane = i + 16*kk holds B[t0+kk][a0+i]
 //   D (16 l x 16 a): lane holds D[kk + 4*v][i], v = 0..3 (the FP64 MFMA's
 //   accumulator layout; the FP32 16x16x4 MFMA uses D[4*kk + v][i])
-constexpr int kWMaxLB = 4;
+// When l_arr is a run of consecutive integers (the reference's l_min:l_max)
+// a thread's l = l_b + 16 j for j = 0..7 come from one sincos and seven
+// rotations by exp(16 i w1); evenly spaced alphas likewise by rotations by
+// exp(-16 i dalpha ln r); otherwise every value is evaluated directly.
+constexpr int kWTk = 16;    // samples t per LDS stage
+constexpr int kWLc = 128;   // l per workgroup (8 l-blocks)
+constexpr int kWAc = 64;    // alphas per workgroup (4 alpha-blocks)
+constexpr int kWMmaWaves = 8;   // MFMA waves; 4 more generate operands
 
-__global__ __launch_bounds__(256) void k_W_f64(
-    const double* __restrict__ u, const double* __restrict__ wt,
-    const double* __restrict__ w1o, const double* __restrict__ mph,
-    const int* __restrict__ valid, const double* __restrict__ alpha,
-    const int* __restrict__ l_arr, int64_t NPh, int N, int Np, int n_l, int nwa_p,
-    double2* __restrict__ W)
+struct WStage {
+    double A[kWTk][kWLc];
+    double Br[kWTk][kWAc], Bi[kWTk][kWAc];
+};
+
+template <bool kSmall>
+__device__ __forceinline__ void w_sincos(double x, double* sn, double* cs)
 {
-    const int64_t k = blockIdx.x;
-    if (!valid[k]) return;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = lane & 15, kk = lane >> 4;
-    const int a0 = blockIdx.y * 32;
-    const int nLB = (n_l + 15) / 16;
-    const int lb0 = blockIdx.z * 4 * kWMaxLB + wave;   // first l-block of this wave
-    const bool bj0 = a0 < N, bj1 = a0 + 16 < N;
-    double al[2];
-    for (int j = 0; j < 2; ++j) {
-        const int a = a0 + 16 * j + i;
+    if (kSmall) sincos_small(x, sn, cs);
+    else sincos(x, sn, cs);
+}
+
+// Operand generation for one stage by the 256 threads of waves 4-7:
+// sample ft = p / 16 of the stage, column group fg = p % 16.
+template <bool kSmall>
+__device__ __forceinline__ void w_fill(
+    WStage& st, int p, int lc0, int nlb, int n_l, const int* __restrict__ l_arr, int l_consec,
+    double lb, const double* al, double dalpha, double ut, double wtt, double w1t, double mp)
+{
+    const int ft = p >> 4, fg = p & 15;
+    if (l_consec) {
+        double s0, c0, sr, cr;
+        w_sincos<kSmall>(lb * w1t + mp, &s0, &c0);
+        w_sincos<kSmall>(16.0 * w1t, &sr, &cr);
+        s0 *= wtt;                           // the rotations carry the weight along
+        c0 *= wtt;
+#pragma unroll
+        for (int j = 0; j < kWLc / 16; ++j) {
+            if (j >= nlb) break;             // rows of inactive l-blocks are never read
+            const int l = lc0 + fg + 16 * j;
+            st.A[ft][fg + 16 * j] = l < n_l ? c0 : 0.0;
+            const double c1 = c0 * cr - s0 * sr;
+            s0 = s0 * cr + c0 * sr;
+            c0 = c1;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kWLc / 16; ++j) {
+            if (j >= nlb) break;
+            const int l = lc0 + fg + 16 * j;
+            double sl = 0.0, cl = 0.0;
+            if (l < n_l) w_sincos<kSmall>((double)l_arr[l] * w1t + mp, &sl, &cl);
+            st.A[ft][fg + 16 * j] = wtt * cl;
+        }
+    }
+    if (dalpha != 0.0) {
+        double sb, cb, sr, cr;
+        w_sincos<kSmall>(-al[0] * ut, &sb, &cb);
+        w_sincos<kSmall>(-16.0 * dalpha * ut, &sr, &cr);
+#pragma unroll
+        for (int j = 0; j < kWAc / 16; ++j) {
+            st.Br[ft][fg + 16 * j] = cb;
+            st.Bi[ft][fg + 16 * j] = sb;
+            const double c1 = cb * cr - sb * sr;
+            sb = sb * cr + cb * sr;
+            cb = c1;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kWAc / 16; ++j) {
+            double sb, cb;
+            w_sincos<kSmall>(-al[j] * ut, &sb, &cb);
+            st.Br[ft][fg + 16 * j] = cb;
+            st.Bi[ft][fg + 16 * j] = sb;
+        }
+    }
+}
+
+// Operand generation by waves 4..: stage s into buffer s & 1, one barrier
+// per stage (nst + 1 in all, matching the MFMA waves).  kSmall selects
+// sincos_small (every argument of the orbit below kSmallTrigArg) or the
+// library's sincos.
+template <bool kSmall>
+__device__ __forceinline__ void w_produce(
+    WStage* st, int64_t k, const double* __restrict__ u, const double* __restrict__ wt,
+    const double* __restrict__ w1o, const double* __restrict__ mph,
+    const double* __restrict__ alpha, const int* __restrict__ l_arr, int l_consec,
+    double dalpha, int N, int n_l, int nwa_p, int lc0, int ac0, int nlb, int nst)
+{
+    const int p = threadIdx.x - 64 * kWMmaWaves, ft = p >> 4, fg = p & 15;
+    const double lb = lc0 + fg < n_l ? (double)l_arr[lc0 + fg] : 0.0;
+    double al[kWAc / 16];
+#pragma unroll
+    for (int j = 0; j < kWAc / 16; ++j) {
+        const int a = ac0 + fg + 16 * j;
         al[j] = a < N ? alpha[a] : 0.0;
     }
-    double lv[kWMaxLB];
-    for (int q = 0; q < kWMaxLB; ++q) {
-        const int li = (lb0 + 4 * q) * 16 + i;
-        lv[q] = li < n_l ? (double)l_arr[li] : 0.0;
-    }
-    v4d accr[kWMaxLB][2], acci[kWMaxLB][2];
-    for (int q = 0; q < kWMaxLB; ++q)
-        for (int j = 0; j < 2; ++j) { accr[q][j] = (v4d){0, 0, 0, 0}; acci[q][j] = accr[q][j]; }
-
     const double* U = u + k * nwa_p;
     const double* WT = wt + k * nwa_p;
     const double* X1 = w1o + k * nwa_p;
     const double* MP = mph + k * nwa_p;
-    for (int t0 = 0; t0 < nwa_p; t0 += 4) {
-        const int t = t0 + kk;
-        const double ut = U[t], wtt = WT[t], w1t = X1[t], mp = MP[t];
-        double br[2], bi[2];
-        for (int j = 0; j < 2; ++j) {
-            const double y = -al[j] * ut;
-            sincos(y, &bi[j], &br[j]);
+    double ut = 0.0, wtt = 0.0, w1t = 0.0, mp = 0.0;
+    if (ft < nwa_p) { ut = U[ft]; wtt = WT[ft]; w1t = X1[ft]; mp = MP[ft]; }
+    for (int s = 0; s <= nst; ++s) {
+        if (s < nst) {
+            const int tn = (s + 1) * kWTk + ft;   // next stage's sample, loaded early
+            double utn = 0.0, wtn = 0.0, w1n = 0.0, mpn = 0.0;
+            if (tn < nwa_p) { utn = U[tn]; wtn = WT[tn]; w1n = X1[tn]; mpn = MP[tn]; }
+            w_fill<kSmall>(st[s & 1], p, lc0, nlb, n_l, l_arr, l_consec, lb, al, dalpha, ut, wtt,
+                           w1t, mp);
+            ut = utn; wtt = wtn; w1t = w1n; mp = mpn;
         }
-#pragma unroll
-        for (int q = 0; q < kWMaxLB; ++q) {
-            if (lb0 + 4 * q >= nLB) break;
-            const double a = wtt * cos(lv[q] * w1t + mp);
-            if (bj0) {
-                accr[q][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, br[0], accr[q][0], 0, 0, 0);
-                acci[q][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bi[0], acci[q][0], 0, 0, 0);
-            }
-            if (bj1) {
-                accr[q][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, br[1], accr[q][1], 0, 0, 0);
-                acci[q][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bi[1], acci[q][1], 0, 0, 0);
-            }
-        }
+        __syncthreads();
     }
+}
+
+// MFMA wave: alpha-block ab, l-blocks lq0 .. lq0 + NQ - 1 (NQ fixed so the
+// MFMA sequence has no branches); the MFMAs of stage s-1 read buffer
+// (s-1) & 1 while stage s is generated.
+template <int NQ>
+__device__ __forceinline__ void w_consume(
+    const WStage* st, int64_t k, int nst, int ab, int lq0, int lc0, int ac0, int64_t NPh, int N,
+    int Np, int n_l, bool on, double2* __restrict__ W)
+{
+    const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
+    v4d accr[NQ], acci[NQ];
 #pragma unroll
-    for (int q = 0; q < kWMaxLB; ++q) {
-        const int lb = lb0 + 4 * q;
-        if (lb >= nLB) break;
-        for (int j = 0; j < 2; ++j) {
-            const int a = a0 + 16 * j + i;
-            if (a >= N) continue;
-            for (int v = 0; v < 4; ++v) {
-                const int li = lb * 16 + kk + 4 * v;   // f64 D layout: row = lane/16 + 4v
-                if (li < n_l)
-                    W[((int64_t)li * NPh + k) * Np + a] = make_double2(accr[q][j][v], acci[q][j][v]);
+    for (int q = 0; q < NQ; ++q) { accr[q] = (v4d){0, 0, 0, 0}; acci[q] = accr[q]; }
+    for (int s = 0; s <= nst; ++s) {
+        if (s >= 1 && on) {
+            const WStage& b = st[(s - 1) & 1];
+            // operands of k-step ks+1 are read before the MFMAs of ks
+            double an[NQ], brn, bin;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) an[q] = b.A[kk][16 * (lq0 + q) + i];
+            brn = b.Br[kk][16 * ab + i];
+            bin = b.Bi[kk][16 * ab + i];
+#pragma unroll
+            for (int ks = 0; ks < kWTk / 4; ++ks) {
+                double a[NQ];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) a[q] = an[q];
+                const double br = brn, bi = bin;
+                if (ks + 1 < kWTk / 4) {
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) an[q] = b.A[4 * (ks + 1) + kk][16 * (lq0 + q) + i];
+                    brn = b.Br[4 * (ks + 1) + kk][16 * ab + i];
+                    bin = b.Bi[4 * (ks + 1) + kk][16 * ab + i];
+                }
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    accr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], br, accr[q], 0, 0, 0);
+                    acci[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], bi, acci[q], 0, 0, 0);
+                }
             }
         }
+        __syncthreads();
+    }
+    if (!on) return;
+    const int a = ac0 + 16 * ab + i;
+    if (a >= N) return;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+        for (int v = 0; v < 4; ++v) {
+            const int li = lc0 + 16 * (lq0 + q) + kk + 4 * v;   // f64 D layout: row = lane/16 + 4v
+            if (li < n_l)
+                W[((int64_t)li * NPh + k) * Np + a] = make_double2(accr[q][v], acci[q][v]);
+        }
+}
+
+__global__ __launch_bounds__(64 * kWMmaWaves + 256, 1) void k_W_f64(
+    const double* __restrict__ u, const double* __restrict__ wt,
+    const double* __restrict__ w1o, const double* __restrict__ mph,
+    const int* __restrict__ valid, const double* __restrict__ alpha,
+    const int* __restrict__ l_arr, int l_consec, double dalpha, int64_t NPh, int N, int Np,
+    int n_l, int nwa_p, double2* __restrict__ W)
+{
+    __shared__ WStage st[2];
+    const int64_t k = blockIdx.x;
+    const int f = valid[k];
+    if (!f) return;
+    const int wave = threadIdx.x >> 6;
+    const int ac0 = blockIdx.y * kWAc, lc0 = blockIdx.z * kWLc;
+    const int nlb = min(kWLc / 16, (n_l - lc0 + 15) / 16);   // active l-blocks
+    const int nst = (nwa_p + kWTk - 1) / kWTk;
+    if (wave >= kWMmaWaves) {
+        if (f & 2)
+            w_produce<true>(st, k, u, wt, w1o, mph, alpha, l_arr, l_consec, dalpha, N, n_l, nwa_p,
+                            lc0, ac0, nlb, nst);
+        else
+            w_produce<false>(st, k, u, wt, w1o, mph, alpha, l_arr, l_consec, dalpha, N, n_l,
+                             nwa_p, lc0, ac0, nlb, nst);
+        return;
+    }
+    // MFMA wave w: alpha-block w % 4, l-blocks 4 (w / 4) .. +3 of the chunk
+    const int ab = wave & 3, lq0 = 4 * (wave >> 2);
+    const int nq = max(0, min(4, nlb - lq0));
+    const bool on = ac0 + 16 * ab < N && nq > 0;
+    switch (nq) {
+    case 4: w_consume<4>(st, k, nst, ab, lq0, lc0, ac0, NPh, N, Np, n_l, on, W); break;
+    case 3: w_consume<3>(st, k, nst, ab, lq0, lc0, ac0, NPh, N, Np, n_l, on, W); break;
+    case 2: w_consume<2>(st, k, nst, ab, lq0, lc0, ac0, NPh, N, Np, n_l, on, W); break;
+    default: w_consume<1>(st, k, nst, ab, lq0, lc0, ac0, NPh, N, Np, n_l, on, W); break;
     }
 }
 
@@ -965,14 +1151,31 @@ static int W_from_orbits(klsp_ctx* ctx, int64_t nwa, const OrbitBufs& ob, const 
     int rc = alloc_W64(ctx);
     if (rc != KLSP_OK) { cleanup(); return rc; }
     hipEventRecord(ctx->ev[0], ctx->stream);
+    double lmax = 0.0, amax = 0.0;
+    for (int64_t j = 0; j < ctx->n_l; ++j) lmax = std::max(lmax, std::fabs((double)l_arr[j]));
+    for (int64_t j = 0; j < ctx->N; ++j) amax = std::max(amax, std::fabs(alpha[j]));
+    // alphas evenly spaced to within 8 ulps of the largest (the reference's
+    // range(-alpha_max, alpha_max, N)) are generated by rotation in the kernel
+    double dalpha = 0.0;
+    if (ctx->N >= 2) {
+        dalpha = (alpha[ctx->N - 1] - alpha[0]) / (double)(ctx->N - 1);
+        const double tol = 8.0 * 2.220446049250313e-16 * amax;
+        for (int64_t j = 0; j < ctx->N && dalpha != 0.0; ++j)
+            if (!(std::fabs(alpha[j] - (alpha[0] + (double)j * dalpha)) <= tol)) dalpha = 0.0;
+    }
+    amax = std::max(amax, 16.0 * std::fabs(dalpha));
     hipLaunchKernelGGL(k_orbit_prep, dim3((unsigned)NPh), dim3(256), 0, ctx->stream, ob.w1, ob.ra,
-                       ob.pha, ob.o1, ob.o2, (int)nwa, (int)nwa_p, ctx->m, d_u, d_wt, d_x1, d_mp,
-                       d_valid);
-    const unsigned lgroups = (unsigned)((ctx->n_l + 16 * 4 * kWMaxLB - 1) / (16 * 4 * kWMaxLB));
-    hipLaunchKernelGGL(k_W_f64, dim3((unsigned)NPh, (unsigned)((ctx->N + 31) / 32), lgroups),
-                       dim3(256), 0,
-                       ctx->stream, d_u, d_wt, d_x1, d_mp, d_valid, d_al, d_l, NPh, (int)ctx->N,
-                       ctx->Np, (int)ctx->n_l, (int)nwa_p, ctx->d_W64);
+                       ob.pha, ob.o1, ob.o2, (int)nwa, (int)nwa_p, ctx->m, lmax, amax, d_u, d_wt,
+                       d_x1, d_mp, d_valid);
+    int l_consec = 1;
+    for (int64_t j = 1; j < ctx->n_l; ++j)
+        if (l_arr[j] != l_arr[0] + j) l_consec = 0;
+    hipLaunchKernelGGL(k_W_f64,
+                       dim3((unsigned)NPh, (unsigned)((ctx->N + kWAc - 1) / kWAc),
+                            (unsigned)((ctx->n_l + kWLc - 1) / kWLc)),
+                       dim3(64 * kWMmaWaves + 256), 0, ctx->stream, d_u, d_wt, d_x1, d_mp, d_valid, d_al, d_l,
+                       l_consec, dalpha, NPh, (int)ctx->N, ctx->Np, (int)ctx->n_l, (int)nwa_p,
+                       ctx->d_W64);
     hipEventRecord(ctx->ev[1], ctx->stream);
     e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = hipGetLastError();

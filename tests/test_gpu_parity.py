@@ -92,6 +92,50 @@ def test_W_quadrature_small_grid(oracle_small, small_cfg_paths):
         assert relnorm(W[:, :, il], oracle_small["W"][:, :, il]) <= 1e-12, il
 
 
+@pytest.mark.parametrize("case", ["gaps_large_l", "large_alpha", "uneven_alpha"])
+def test_W_quadrature_general_l_and_large_arguments(case):
+    """compute_W with l values that are not one consecutive run (every cos
+    evaluated directly), with alphas that are not evenly spaced (every
+    exp(-i alpha ln r) evaluated directly) and with sine/cosine arguments
+    beyond 2^20 (the
+    library's sincos instead of the short reduction; both kinds of orbit in
+    one launch for large alpha).  With alpha up to 5e5 the phase alpha ln r
+    inherits alpha times the one-ulp difference between the device's and the
+    host's ln, so that case is held to 4 max|alpha| eps max|ln r| instead of
+    1e-12."""
+    p = oracle.Params(NR=9, Ne=7, Rc_min=0.05, Rc_max=20.0, v_min=0.01, N_alpha=21,
+                      alpha_max=8.0, l_min=-3, l_max=3, nw=401, nwa=41)
+    g = oracle.compute_grids(p)
+    o = oracle.compute_orbits(p, g)
+    if case == "gaps_large_l":
+        l_arr = np.array([-400000, -3, 0, 2, 5, 350001, 17], dtype=np.int32)
+        alpha = oracle.linrange(-8.0, 8.0, 21)
+    elif case == "uneven_alpha":
+        l_arr = np.arange(-3, 4, dtype=np.int32)
+        alpha = np.sort(np.random.default_rng(7).uniform(-8.0, 8.0, 37))
+    else:
+        l_arr = np.arange(-3, 4, dtype=np.int32)
+        alpha = oracle.linrange(-5e5, 5e5, 23)
+    Wo = oracle.compute_W(p, o, alpha, l_arr)
+    NPh, N, n_l = p.NPh, alpha.size, l_arr.size
+    ctx = _ffi.Context(0)
+    try:
+        ctx.upload_problem(NPh, N, n_l, p.m, 1.0, np.zeros(NPh), np.zeros((NPh, n_l)),
+                           np.zeros((NPh, n_l)), np.zeros(N))
+        # the oracle's [NPh, nwa] row-major samples are the library's [nwa, NPh]
+        W = ctx.compute_W(o["w1"], o["ra"], o["pha"], o["Om1"], o["Om2"], alpha, l_arr,
+                          download=True)
+    finally:
+        ctx.close()
+    tol = 1e-12
+    if case == "large_alpha":
+        tol = 4 * np.max(np.abs(alpha)) * np.finfo(float).eps * np.nanmax(np.abs(np.log(o["ra"])))
+    for il in range(n_l):
+        err = relnorm(W[:, :, il], Wo[:, :, il])
+        print(case, "l", l_arr[il], f"relnorm {err:.2e} (tolerance {tol:.1e})", flush=True)
+        assert err <= tol, il
+
+
 @pytest.mark.parametrize("omega", OMEGAS)
 def test_M_fp64_matches_oracle(oracle_tiny, omega):
     """build_M_matrix (MatrixBuilder.jl:27) on FP64 MFMA."""
