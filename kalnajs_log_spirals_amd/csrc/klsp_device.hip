@@ -301,23 +301,25 @@ __global__ __launch_bounds__(256) void k_orbits(
 // W_k = A_k B_k per orbit on FP64 MFMA: A[l, t] = wt_t cos(l w1_t + m phi_t)
 // (real, n_l x nwa), B[t, a] = exp(-i alpha_a ln r_t) (nwa x N).
 // Workgroup = (orbit k, 64 alphas, 128 l) of twelve waves: waves 8-11
-// generate the operands of stage s (16 samples t) into one LDS buffer while
+// generate the operands of stage s (32 samples t) into one LDS buffer while
 // waves 0-7 run the MFMAs of stage s-1 from the other; one barrier per
 // stage.  MFMA wave w owns alpha-block w % 4 (re and im) and l-blocks
 // 4 (w / 4) .. +3.  On gfx950 FP64 VALU instructions and FP64 MFMAs share
 // one pipeline per SIMD (measured: a wave of each on one SIMD takes the sum
 // of their times), so the split does not hide operand generation; the time
 // is the MFMAs plus the generation's FP64 instructions, which is why the
-// generation uses rotations and a branch-free short sincos.  Lane layouts:
+// generation uses rotations and a branch-free short sincos, and each
+// generating thread covers 16 l and 8 alphas per sincos pair.  Lane layouts:
 //   A (16 l x 4 t): lane = i + 16*kk holds A[l0+i][t0+kk]
 //   B (4 t x 16 a): lane = i + 16*kk holds B[t0+kk][a0+i]
 //   D (16 l x 16 a): lane holds D[kk + 4*v][i], v = 0..3 (the FP64 MFMA's
 //   accumulator layout; the FP32 16x16x4 MFMA uses D[4*kk + v][i])
 // When l_arr is a run of consecutive integers (the reference's l_min:l_max)
-// a thread's l = l_b + 16 j for j = 0..7 come from one sincos and seven
-// rotations by exp(16 i w1); evenly spaced alphas likewise by rotations by
-// exp(-16 i dalpha ln r); otherwise every value is evaluated directly.
-constexpr int kWTk = 16;    // samples t per LDS stage
+// a thread's l = l_b + 8 j for j = 0..15 come from one sincos and rotations
+// by exp(8 i w1); evenly spaced alphas a_b + 8 j (j = 0..7) likewise by
+// rotations by exp(-8 i dalpha ln r); otherwise every value is evaluated
+// directly.
+constexpr int kWTk = 32;    // samples t per LDS stage
 constexpr int kWLc = 128;   // l per workgroup (8 l-blocks)
 constexpr int kWAc = 64;    // alphas per workgroup (4 alpha-blocks)
 constexpr int kWMmaWaves = 8;   // MFMA waves; 4 more generate operands
@@ -334,58 +336,59 @@ __device__ __forceinline__ void w_sincos(double x, double* sn, double* cs)
     else sincos(x, sn, cs);
 }
 
-// Operand generation for one stage by the 256 threads of waves 4-7:
-// sample ft = p / 16 of the stage, column group fg = p % 16.
+// Operand generation for one stage by the 256 generating threads: sample
+// ft = p / 8 of the stage, column group fg = p % 8 (columns fg + 8 j).
 template <bool kSmall>
 __device__ __forceinline__ void w_fill(
-    WStage& st, int p, int lc0, int nlb, int n_l, const int* __restrict__ l_arr, int l_consec,
+    WStage& st, int p, int nlb, int n_l, const int* __restrict__ l_arr, int lc0, int l_consec,
     double lb, const double* al, double dalpha, double ut, double wtt, double w1t, double mp)
 {
-    const int ft = p >> 4, fg = p & 15;
+    const int ft = p >> 3, fg = p & 7;
     if (l_consec) {
         double s0, c0, sr, cr;
         w_sincos<kSmall>(lb * w1t + mp, &s0, &c0);
-        w_sincos<kSmall>(16.0 * w1t, &sr, &cr);
+        w_sincos<kSmall>(8.0 * w1t, &sr, &cr);
         s0 *= wtt;                           // the rotations carry the weight along
         c0 *= wtt;
 #pragma unroll
-        for (int j = 0; j < kWLc / 16; ++j) {
-            if (j >= nlb) break;             // rows of inactive l-blocks are never read
-            const int l = lc0 + fg + 16 * j;
-            st.A[ft][fg + 16 * j] = l < n_l ? c0 : 0.0;
+        for (int j = 0; j < kWLc / 8; ++j) {
+            // column fg + 8 j is in l-block j / 2; rows of inactive blocks are
+            // never read, rows past n_l only feed output rows that are not stored
+            if ((j >> 1) >= nlb) break;
+            st.A[ft][fg + 8 * j] = c0;
             const double c1 = c0 * cr - s0 * sr;
             s0 = s0 * cr + c0 * sr;
             c0 = c1;
         }
     } else {
 #pragma unroll
-        for (int j = 0; j < kWLc / 16; ++j) {
-            if (j >= nlb) break;
-            const int l = lc0 + fg + 16 * j;
+        for (int j = 0; j < kWLc / 8; ++j) {
+            if ((j >> 1) >= nlb) break;
+            const int l = lc0 + fg + 8 * j;
             double sl = 0.0, cl = 0.0;
             if (l < n_l) w_sincos<kSmall>((double)l_arr[l] * w1t + mp, &sl, &cl);
-            st.A[ft][fg + 16 * j] = wtt * cl;
+            st.A[ft][fg + 8 * j] = wtt * cl;
         }
     }
     if (dalpha != 0.0) {
         double sb, cb, sr, cr;
         w_sincos<kSmall>(-al[0] * ut, &sb, &cb);
-        w_sincos<kSmall>(-16.0 * dalpha * ut, &sr, &cr);
+        w_sincos<kSmall>(-8.0 * dalpha * ut, &sr, &cr);
 #pragma unroll
-        for (int j = 0; j < kWAc / 16; ++j) {
-            st.Br[ft][fg + 16 * j] = cb;
-            st.Bi[ft][fg + 16 * j] = sb;
+        for (int j = 0; j < kWAc / 8; ++j) {
+            st.Br[ft][fg + 8 * j] = cb;
+            st.Bi[ft][fg + 8 * j] = sb;
             const double c1 = cb * cr - sb * sr;
             sb = sb * cr + cb * sr;
             cb = c1;
         }
     } else {
 #pragma unroll
-        for (int j = 0; j < kWAc / 16; ++j) {
+        for (int j = 0; j < kWAc / 8; ++j) {
             double sb, cb;
             w_sincos<kSmall>(-al[j] * ut, &sb, &cb);
-            st.Br[ft][fg + 16 * j] = cb;
-            st.Bi[ft][fg + 16 * j] = sb;
+            st.Br[ft][fg + 8 * j] = cb;
+            st.Bi[ft][fg + 8 * j] = sb;
         }
     }
 }
@@ -401,12 +404,12 @@ __device__ __forceinline__ void w_produce(
     const double* __restrict__ alpha, const int* __restrict__ l_arr, int l_consec,
     double dalpha, int N, int n_l, int nwa_p, int lc0, int ac0, int nlb, int nst)
 {
-    const int p = threadIdx.x - 64 * kWMmaWaves, ft = p >> 4, fg = p & 15;
+    const int p = threadIdx.x - 64 * kWMmaWaves, ft = p >> 3, fg = p & 7;
     const double lb = lc0 + fg < n_l ? (double)l_arr[lc0 + fg] : 0.0;
-    double al[kWAc / 16];
+    double al[kWAc / 8];
 #pragma unroll
-    for (int j = 0; j < kWAc / 16; ++j) {
-        const int a = ac0 + fg + 16 * j;
+    for (int j = 0; j < kWAc / 8; ++j) {
+        const int a = ac0 + fg + 8 * j;
         al[j] = a < N ? alpha[a] : 0.0;
     }
     const double* U = u + k * nwa_p;
@@ -420,7 +423,7 @@ __device__ __forceinline__ void w_produce(
             const int tn = (s + 1) * kWTk + ft;   // next stage's sample, loaded early
             double utn = 0.0, wtn = 0.0, w1n = 0.0, mpn = 0.0;
             if (tn < nwa_p) { utn = U[tn]; wtn = WT[tn]; w1n = X1[tn]; mpn = MP[tn]; }
-            w_fill<kSmall>(st[s & 1], p, lc0, nlb, n_l, l_arr, l_consec, lb, al, dalpha, ut, wtt,
+            w_fill<kSmall>(st[s & 1], p, nlb, n_l, l_arr, lc0, l_consec, lb, al, dalpha, ut, wtt,
                            w1t, mp);
             ut = utn; wtt = wtn; w1t = w1n; mp = mpn;
         }
