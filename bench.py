@@ -61,6 +61,9 @@ def main() -> int:
     ap.add_argument("--uneven-orbits", choices=("host", "device"), default="device",
                     help="where the uneven draws' orbits are integrated")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--scan-full", action="store_true",
+                    help="also run the whole 64 x 64 omega-grid scan in BF16x3 and check it against "
+                         "FP64 dets at 64 SplitMix64-drawn grid points (about a minute)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -210,6 +213,24 @@ def main() -> int:
                       "scan_bf16x3_rel_err_vs_fp64": err})
         log(f"[bench] BF16x3 scan column: {64 / ts:.1f} det/s (64 x 64 window ~{4096 * ts / 64:.1f} s), "
             f"rel err vs FP64 at 8 points {err:.2e}")
+
+    if args.scan_full:
+        # the whole window: Re w/m in linspace(0.30, 0.60, 64), Im w in linspace(0.02, 0.25, 64)
+        from kalnajs_log_spirals_amd.prng import SplitMix64
+        t0 = time.perf_counter()
+        grid, Op0, g0 = ctx.grid_scan(64, 64, 0.30, 0.60, 0.02, 0.25, precision=_ffi.PREC_BF16X3)
+        tsf = time.perf_counter() - t0
+        Ops, gams = np.linspace(0.30, 0.60, 64), np.linspace(0.02, 0.25, 64)
+        rng = SplitMix64(0)
+        pts = [(rng.below(64), rng.below(64)) for _ in range(64)]
+        ref = ctx.det([complex(2 * Ops[j], gams[i]) for i, j in pts], _ffi.PREC_FP64)
+        med = float(np.median(np.abs(ref)))
+        errf = max(abs(grid[i, j] - ref[k]) / max(abs(ref[k]), med) for k, (i, j) in enumerate(pts))
+        extra.update({"scan_64x64_s_bf16x3": tsf, "scan_64x64_dets_per_s_bf16x3": 4096 / tsf,
+                      "scan_64x64_rel_err_vs_fp64_64pts": errf,
+                      "scan_64x64_argmin": [Op0, g0]})
+        log(f"[bench] BF16x3 64 x 64 scan: {tsf:.1f} s ({4096 / tsf:.1f} det/s), rel err vs FP64 at "
+            f"64 drawn points {errf:.2e}, argmin |det| at Omega_p={Op0:.4f} gamma={g0:.4f}")
 
     if not args.no_newton:
         t0 = time.perf_counter()
