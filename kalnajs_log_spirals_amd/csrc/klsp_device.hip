@@ -353,15 +353,14 @@ __device__ __forceinline__ void w_fill(
 #pragma unroll
         for (int j = 0; j < kWLc / 8; ++j) {
             // column fg + 8 j is in l-block j / 2; rows of inactive blocks are
-            // never read, rows past n_l only feed output rows that are not stored
-            if ((j >> 1) >= nlb) break;
+            // never read, rows past n_l only feed output rows that are not
+            // stored, so every row is written (no branch in the unrolled loop)
             st.A[ft][fg + 8 * j] = c0;
             const double c1 = c0 * cr - s0 * sr;
             s0 = s0 * cr + c0 * sr;
             c0 = c1;
         }
     } else {
-#pragma unroll
         for (int j = 0; j < kWLc / 8; ++j) {
             if ((j >> 1) >= nlb) break;
             const int l = lc0 + fg + 8 * j;
@@ -383,7 +382,6 @@ __device__ __forceinline__ void w_fill(
             cb = c1;
         }
     } else {
-#pragma unroll
         for (int j = 0; j < kWAc / 8; ++j) {
             double sb, cb;
             w_sincos<kSmall>(-al[j] * ut, &sb, &cb);
