@@ -90,12 +90,12 @@ __device__ __forceinline__ void sincos_small(double x, double* sn, double* cs)
     pc = fma(z, pc, 1.0 / 24.0);
     pc = fma(z, pc, -0.5);
     const double C = fma(z, pc, 1.0);
+    // quadrant n = q mod 4: (sin, cos) = (S, C), (C, -S), (-S, -C), (-C, S);
+    // the sign flips are xors of the high words' sign bits
     const int n = (int)q & 3;
-    double ss = (n & 1) ? C : S, cc = (n & 1) ? S : C;
-    if (n & 2) ss = -ss;
-    if ((n + 1) & 2) cc = -cc;
-    *sn = ss;
-    *cs = cc;
+    const double ss = (n & 1) ? C : S, cc = (n & 1) ? S : C;
+    *sn = __hiloint2double(__double2hiint(ss) ^ ((n & 2) << 30), __double2loint(ss));
+    *cs = __hiloint2double(__double2hiint(cc) ^ (((n + 1) & 2) << 30), __double2loint(cc));
 }
 
 // Largest |argument| given to sincos_small by the W quadrature.
