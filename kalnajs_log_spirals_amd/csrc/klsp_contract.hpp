@@ -68,8 +68,10 @@ __device__ __forceinline__ void cstep(const S2 (&A)[MB], const S2 (&X)[NB], cons
         const V c0 = (V)C[w].x, c1 = (V)C[w].y;
 #pragma unroll
         for (int n = 0; n < NB; ++n) {
-            Bv[w][n].x = c0 * X[n].x + c1 * X[n].y;
-            Bv[w][n].y = c1 * X[n].x - c0 * X[n].y;
+            // c conj(x) with one multiply and one fused multiply-add per part
+            // (VALU work shares the FP64 MFMA's pipeline)
+            Bv[w][n].x = __builtin_fma(c0, X[n].x, c1 * X[n].y);
+            Bv[w][n].y = __builtin_fma(c1, X[n].x, -(c0 * X[n].y));
         }
     }
     V nAy[MB];
