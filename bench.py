@@ -36,7 +36,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the FP64 contraction at the default grid, from two
 # rocprofv3 --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, calibrated on a
 # copy kernel of known size); profiles/det_default_grid.md holds the run.
-PROFILED_KERNEL = "klsp::k_contract<true, 4, 2, 1>"
+PROFILED_KERNEL = "klsp::k_contract<true, 4, 4, 1>"
 PROFILED_TRAFFIC_BYTES = 199.414e9 + 0.059e9
 
 

@@ -27,14 +27,17 @@ COMMON = [
 ]
 DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-# (source, is HIP device code, extra flags).  The contractions keep their
-# MFMA accumulators in VGPRs (-amdgpu-mfma-vgpr-form): with the default AGPR
-# form the compiler shuffles some of them between the two register files
-# every loop trip, and those moves take issue slots from the MFMAs.
+# (source, is HIP device code, extra flags).  The contractions with at most
+# ~8 accumulator blocks per wave keep them in VGPRs (-amdgpu-mfma-vgpr-form):
+# with the default AGPR form the compiler shuffles some of them between the
+# two register files every loop trip, and those moves take issue slots from
+# the MFMAs.  The one-omega FP64 contraction (16 blocks) fills the AGPRs and
+# keeps the default form.
 SOURCES = [
     ("klsp_host.cpp", False, []),
     ("klsp_device.hip", True, []),
-    ("klsp_contract_f64.hip", True, ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
+    ("klsp_contract_f64.hip", True, []),
+    ("klsp_contract_f64_batch.hip", True, ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
     ("klsp_contract_f32.hip", True, ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
     ("klsp_contract_bf16.hip", True, ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
 ]
