@@ -641,7 +641,7 @@ __global__ void k_reduce(const double2* __restrict__ P, int S, int Np, int N, in
 // loaded into LDS and factored there column by column by one wave (no
 // workgroup barrier per column); then every wave takes
 // kLuCols trailing columns at a time, applies the panel's row permutation (one
-// gather), solves U12 = L11^-1 A12 with lane shuffles and subtracts
+// gather), solves U12 = L11^-1 A12 with lane broadcasts and subtracts
 // L21 U12 (L21 read from LDS), so the trailing matrix in global memory
 // (L2-resident) is read and written once per panel.  Only what the
 // determinant needs is kept: the L and U12 parts are not written back.
@@ -663,10 +663,15 @@ __device__ __forceinline__ double2 cfms(double2 x, double2 a, double2 b)
     return make_double2(fma(-a.x, b.x, fma(a.y, b.y, x.x)), fma(-a.x, b.y, fma(-a.y, b.x, x.y)));
 }
 
-__device__ __forceinline__ double2 shfl2(double2 v, int src)
+// lane src's value in every lane (src wave-uniform)
+__device__ __forceinline__ double bcast(double v, int src)
 {
-    return make_double2(__shfl(v.x, src), __shfl(v.y, src));
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
+
 
 // With coef == nullptr matrix b is I - M[b]; otherwise it is
 // I - sum_j coef[b*nt + j] M[j] (a point of the Taylor model of M).
@@ -783,16 +788,25 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_det(const double2* __restrict
                 const int r = r0 + lane;
                 if (r0 + 64 < m) gather(r + 64, xn);
                 if (r0 == 0) {
-                    // U12 = L11^-1 A12 (unit lower L11): lane t holds row t
+                    // U12 = L11^-1 A12 (unit lower L11): lane t holds row t.
+                    // The L11 entries are read first and the rows broadcast
+                    // with readlane, so the dependent chain has no LDS trip.
+                    double2 lrow[kLuMaxNb];
+#pragma unroll
+                    for (int t = 0; t < kLuMaxNb; ++t)
+                        lrow[t] = (t < kb && lane > t && lane < kb) ? P[t * m + lane]
+                                                                    : make_double2(0.0, 0.0);
 #pragma unroll
                     for (int t = 0; t < kLuMaxNb; ++t) {
-                        if (t >= kb) break;
+                        if (t < kb) {
 #pragma unroll
-                        for (int q = 0; q < kLuCols; ++q) u[q][t] = shfl2(x[q], t);
-                        if (lane > t && lane < kb) {
-                            const double2 l = P[t * m + lane];
+                            for (int q = 0; q < kLuCols; ++q)
+                                u[q][t] = make_double2(bcast(x[q].x, t), bcast(x[q].y, t));
+                            if (lane > t && lane < kb) {
 #pragma unroll
-                            for (int q = 0; q < kLuCols; ++q) x[q] = cfms(x[q], l, u[q][t]);
+                                for (int q = 0; q < kLuCols; ++q)
+                                    x[q] = cfms(x[q], lrow[t], u[q][t]);
+                            }
                         }
                     }
                 }
