@@ -167,9 +167,9 @@ __global__ __launch_bounds__(256) void k_orbit_prep(
 // Circular (e < 1e-10) and epicyclic (e < 0.01) orbits are closed forms;
 // general orbits need the cumulative trapezoid sums of drs dw sv over the nw
 // phase samples (t = cumsum, :277-281; phase, :287-295).  They run as block
-// scans: each thread sums its contiguous share of the terms, the shares are
-// prefix-summed, and a second pass re-walks each share from its offset and
-// stores every stride-th sample.  The endpoint extrapolations of sv
+// scans: each thread sums its contiguous share of the terms, storing its
+// running sums at the stride-th samples, the shares are prefix-summed, and
+// each thread then adds its offset to the samples it stored.  The endpoint extrapolations of sv
 // (2 sv[1] - sv[2], ...) are taken from the recomputed neighbours.
 __device__ __forceinline__ double orbit_rv(double x, double E, double l2, bool nonradial)
 {
@@ -234,12 +234,16 @@ __global__ __launch_bounds__(256) void k_orbits(
         else v = sv2_raw(i);
         return drs * dw * v / xs(i);
     };
-    // terms j = 1..nw-1; thread t owns [j0, j1)
+    // terms j = 1..nw-1; thread t owns [j0, j1).  One pass: the running
+    // sums of the thread's own range are stored at its samples a * stride,
+    // and after the block scan the thread adds its offset to them.
     const int chunk = (nw - 1 + nth - 1) / nth;
     const int j0 = 1 + tid * chunk, j1 = min(nw, j0 + chunk);
+    const int a_lo = (j0 + stride - 1) / stride;          // first stored sample in range
     double s1 = 0.0, s2 = 0.0;
     if (j0 < j1) {
         double p1 = d1(j0 - 1), p2 = phase ? d2(j0 - 1) : 0.0;
+        int next = a_lo * stride;
         for (int j = j0; j < j1; ++j) {
             const double c1 = d1(j);
             s1 += (p1 + c1) / 2;
@@ -248,6 +252,13 @@ __global__ __launch_bounds__(256) void k_orbits(
                 const double c2v = d2(j);
                 s2 += (p2 + c2v) / 2;
                 p2 = c2v;
+            }
+            if (j == next) {
+                const int a = j / stride;
+                W1[a] = s1;
+                PH[a] = s2;
+                RA[a] = xs(j);
+                next += stride;
             }
         }
     }
@@ -277,23 +288,11 @@ __global__ __launch_bounds__(256) void k_orbits(
         Om2[k] = o1 * (phase ? Lk * tot[1] : pi / 2 * 1.0 * sg) / pi;
     }
     if (j0 < j1) {
-        double acc1 = sh1[tid], acc2 = sh2[tid];
-        double p1 = d1(j0 - 1), p2 = phase ? d2(j0 - 1) : 0.0;
-        for (int j = j0; j < j1; ++j) {
-            const double c1 = d1(j);
-            acc1 += (p1 + c1) / 2;
-            p1 = c1;
-            if (phase) {
-                const double c2v = d2(j);
-                acc2 += (p2 + c2v) / 2;
-                p2 = c2v;
-            }
-            if (j % stride == 0) {
-                const int a = j / stride;
-                W1[a] = acc1 * o1;
-                RA[a] = xs(j);
-                PH[a] = phase ? Lk * acc2 : pi / 2 * 1.0 * sg;
-            }
+        const double off1 = sh1[tid], off2 = sh2[tid];
+        for (int j = a_lo * stride; j < j1; j += stride) {
+            const int a = j / stride;
+            W1[a] = (off1 + W1[a]) * o1;
+            PH[a] = phase ? Lk * (off2 + PH[a]) : pi / 2 * 1.0 * sg;
         }
     }
 }
