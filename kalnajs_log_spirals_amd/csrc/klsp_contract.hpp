@@ -311,8 +311,10 @@ constexpr int kBf16Waves = 4, kBf16MB = 5, kBf16NB = 1;
 struct Bf16Geometry {
     int wrows, wcols, wg_per_cu;   // workgroup tiles per gang, resident workgroups per CU
 };
-Bf16Geometry contract_bf16_geometry(int Np, int nw);
-hipError_t contract_bf16_launch(int nw, const ContractArgs& a, hipStream_t stream);
+// nf: the first nf of the nw weight sets take the three-term products, the
+// others the single hi x hi product (nf = nw except in the mixed search).
+Bf16Geometry contract_bf16_geometry(int Np, int nw, int nf);
+hipError_t contract_bf16_launch(int nw, int nf, const ContractArgs& a, hipStream_t stream);
 // W complex128 rows of K (`rows` present) -> bf16 hi and lo planes of nq
 // rows of four K (2 x nq x Np x 16 bytes at Wb).
 hipError_t split_W_bf16(const double2* W64, int64_t rows, int Np, int64_t nq, void* Wb,

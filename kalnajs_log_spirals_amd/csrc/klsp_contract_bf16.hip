@@ -18,7 +18,11 @@
 // 16 K rows take three MFMAs per output part:
 //   Re += H . ReH + L . ReH + H . ReL,   Im += H . ImH + L . ImH + H . ImL.
 // Accumulation is FP32 over a gang's K range; the gang partials are summed
-// in FP64 by k_reduce.
+// in FP64 by k_reduce.  Weight sets from NF on take the single product
+// Re += H . ReH, Im += H . ImH (relative error ~2^-8 instead of ~2^-16): the
+// mixed Newton search uses this for the Taylor term T_4, which enters M
+// multiplied by s^3 and the anchored FP64 finish multiplied by
+// 3 s_a^2 (s - s_a) (|s_a| / Im w ~ 0.03).
 //
 // Work decomposition.  A workgroup is 4 waves stacked along beta (5 blocks
 // of 16 rows per wave, 320 rows) over one alpha block (16 columns).  Per
@@ -72,7 +76,7 @@ struct StageB {
 
 // One K row (k = 0..15 within the stage) x one column of B for NW weight
 // sets into LDS; x = (hi, lo) of W at that row and column.
-template <int NW>
+template <int NW, int NF>
 __device__ __forceinline__ void produce_B(StageB& sb, v2u x, const v2f (&cw)[NW], int k, int c)
 {
     const float xr = lo_f(x.x) + lo_f(x.y), xi = hi_f(x.x) + hi_f(x.y);
@@ -83,11 +87,13 @@ __device__ __forceinline__ void produce_B(StageB& sb, v2u x, const v2f (&cw)[NW]
         const float br = cr * xr + ci * xi;      // c conj(x)
         const float bi = ci * xr - cr * xi;
         const unsigned hp = pk_bf16(br, bi);
-        const unsigned lp = pk_bf16(br - lo_f(hp), bi - hi_f(hp));
         ((unsigned*)&sb.rec[w][0][g][c])[t] = hp ^ 0x80000000u;
         ((unsigned*)&sb.rec[w][1][g][c])[t] = swap16(hp);
-        ((unsigned*)&sb.rec[w][2][g][c])[t] = lp ^ 0x80000000u;
-        ((unsigned*)&sb.rec[w][3][g][c])[t] = swap16(lp);
+        if (w < NF) {
+            const unsigned lp = pk_bf16(br - lo_f(hp), bi - hi_f(hp));
+            ((unsigned*)&sb.rec[w][2][g][c])[t] = lp ^ 0x80000000u;
+            ((unsigned*)&sb.rec[w][3][g][c])[t] = swap16(lp);
+        }
     }
 }
 
@@ -117,7 +123,7 @@ __device__ __forceinline__ void load_A(const v4u* __restrict__ Wh, const v4u* __
     }
 }
 
-template <int MB, int NB, int NW>
+template <int MB, int NB, int NW, int NF>
 __device__ __forceinline__ void mfma_stage(const StageB& sb, const v4u (&A)[MB > 0 ? MB : 1][2],
                                            Acc<MB, NB, NW>& acc, int g, int i)
 {
@@ -127,11 +133,15 @@ __device__ __forceinline__ void mfma_stage(const StageB& sb, const v4u (&A)[MB >
         for (int w = 0; w < NW; ++w) {
             const int col = 16 * n + i;
             const v4u rh = sb.rec[w][0][g][col], ih = sb.rec[w][1][g][col];
-            const v4u rl = sb.rec[w][2][g][col], il = sb.rec[w][3][g][col];
 #pragma unroll
             for (int m = 0; m < MB; ++m) acc.r[w][m][n] = mfma(A[m][0], rh, acc.r[w][m][n]);
 #pragma unroll
             for (int m = 0; m < MB; ++m) acc.i[w][m][n] = mfma(A[m][0], ih, acc.i[w][m][n]);
+            if (w >= NF) {
+                __builtin_amdgcn_sched_barrier(0);
+                continue;
+            }
+            const v4u rl = sb.rec[w][2][g][col], il = sb.rec[w][3][g][col];
 #pragma unroll
             for (int m = 0; m < MB; ++m) acc.r[w][m][n] = mfma(A[m][1], rh, acc.r[w][m][n]);
 #pragma unroll
@@ -147,7 +157,7 @@ __device__ __forceinline__ void mfma_stage(const StageB& sb, const v4u (&A)[MB >
 
 // One workgroup's walk over [Kb, Ke) (multiples of 16) for a wave owning
 // MB beta blocks from b0 and the workgroup's NB alpha blocks from a0.
-template <int MB, int NB, int NW>
+template <int MB, int NB, int NW, int NF>
 __device__ __forceinline__ void wg_walk(const ContractArgs& a, StageB* sb, int64_t Kb, int64_t Ke,
                                         int b0, int a0, int gang)
 {
@@ -193,10 +203,10 @@ __device__ __forceinline__ void wg_walk(const ContractArgs& a, StageB* sb, int64
     int buf = 0;
     for (int64_t K0 = Kb; K0 < Ke; K0 += 16) {
         load_A<MB>(Wh, Wl, K0 >> 2, Np, b0, g, i, A);
-        produce_B<NW>(sb[buf], X, C, kr, c);
+        produce_B<NW, NF>(sb[buf], X, C, kr, c);
         load_B_in(K0 + 16 < Ke ? K0 + 16 : K0, Xn, Cn);
         lds_barrier();
-        mfma_stage<MB, NB, NW>(sb[buf], A, acc, g, i);
+        mfma_stage<MB, NB, NW, NF>(sb[buf], A, acc, g, i);
         X = Xn;
 #pragma unroll
         for (int w = 0; w < NW; ++w) C[w] = Cn[w];
@@ -218,7 +228,7 @@ __device__ __forceinline__ void wg_walk(const ContractArgs& a, StageB* sb, int64
 }
 
 
-template <int NW>
+template <int NW, int NF>
 __global__ __launch_bounds__(kThreads, 2) void k_contract_bf16(ContractArgs a)
 {
     __shared__ StageB sb[2];
@@ -237,12 +247,12 @@ __global__ __launch_bounds__(kThreads, 2) void k_contract_bf16(ContractArgs a)
     // every wave runs the same stages (the barriers need it); waves past the
     // last beta block only produce B
     switch (mv) {
-    case 5: wg_walk<5, kNB, NW>(a, sb, Kb, Ke, b0, a0, gang); break;
-    case 4: wg_walk<4, kNB, NW>(a, sb, Kb, Ke, b0, a0, gang); break;
-    case 3: wg_walk<3, kNB, NW>(a, sb, Kb, Ke, b0, a0, gang); break;
-    case 2: wg_walk<2, kNB, NW>(a, sb, Kb, Ke, b0, a0, gang); break;
-    case 1: wg_walk<1, kNB, NW>(a, sb, Kb, Ke, b0, a0, gang); break;
-    default: wg_walk<0, kNB, NW>(a, sb, Kb, Ke, b0, a0, gang); break;
+    case 5: wg_walk<5, kNB, NW, NF>(a, sb, Kb, Ke, b0, a0, gang); break;
+    case 4: wg_walk<4, kNB, NW, NF>(a, sb, Kb, Ke, b0, a0, gang); break;
+    case 3: wg_walk<3, kNB, NW, NF>(a, sb, Kb, Ke, b0, a0, gang); break;
+    case 2: wg_walk<2, kNB, NW, NF>(a, sb, Kb, Ke, b0, a0, gang); break;
+    case 1: wg_walk<1, kNB, NW, NF>(a, sb, Kb, Ke, b0, a0, gang); break;
+    default: wg_walk<0, kNB, NW, NF>(a, sb, Kb, Ke, b0, a0, gang); break;
     }
 }
 static_assert(kMB == 5, "the switch above enumerates the wave tile heights");
@@ -269,45 +279,48 @@ __global__ void k_W64_to_bf16(const double2* __restrict__ W, int64_t rows, int N
     }
 }
 
-template <int NW>
+template <int NW, int NF>
 const void* kernel_bf16()
 {
-    return (const void*)k_contract_bf16<NW>;
+    return (const void*)k_contract_bf16<NW, NF>;
 }
 
-const void* kernel_for(int nw)
+// nf < nw only for the four Taylor terms of the mixed search (three full)
+const void* kernel_for(int nw, int nf)
 {
-    return nw <= 1 ? kernel_bf16<1>() : nw == 2 ? kernel_bf16<2>()
-         : nw == 3 ? kernel_bf16<3>() : kernel_bf16<4>();
+    if (nw >= 4 && nf < 4) return kernel_bf16<4, 3>();
+    return nw <= 1 ? kernel_bf16<1, 1>() : nw == 2 ? kernel_bf16<2, 2>()
+         : nw == 3 ? kernel_bf16<3, 3>() : kernel_bf16<4, 4>();
 }
 
 }  // namespace
 
-Bf16Geometry contract_bf16_geometry(int Np, int nw)
+Bf16Geometry contract_bf16_geometry(int Np, int nw, int nf)
 {
-    static int occ[kMaxNW + 1] = {};
+    static int occ[kMaxNW + 1][2] = {};
     const int i = nw < 1 ? 1 : (nw > kMaxNW ? kMaxNW : nw);
-    if (occ[i] == 0) {
+    const int h = (i >= 4 && nf < 4) ? 1 : 0;
+    if (occ[i][h] == 0) {
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_for(i), kThreads, 0) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_for(i, nf), kThreads, 0) !=
                 hipSuccess || per_cu < 1)
             per_cu = 1;
-        occ[i] = per_cu;
+        occ[i][h] = per_cu;
     }
     const int nb = Np / 16;
     Bf16Geometry g;
     g.wrows = (nb + kBf16Waves * kMB - 1) / (kBf16Waves * kMB);
     g.wcols = (nb + kNB - 1) / kNB;
-    g.wg_per_cu = occ[i];
+    g.wg_per_cu = occ[i][h];
     return g;
 }
 
-hipError_t contract_bf16_launch(int nw, const ContractArgs& a, hipStream_t stream)
+hipError_t contract_bf16_launch(int nw, int nf, const ContractArgs& a, hipStream_t stream)
 {
     const unsigned grid = (unsigned)(a.n_gangs * a.wrows * a.wcols);
     ContractArgs args = a;
     void* params[] = {(void*)&args};
-    return hipLaunchKernel(kernel_for(nw), dim3(grid), dim3(kThreads), params, 0, stream);
+    return hipLaunchKernel(kernel_for(nw, nf), dim3(grid), dim3(kThreads), params, 0, stream);
 }
 
 hipError_t split_W_bf16(const double2* W64, int64_t rows, int Np, int64_t nq, void* Wb,

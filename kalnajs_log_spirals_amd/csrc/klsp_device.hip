@@ -938,9 +938,9 @@ static GangPlan plan_gangs(const klsp_ctx* ctx, const ContractShape& sh)
 // Gang geometry of the BF16 contraction: workgroup tiles of 320 beta rows x
 // 16 alpha columns; gangs per XCD chosen so that the gang workgroups fill the
 // XCD's resident workgroup slots in whole rounds where possible.
-static GangPlan plan_gangs_bf16(const klsp_ctx* ctx, int nw)
+static GangPlan plan_gangs_bf16(const klsp_ctx* ctx, int nw, int nf)
 {
-    const Bf16Geometry geo = contract_bf16_geometry(ctx->Np, nw);
+    const Bf16Geometry geo = contract_bf16_geometry(ctx->Np, nw, nf);
     GangPlan gp;
     gp.wrows = geo.wrows;
     gp.wcols = geo.wcols;
@@ -977,7 +977,8 @@ static int max_gangs(const klsp_ctx* ctx)
     int g = 8;
     for (int prec : {KLSP_PREC_FP64, KLSP_PREC_FP32})
         for (int nw = 1; nw <= kMaxBatch; ++nw) g = std::max(g, plan_gangs(ctx, shape_for(prec, nw)).n_gangs);
-    for (int nw = 1; nw <= kMaxW; ++nw) g = std::max(g, plan_gangs_bf16(ctx, nw).n_gangs);
+    for (int nw = 1; nw <= kMaxW; ++nw) g = std::max(g, plan_gangs_bf16(ctx, nw, nw).n_gangs);
+    g = std::max(g, plan_gangs_bf16(ctx, kMaxW, kMaxW - 1).n_gangs);
     return g;
 }
 
@@ -1367,11 +1368,13 @@ int klsp_prepare(klsp_ctx* ctx, int precision)
 
 }  // extern "C"
 
-// Launches the contraction for nb omegas (weights already in ctx->d_c).
-static int launch_contract(klsp_ctx* ctx, int precision, int nb)
+// Launches the contraction for nb omegas (weights already in ctx->d_c); in
+// BF16x3 the weight sets from nf on take the single hi x hi product.
+static int launch_contract(klsp_ctx* ctx, int precision, int nb, int nf)
 {
     const bool bf = precision == KLSP_PREC_BF16X3;
-    const GangPlan gp = bf ? plan_gangs_bf16(ctx, nb) : plan_gangs(ctx, shape_for(precision, nb));
+    const GangPlan gp = bf ? plan_gangs_bf16(ctx, nb, nf)
+                           : plan_gangs(ctx, shape_for(precision, nb));
     ContractArgs a;
     a.W = precision == KLSP_PREC_FP64 ? (const void*)ctx->d_W64
           : bf                        ? (const void*)ctx->d_Wb
@@ -1386,7 +1389,7 @@ static int launch_contract(klsp_ctx* ctx, int precision, int nb)
     a.wrows = gp.wrows;
     a.wcols = gp.wcols;
     const hipError_t e = precision == KLSP_PREC_FP64 ? contract_f64_launch(nb, a, ctx->stream)
-                         : bf                        ? contract_bf16_launch(nb, a, ctx->stream)
+                         : bf                        ? contract_bf16_launch(nb, nf, a, ctx->stream)
                                                      : contract_f32_launch(nb, a, ctx->stream);
     if (e != hipSuccess)
         return fail(ctx, KLSP_ERR_HIP, std::string("contraction launch: ") + hipGetErrorString(e));
@@ -1396,12 +1399,14 @@ static int launch_contract(klsp_ctx* ctx, int precision, int nb)
 
 // Builds M for nb <= kMaxBatch omegas into ctx->d_M ([nb][N,N]).  With
 // taylor set, om is one centre and the nb matrices are its Taylor terms.
+// nf < nb (BF16x3 Taylor terms only): terms nf.. take the single product.
 static void add_build_timing(klsp_ctx* ctx, int nb);
 
 // ... into matrix slots slot..slot+nb-1 of ctx->d_M.
 static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
-                       bool taylor = false, int slot = 0)
+                       bool taylor = false, int slot = 0, int nf = 0)
 {
+    if (nf <= 0 || nf > nb || precision != KLSP_PREC_BF16X3 || nb != kMaxW) nf = nb;
     const int64_t Kpc = ctx->Kp + kRowPad;
     int rc;
     if ((rc = ensure(ctx, ctx->d_c, ctx->c_cap, (size_t)kMaxW * Kpc))) return rc;
@@ -1429,7 +1434,7 @@ static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
                            ctx->stream, ctx->d_DJ, ctx->d_F0l, ctx->d_Ores, ctx->NPh, ctx->K,
                            Kpc, ctx->d_om, nb, ctx->d_c, cf);
     if (ctx->prof) hipEventRecord(ctx->ev[1], ctx->stream);
-    if ((rc = launch_contract(ctx, precision, nb))) return rc;
+    if ((rc = launch_contract(ctx, precision, nb, nf))) return rc;
     if (ctx->prof) hipEventRecord(ctx->ev[2], ctx->stream);
     hipLaunchKernelGGL(k_reduce, dim3(grid_for((int64_t)nb * ctx->N * ctx->N)), dim3(256), 0,
                        ctx->stream, ctx->d_P, ctx->last_S, ctx->Np, (int)ctx->N, nb, ctx->G,
@@ -1607,7 +1612,10 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
     for (int pass = 0; pass < np->max_iter; ++pass) {
         wc = w;
         const double om[2] = {w.real(), w.imag()};
-        if ((rc = build_batch(ctx, om, nt, lp_prec, true))) return rc;
+        // four BF16x3 terms: T_4 takes the single hi x hi product (it enters the
+        // model times s^3 and the anchored finish times 3 s_a^2 (s - s_a))
+        if ((rc = build_batch(ctx, om, nt, lp_prec, true, 0, nt == kMaxW ? kMaxW - 1 : nt)))
+            return rc;
         KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
         acc_timing(acc, ctx);
         ++n_lp;
