@@ -69,6 +69,15 @@ const char* klsp_version(void);
 /* Number of visible HIP devices (0 if none); never fails on a CPU host. */
 int klsp_device_count(int* n_out);
 
+/* Name ("<marketing name> (<gfx ISA>)", or the ISA name alone where the
+ * driver reports no marketing name; truncated to name_cap bytes with the
+ * terminating NUL), total device memory in bytes and compute-unit count of
+ * HIP device `device`; each output may be NULL.  Replaces
+ * GPUBackend.gpu_device_name (GPUBackend.jl:246) and the runner's
+ * gpu_device_name / gpu_total_memory (run_kalnajs_gpu.jl:62-79). */
+int klsp_device_info(int device, char* name_out, int name_cap, int64_t* total_mem_out,
+                     int* cu_count_out);
+
 /* Create a context on HIP device `device`.  Replaces the device side of
  * GPUBackend.gpu_device! + to_device (GPUBackend.jl:147, BasisFunctions.jl:397). */
 int klsp_create(int device, klsp_ctx** ctx_out);

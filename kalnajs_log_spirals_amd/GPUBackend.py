@@ -12,7 +12,7 @@ import enum
 from . import _ffi
 
 __all__ = ["GPUType", "detect_gpu_type", "gpu_device", "current_device", "gpu_ndevices",
-           "gpu_synchronize", "gpu_device_name", "NVIDIA", "AMD", "NONE"]
+           "gpu_synchronize", "gpu_device_name", "gpu_total_memory", "NVIDIA", "AMD", "NONE"]
 
 
 class GPUType(enum.Enum):
@@ -56,5 +56,11 @@ def gpu_synchronize() -> None:
     return None
 
 
-def gpu_device_name() -> str:
-    return _ffi.version()
+def gpu_device_name(dev_id: int = -1) -> str:
+    """GPUBackend.jl:246: name of device dev_id (-1: the current device)."""
+    return _ffi.device_info(_current if dev_id < 0 else dev_id)[0]
+
+
+def gpu_total_memory(dev_id: int = -1) -> int:
+    """Device memory in bytes (run_kalnajs_gpu.jl:73 gpu_total_memory)."""
+    return _ffi.device_info(_current if dev_id < 0 else dev_id)[1]

@@ -7,7 +7,7 @@ difference pair of each step evaluated as one two-omega batch.
 
 ``precision="mixed"`` (no counterpart in the reference) selects libklsp's
 mixed-precision search: BF16x3 passes ("mixed_fp32": FP32 passes) that each
-build the first three Taylor terms of M around a centre and run Newton on
+build the first four (FP32: three) Taylor terms of M around a centre and run Newton on
 that model, then one FP64 build that anchors the model for the FP64 finish;
 it returns the FP64 root to |d omega| <= 1e-10 with one FP64 build.
 """

@@ -37,7 +37,7 @@ _STATUS_NAMES = {
 
 # Every symbol include/klsp.h declares (checked by tests/test_abi.py).
 EXPORTED = [
-    "klsp_version", "klsp_device_count", "klsp_create", "klsp_destroy", "klsp_last_error",
+    "klsp_version", "klsp_device_count", "klsp_device_info", "klsp_create", "klsp_destroy", "klsp_last_error",
     "klsp_trapezoid_coef", "klsp_simpson_coef", "klsp_linrange", "klsp_gamma_complex",
     "klsp_compute_N_kernel", "klsp_compute_grids", "klsp_compute_orbits",
     "klsp_compute_jacobian_df", "klsp_orbit_phase_grid", "klsp_precompute_arrays",
@@ -117,6 +117,8 @@ def lib() -> ct.CDLL:
     L.klsp_version.restype = ct.c_char_p
     L.klsp_version.argtypes = []
     L.klsp_device_count.argtypes = [ct.POINTER(ct.c_int)]
+    L.klsp_device_info.argtypes = [ct.c_int, ct.c_char_p, ct.c_int, ct.POINTER(i64),
+                                   ct.POINTER(ct.c_int)]
     L.klsp_create.argtypes = [ct.c_int, ct.POINTER(P)]
     L.klsp_destroy.argtypes = [P]
     L.klsp_last_error.restype = ct.c_char_p
@@ -178,6 +180,15 @@ def device_count() -> int:
     n = ct.c_int(0)
     lib().klsp_device_count(ct.byref(n))
     return n.value
+
+
+def device_info(device: int = 0):
+    """(name, total memory in bytes, compute units) of HIP device `device`."""
+    buf = ct.create_string_buffer(256)
+    mem, cus = ct.c_int64(0), ct.c_int(0)
+    check(lib().klsp_device_info(int(device), buf, len(buf), ct.byref(mem), ct.byref(cus)),
+          what=f"device_info: no HIP device {device}")
+    return buf.value.decode(errors="replace"), int(mem.value), int(cus.value)
 
 
 def host_threads() -> int:

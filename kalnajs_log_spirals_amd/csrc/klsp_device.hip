@@ -1007,6 +1007,27 @@ int klsp_device_count(int* n_out)
     return KLSP_OK;
 }
 
+int klsp_device_info(int device, char* name_out, int name_cap, int64_t* total_mem_out,
+                     int* cu_count_out)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    if (device < 0 || device >= n) return KLSP_ERR_INVALID_ARG;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return KLSP_ERR_HIP;
+    if (name_out && name_cap > 0) {
+        // marketing name, then the ISA name (gfx950:sramecc+:xnack-); some
+        // drivers report no marketing name
+        if (prop.name[0])
+            std::snprintf(name_out, (size_t)name_cap, "%s (%s)", prop.name, prop.gcnArchName);
+        else
+            std::snprintf(name_out, (size_t)name_cap, "%s", prop.gcnArchName);
+    }
+    if (total_mem_out) *total_mem_out = (int64_t)prop.totalGlobalMem;
+    if (cu_count_out) *cu_count_out = prop.multiProcessorCount;
+    return KLSP_OK;
+}
+
 int klsp_create(int device, klsp_ctx** ctx_out)
 {
     if (!ctx_out) return KLSP_ERR_INVALID_ARG;
