@@ -20,6 +20,17 @@
 // Each gang writes one partial M for its K range; the partials are summed in
 // FP64 by k_reduce.
 //
+// Balanced launches (FP64 and FP32).  Where Np/16 is not a multiple of the
+// tile size, edge tiles hold fewer blocks than full ones, and with one wave
+// per SIMD and equal K ranges their SIMDs idle for the difference (12% at
+// 19 x 19 blocks in 4 x 4 tiles).  A work table instead gives each tile a
+// number of K pieces proportional to its block count, so that every wave
+// does the same work; the host packs the (tile, piece) units onto the 8 XCDs
+// within their resident wave slots, units of the same K range together, and
+// workgroup L reads its unit from the table (XCD = L mod 8).  Partial slice
+// `slot` of a tile holds its piece `slot`; k_reduce sums each output block
+// over its tile's piece count.
+//
 // Inner loop.  Two K-steps per trip on two register operand sets
 // (ping-pong): the loads of step k+1 are issued before the MFMAs of step k
 // (scheduling barriers pin the order), so memory latency hides behind the
@@ -117,6 +128,13 @@ __device__ __forceinline__ void cload(const S2* pa, const S2* px, const double2*
     for (int w = 0; w < NW; ++w) C[w] = pc[w * Kpc];
 }
 
+// One workgroup of a balanced launch: wave tile widx over rows [Kb, Ke) of
+// K (multiples of 8) into partial slice `slot`; widx < 0 is an idle slot.
+struct WorkUnit {
+    int64_t Kb, Ke;
+    int widx, slot;
+};
+
 // Arguments shared by every contraction launch.
 struct ContractArgs {
     const void* W;         // double2 or float2 rows of K, or paired bf16 rows
@@ -127,6 +145,8 @@ struct ContractArgs {
     int Np;
     int64_t Kg;            // K rows per gang (multiple of kF32Tile)
     int n_gangs, wrows, wcols;
+    const WorkUnit* units; // balanced launch: one unit per workgroup (else null)
+    int n_units;           // workgroups of a balanced launch
 };
 
 template <int MB, int NB, int NW>
@@ -284,14 +304,26 @@ template <bool F64, int MB, int NB, int NW>
 __global__ __launch_bounds__(64) void k_contract(ContractArgs a)
 {
     const int L = blockIdx.x;
-    const int xcd = L & 7, j = L >> 3;
-    const int gw = a.wrows * a.wcols;
-    const int gang = xcd + 8 * (j / gw), widx = j % gw;
-    if (gang >= a.n_gangs) return;
+    int gang, widx;
+    int64_t Kb, Ke;
+    if (a.units) {
+        const WorkUnit u = a.units[L];
+        if (u.widx < 0) return;
+        widx = u.widx;
+        gang = u.slot;
+        Kb = u.Kb;
+        Ke = u.Ke;
+    } else {
+        const int xcd = L & 7, j = L >> 3;
+        const int gw = a.wrows * a.wcols;
+        gang = xcd + 8 * (j / gw);
+        widx = j % gw;
+        if (gang >= a.n_gangs) return;
+        Kb = min((int64_t)gang * a.Kg, a.Kp);
+        Ke = min(a.Kp, Kb + a.Kg);
+    }
     const int b0 = (widx / a.wcols) * MB * 16, a0 = (widx % a.wcols) * NB * 16;
     const int mv = min(MB, (a.Np - b0) / 16), nv = min(NB, (a.Np - a0) / 16);
-    const int64_t Kb = min((int64_t)gang * a.Kg, a.Kp);
-    const int64_t Ke = min(a.Kp, Kb + a.Kg);
     dispatch<F64, MB, NB, NW>(mv, nv, a, Kb, Ke, b0, a0, gang);
 }
 

@@ -42,7 +42,7 @@ ContractShape contract_f64_shape(int nw)
 
 hipError_t contract_f64_launch(int nw, const ContractArgs& a, hipStream_t stream)
 {
-    const unsigned grid = (unsigned)(a.n_gangs * a.wrows * a.wcols);
+    const unsigned grid = a.units ? (unsigned)a.n_units : (unsigned)(a.n_gangs * a.wrows * a.wcols);
     ContractArgs args = a;
     void* params[] = {(void*)&args};
     int MB = 0, NB = 0;

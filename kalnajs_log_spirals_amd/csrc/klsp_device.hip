@@ -608,10 +608,13 @@ __global__ void k_weights_taylor(const double* __restrict__ DJ, const double* __
 // klsp_contract.hpp (kernels in klsp_contract_f64.hip / klsp_contract_f32.hip).
 // =====================================================================
 
-// Sum of the S K-split partials (fixed order, FP64), then *G and *N[alpha]
-// (MatrixBuilder.jl:144-149).  M is [nb][N, N] column-major.
-__global__ void k_reduce(const double2* __restrict__ P, int S, int Np, int N, int nb, double G,
-                         const double* __restrict__ Nk, double2* __restrict__ M)
+// Sum of the K-split partials (fixed order, FP64), then *G and *N[alpha]
+// (MatrixBuilder.jl:144-149).  M is [nb][N, N] column-major.  P holds S
+// slices per omega; with cnt (balanced launches) the block (beta/16,
+// alpha/16) sums its first cnt[beta/16 + (alpha/16) Np/16] slices.
+__global__ void k_reduce(const double2* __restrict__ P, int S, const int* __restrict__ cnt,
+                         int Np, int N, int nb, double G, const double* __restrict__ Nk,
+                         double2* __restrict__ M)
 {
     const int64_t NN = (int64_t)N * N, n = nb * NN;
     for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
@@ -620,8 +623,9 @@ __global__ void k_reduce(const double2* __restrict__ P, int S, int Np, int N, in
         const int64_t r = x - b * NN;
         const int beta = (int)(r % N), alpha = (int)(r / N);
         const double2* src = P + (int64_t)b * S * Np * Np + (int64_t)alpha * Np + beta;
+        const int ns = cnt ? cnt[(beta >> 4) + (alpha >> 4) * (Np >> 4)] : S;
         double sr = 0.0, si = 0.0;
-        for (int s = 0; s < S; ++s) {
+        for (int s = 0; s < ns; ++s) {
             const double2 v = src[(int64_t)s * Np * Np];
             sr += v.x;
             si += v.y;
@@ -864,6 +868,15 @@ struct klsp_ctx {
     double2* d_A = nullptr;   size_t A_cap = 0;
     double2* d_det = nullptr; size_t det_cap = 0;
     int last_S = 0;           // K-splits of the last contraction launch
+    const int* last_count = nullptr;  // per-block piece counts of a balanced launch
+    // balanced work tables of the FP64 / FP32 contractions for 1..kMaxBatch
+    // omegas, built on first use for the current problem
+    struct Balanced {
+        bool built = false;
+        int wrows = 0, wcols = 0, n_slots = 0, n_units = 0;
+        WorkUnit* d_units = nullptr;
+        int* d_count = nullptr;
+    } bal[2][kMaxBatch + 1];
     // profiling
     bool prof = false;
     hipEvent_t ev[6] = {};
@@ -908,31 +921,135 @@ static int grid_for(int64_t n, int block = 256)
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, 65536));
 }
 
-// K-split count and K per split for the contraction.
+// K-split geometry of a contraction launch.
 struct GangPlan {
     int wrows, wcols, gangs_per_xcd, n_gangs;
     int64_t Kg;
 };
 
-// Gang geometry for a contraction shape: wave tiles of MB x NB blocks cover
-// the Np/16 x Np/16 block grid; as many gangs per XCD as its resident wave
-// slots hold (four rounds of gangs, letting freed slots take up the next
-// gang's waves, measured 7% slower: the gangs lose L2 locality); K per gang a
-// multiple of kF32Tile (so also of the 8-row pair).
-static GangPlan plan_gangs(const klsp_ctx* ctx, const ContractShape& sh)
+// Balanced work table of an FP64 / FP32 contraction shape (see
+// klsp_contract.hpp).  A tile of b blocks gets ceil(b * pf / bmax) K pieces,
+// so no unit does more than bmax / pf blocks x K of work; pf is the largest
+// value whose units fit the resident wave slots of the 8 XCDs in one round
+// (with equal gangs, four rounds letting freed slots take the next gang's
+// waves measured 7% slower: the gangs lose L2 locality), with at least 32
+// loop trips (256 rows) per unit.  Units with the same piece count and index
+// cover the same K range and are packed onto one XCD where it has room
+// (largest groups first, least-loaded XCD).  With `uniform` every tile gets
+// the same number of pieces, 8 x (XCD slots / tiles): equal gangs of whole
+// tile sets, one K range each.
+static bool plan_balanced(const klsp_ctx* ctx, const ContractShape& sh, bool uniform,
+                          klsp_ctx::Balanced& out, std::vector<WorkUnit>& units,
+                          std::vector<int>& count)
 {
-    GangPlan gp;
     const int nb = ctx->Np / 16;
-    gp.wrows = (nb + sh.MB - 1) / sh.MB;
-    gp.wcols = (nb + sh.NB - 1) / sh.NB;
-    const int gw = gp.wrows * gp.wcols;
-    const int slots_per_xcd = std::max(1, ctx->n_cu / 8) * std::max(1, sh.waves_per_cu);
-    gp.gangs_per_xcd = std::max(1, slots_per_xcd / gw);
-    const int64_t max_gangs = std::max<int64_t>(1, ctx->Kp / kF32Tile);
-    while (gp.gangs_per_xcd > 1 && (int64_t)gp.gangs_per_xcd * 8 > max_gangs) --gp.gangs_per_xcd;
-    gp.n_gangs = 8 * gp.gangs_per_xcd;
-    gp.Kg = round_up((ctx->Kp + gp.n_gangs - 1) / gp.n_gangs, kF32Tile);
-    return gp;
+    const int wrows = (nb + sh.MB - 1) / sh.MB, wcols = (nb + sh.NB - 1) / sh.NB;
+    const int nt = wrows * wcols;
+    std::vector<int> blocks(nt);
+    int bmax = 1;
+    for (int t = 0; t < nt; ++t) {
+        const int r = t / wcols, c = t % wcols;
+        blocks[t] = std::min(sh.MB, nb - r * sh.MB) * std::min(sh.NB, nb - c * sh.NB);
+        bmax = std::max(bmax, blocks[t]);
+    }
+    const int slots = std::max(1, ctx->n_cu / 8) * std::max(1, sh.waves_per_cu);
+    const int64_t Kq = ctx->Kp / 8;       // K in units of the 8-row loop trip
+    std::vector<int> pieces(nt);
+    const int64_t pf_max = std::max<int64_t>(1, std::min<int64_t>(Kq / 32, 8 * slots));
+    const int64_t pu = std::max<int64_t>(1, std::min<int64_t>(Kq / 32, 8 * (slots / nt)));
+    for (int pf = (int)pf_max; pf >= 1; --pf) {
+        int64_t total = 0;
+        for (int t = 0; t < nt; ++t) {
+            pieces[t] = uniform ? (int)std::min<int64_t>(pu, pf)
+                                : (int)std::min<int64_t>(Kq, ((int64_t)blocks[t] * pf + bmax - 1) / bmax);
+            total += pieces[t];
+        }
+        if (total > 8 * (int64_t)slots) continue;
+        // groups of tiles sharing (pieces, piece index), i.e. one K range
+        std::vector<std::pair<std::pair<int, int>, std::vector<int>>> groups;
+        {
+            std::vector<int> pvals(pieces);
+            std::sort(pvals.begin(), pvals.end());
+            pvals.erase(std::unique(pvals.begin(), pvals.end()), pvals.end());
+            for (int pv : pvals)
+                for (int j = 0; j < pv; ++j) {
+                    std::vector<int> ts;
+                    for (int t = 0; t < nt; ++t)
+                        if (pieces[t] == pv) ts.push_back(t);
+                    groups.push_back({{pv, j}, ts});
+                }
+        }
+        std::stable_sort(groups.begin(), groups.end(), [](const auto& x, const auto& y) {
+            return x.second.size() > y.second.size();
+        });
+        std::vector<std::vector<WorkUnit>> bins(8);
+        for (const auto& g : groups) {
+            const int pv = g.first.first, j = g.first.second;
+            const int64_t Kb = (int64_t)j * Kq / pv * 8, Ke = (int64_t)(j + 1) * Kq / pv * 8;
+            size_t next = 0;
+            while (next < g.second.size()) {
+                // least-loaded XCD that holds the rest of the group, else the
+                // least-loaded one with any room
+                int best = -1;
+                for (int x = 0; x < 8; ++x)
+                    if ((int)(bins[x].size() + g.second.size() - next) <= slots &&
+                        (best < 0 || bins[x].size() < bins[best].size()))
+                        best = x;
+                if (best < 0)
+                    for (int x = 0; x < 8; ++x)
+                        if ((int)bins[x].size() < slots &&
+                            (best < 0 || bins[x].size() < bins[best].size()))
+                            best = x;
+                while (next < g.second.size() && (int)bins[best].size() < slots)
+                    bins[best].push_back(WorkUnit{Kb, Ke, g.second[next++], j});
+            }
+        }
+        size_t per = 0;
+        for (const auto& b : bins) per = std::max(per, b.size());
+        units.assign(8 * per, WorkUnit{0, 0, -1, 0});
+        for (int x = 0; x < 8; ++x)
+            for (size_t k = 0; k < bins[x].size(); ++k) units[8 * k + x] = bins[x][k];
+        count.assign((size_t)nb * nb, 1);
+        for (int bi = 0; bi < nb; ++bi)
+            for (int bj = 0; bj < nb; ++bj)
+                count[bi + (size_t)bj * nb] = pieces[(bi / sh.MB) * wcols + bj / sh.NB];
+        out.wrows = wrows;
+        out.wcols = wcols;
+        out.n_slots = *std::max_element(pieces.begin(), pieces.end());
+        out.n_units = (int)units.size();
+        return true;
+    }
+    return false;
+}
+
+static ContractShape shape_for(int precision, int nw);
+
+// The balanced table for (precision, nw), built and uploaded on first use.
+static int get_balanced(klsp_ctx* ctx, int precision, int nw, const klsp_ctx::Balanced** out)
+{
+    klsp_ctx::Balanced& b = ctx->bal[precision == KLSP_PREC_FP64 ? 0 : 1][nw];
+    if (!b.built) {
+        std::vector<WorkUnit> units;
+        std::vector<int> count;
+        klsp_ctx::Balanced nbal;
+        // the one-omega FP32 kernel (4 x 2 tiles) keeps equal gangs: its
+        // balanced table measured 9% slower at the default grid (43.3 vs 39.6
+        // ms; more K ranges in flight per XCD), while the FP64 kernels gain 8%
+        // (59.5 -> 54.8 ms) and the three-omega FP32 kernel 11%
+        const bool uniform = precision == KLSP_PREC_FP32 && nw == 1;
+        if (!plan_balanced(ctx, shape_for(precision, nw), uniform, nbal, units, count))
+            return fail(ctx, KLSP_ERR_INVALID_ARG, "contraction: no work plan fits the device");
+        KLSP_HIP(ctx, hipMalloc((void**)&nbal.d_units, units.size() * sizeof(WorkUnit)));
+        KLSP_HIP(ctx, hipMalloc((void**)&nbal.d_count, count.size() * sizeof(int)));
+        KLSP_HIP(ctx, hipMemcpy(nbal.d_units, units.data(), units.size() * sizeof(WorkUnit),
+                                hipMemcpyHostToDevice));
+        KLSP_HIP(ctx, hipMemcpy(nbal.d_count, count.data(), count.size() * sizeof(int),
+                                hipMemcpyHostToDevice));
+        nbal.built = true;
+        b = nbal;
+    }
+    *out = &b;
+    return KLSP_OK;
 }
 
 // Gang geometry of the BF16 contraction: workgroup tiles of 320 beta rows x
@@ -971,19 +1088,36 @@ static ContractShape shape_for(int precision, int nw)
     return precision == KLSP_PREC_FP64 ? contract_f64_shape(nw) : contract_f32_shape(nw);
 }
 
-// Largest gang count of any launch shape (sizes the partials buffer).
-static int max_gangs(const klsp_ctx* ctx)
+// Largest partial-slice count of any launch shape (sizes the partials
+// buffer); builds the balanced tables.
+static int max_gangs(klsp_ctx* ctx, int* rc)
 {
     int g = 8;
+    *rc = KLSP_OK;
     for (int prec : {KLSP_PREC_FP64, KLSP_PREC_FP32})
-        for (int nw = 1; nw <= kMaxBatch; ++nw) g = std::max(g, plan_gangs(ctx, shape_for(prec, nw)).n_gangs);
+        for (int nw = 1; nw <= kMaxBatch; ++nw) {
+            const klsp_ctx::Balanced* b = nullptr;
+            if ((*rc = get_balanced(ctx, prec, nw, &b))) return 0;
+            g = std::max(g, b->n_slots);
+        }
     for (int nw = 1; nw <= kMaxW; ++nw) g = std::max(g, plan_gangs_bf16(ctx, nw, nw).n_gangs);
     g = std::max(g, plan_gangs_bf16(ctx, kMaxW, kMaxW - 1).n_gangs);
     return g;
 }
 
+static void free_balanced(klsp_ctx* ctx)
+{
+    for (auto& row : ctx->bal)
+        for (auto& b : row) {
+            hipFree(b.d_units);
+            hipFree(b.d_count);
+            b = klsp_ctx::Balanced();
+        }
+}
+
 static int free_problem(klsp_ctx* ctx)
 {
+    free_balanced(ctx);
     hipFree(ctx->d_DJ); hipFree(ctx->d_F0l); hipFree(ctx->d_Ores); hipFree(ctx->d_Nk);
     hipFree(ctx->d_W64); hipFree(ctx->d_W32); hipFree(ctx->d_Wb);
     ctx->d_DJ = ctx->d_F0l = ctx->d_Ores = ctx->d_Nk = nullptr;
@@ -1394,9 +1528,21 @@ int klsp_prepare(klsp_ctx* ctx, int precision)
 static int launch_contract(klsp_ctx* ctx, int precision, int nb, int nf)
 {
     const bool bf = precision == KLSP_PREC_BF16X3;
-    const GangPlan gp = bf ? plan_gangs_bf16(ctx, nb, nf)
-                           : plan_gangs(ctx, shape_for(precision, nb));
+    GangPlan gp{};
+    const klsp_ctx::Balanced* bal = nullptr;
+    if (bf) {
+        gp = plan_gangs_bf16(ctx, nb, nf);
+    } else {
+        int rc = get_balanced(ctx, precision, nb, &bal);
+        if (rc) return rc;
+        gp.wrows = bal->wrows;
+        gp.wcols = bal->wcols;
+        gp.n_gangs = bal->n_slots;
+        gp.Kg = 0;
+    }
     ContractArgs a;
+    a.units = bal ? bal->d_units : nullptr;
+    a.n_units = bal ? bal->n_units : 0;
     a.W = precision == KLSP_PREC_FP64 ? (const void*)ctx->d_W64
           : bf                        ? (const void*)ctx->d_Wb
                                       : (const void*)ctx->d_W32;
@@ -1415,6 +1561,7 @@ static int launch_contract(klsp_ctx* ctx, int precision, int nb, int nf)
     if (e != hipSuccess)
         return fail(ctx, KLSP_ERR_HIP, std::string("contraction launch: ") + hipGetErrorString(e));
     ctx->last_S = gp.n_gangs;
+    ctx->last_count = bal ? bal->d_count : nullptr;
     return KLSP_OK;
 }
 
@@ -1436,7 +1583,8 @@ static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
         if ((rc = ensure(ctx, ctx->d_cf, ctx->cf_cap, (size_t)kMaxW * Kpc))) return rc;
         cf = ctx->d_cf;
     }
-    const int mg = max_gangs(ctx);
+    const int mg = max_gangs(ctx, &rc);
+    if (rc) return rc;
     if ((rc = ensure(ctx, ctx->d_P, ctx->P_cap, (size_t)kMaxW * mg * ctx->Np * ctx->Np)))
         return rc;
     if ((rc = ensure(ctx, ctx->d_M, ctx->M_cap,
@@ -1458,8 +1606,8 @@ static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
     if ((rc = launch_contract(ctx, precision, nb, nf))) return rc;
     if (ctx->prof) hipEventRecord(ctx->ev[2], ctx->stream);
     hipLaunchKernelGGL(k_reduce, dim3(grid_for((int64_t)nb * ctx->N * ctx->N)), dim3(256), 0,
-                       ctx->stream, ctx->d_P, ctx->last_S, ctx->Np, (int)ctx->N, nb, ctx->G,
-                       ctx->d_Nk, ctx->d_M + (size_t)slot * ctx->N * ctx->N);
+                       ctx->stream, ctx->d_P, ctx->last_S, ctx->last_count, ctx->Np, (int)ctx->N,
+                       nb, ctx->G, ctx->d_Nk, ctx->d_M + (size_t)slot * ctx->N * ctx->N);
     if (ctx->prof) hipEventRecord(ctx->ev[3], ctx->stream);
     KLSP_HIP(ctx, hipGetLastError());
     add_build_timing(ctx, nb);
