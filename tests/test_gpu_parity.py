@@ -188,7 +188,52 @@ def test_random_problem_ragged_sizes(NPh, N, n_l):
         assert abs(d - do) <= 1e-11 * max(abs(do), 1e-300)
 
 
-@pytest.mark.parametrize("N", [100, 300, 520, 1000])
+@pytest.mark.parametrize("N,n_l", [(301, 5), (45, 1)])
+def test_balanced_work_tables(N, n_l):
+    """The contraction's work tables: N = 301 (19 x 19 blocks: 4 x 4 tiles of
+    16, 12 and 9 blocks get 45 / 34 / 26 K pieces) and N = 45 (3 x 3 blocks,
+    one edge class); every FP64 and FP32 launch shape (1, 2, 3 weight sets)
+    against numpy ZGEMMs per l (the BLAS form), batched FP32 dets against
+    single ones."""
+    from oracle import blas_form
+    rng = np.random.default_rng(N + 7 * n_l)
+    NPh = 2501          # K / 8 >= 32 x 45: the full piece counts are reached
+    W = (rng.standard_normal((NPh, N, n_l)) + 1j * rng.standard_normal((NPh, N, n_l)))
+    W /= np.sqrt(NPh * n_l)
+    DJ, F0l = rng.uniform(0.1, 1, NPh), rng.standard_normal((NPh, n_l))
+    Ores, Nk = rng.uniform(-2, 2, (NPh, n_l)), rng.uniform(0.05, 0.3, N)
+    pre = from_arrays(2, 0.7, W, DJ, F0l, Ores, Nk)
+    oms = [0.3 + 0.4j, -1.2 + 0.05j, 0.9 + 0.2j]
+    for om in oms:
+        ref = blas_form.build_M(W, DJ, F0l, Ores, Nk, 0.7, om)
+        assert relnorm(pre.ctx.build_M(om, _ffi.PREC_FP64), ref) <= 1e-12
+        assert relnorm(pre.ctx.build_M(om, _ffi.PREC_FP32), ref) <= 2e-6
+    # FP32 dets of 2 and 3 omegas per pass (the batched kernels) vs one per pass
+    single = [pre.ctx.det([om], _ffi.PREC_FP32)[0] for om in oms]
+    for nb in (2, 3):
+        db = pre.ctx.det(oms[:nb], _ffi.PREC_FP32)
+        for i in range(nb):
+            assert abs(db[i] - single[i]) <= 1e-5 * abs(single[i]), (nb, i)
+    # Taylor terms T_j = G sum_l W_l^T diag(DJ F0l / d^j) conj(W_l) diag(N), d = w_c - Omega_res:
+    # the batched FP64 kernels (2 and 3 weight sets) and the FP32 ones
+    wc = 0.9 + 0.2j
+    ref = []
+    for j in (1, 2, 3):
+        T = np.zeros((N, N), complex)
+        for il in range(n_l):
+            c = DJ * F0l[:, il] / (wc - Ores[:, il]) ** j
+            T += W[:, :, il].T @ (c[:, None] * W[:, :, il].conj())
+        ref.append(0.7 * T * Nk[None, :])
+    for nt in (2, 3):
+        T64 = pre.ctx.build_taylor(wc, nt, _ffi.PREC_FP64)
+        T32 = pre.ctx.build_taylor(wc, nt, _ffi.PREC_FP32)
+        for j in range(nt):
+            assert relnorm(T64[j], ref[j]) <= 1e-12, (nt, j)
+            assert relnorm(T32[j], ref[j]) <= 1e-5, (nt, j)
+    pre.ctx.close()
+
+
+@pytest.mark.parametrize("N", [100, 300, 520, 1000, 1024])
 def test_lu_dense_pivoting(N):
     """Full-rank M (K = 2N) scaled so that I - M is a general dense complex
     matrix: the blocked LU pivots in most columns; det vs LAPACK on the same
