@@ -1039,14 +1039,23 @@ static int get_balanced(klsp_ctx* ctx, int precision, int nw, const klsp_ctx::Ba
         const bool uniform = precision == KLSP_PREC_FP32 && nw == 1;
         if (!plan_balanced(ctx, shape_for(precision, nw), uniform, nbal, units, count))
             return fail(ctx, KLSP_ERR_INVALID_ARG, "contraction: no work plan fits the device");
-        KLSP_HIP(ctx, hipMalloc((void**)&nbal.d_units, units.size() * sizeof(WorkUnit)));
-        KLSP_HIP(ctx, hipMalloc((void**)&nbal.d_count, count.size() * sizeof(int)));
-        KLSP_HIP(ctx, hipMemcpy(nbal.d_units, units.data(), units.size() * sizeof(WorkUnit),
+        // stored in b before the copies, so that a failed call leaves nothing
+        // unowned (a retry or free_balanced releases it)
+        hipFree(b.d_units);
+        hipFree(b.d_count);
+        b.d_units = nullptr;
+        b.d_count = nullptr;
+        KLSP_HIP(ctx, hipMalloc((void**)&b.d_units, units.size() * sizeof(WorkUnit)));
+        KLSP_HIP(ctx, hipMalloc((void**)&b.d_count, count.size() * sizeof(int)));
+        KLSP_HIP(ctx, hipMemcpy(b.d_units, units.data(), units.size() * sizeof(WorkUnit),
                                 hipMemcpyHostToDevice));
-        KLSP_HIP(ctx, hipMemcpy(nbal.d_count, count.data(), count.size() * sizeof(int),
+        KLSP_HIP(ctx, hipMemcpy(b.d_count, count.data(), count.size() * sizeof(int),
                                 hipMemcpyHostToDevice));
-        nbal.built = true;
-        b = nbal;
+        b.wrows = nbal.wrows;
+        b.wcols = nbal.wcols;
+        b.n_slots = nbal.n_slots;
+        b.n_units = nbal.n_units;
+        b.built = true;
     }
     *out = &b;
     return KLSP_OK;
