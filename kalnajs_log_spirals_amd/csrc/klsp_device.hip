@@ -611,27 +611,48 @@ __global__ void k_weights_taylor(const double* __restrict__ DJ, const double* __
 // Sum of the K-split partials (fixed order, FP64), then *G and *N[alpha]
 // (MatrixBuilder.jl:144-149).  M is [nb][N, N] column-major.  P holds S
 // slices per omega; with cnt (balanced launches) the block (beta/16,
-// alpha/16) sums its first cnt[beta/16 + (alpha/16) Np/16] slices.
-__global__ void k_reduce(const double2* __restrict__ P, int S, const int* __restrict__ cnt,
-                         int Np, int N, int nb, double G, const double* __restrict__ Nk,
-                         double2* __restrict__ M)
+// alpha/16) sums its first cnt[beta/16 + (alpha/16) Np/16] slices.  A
+// workgroup takes kRedElems consecutive outputs; its kRedParts waves sum the
+// slices s = part (mod kRedParts) of each, and the parts are added in order.
+constexpr int kRedElems = 64, kRedParts = 4;
+
+__global__ __launch_bounds__(kRedElems * kRedParts) void k_reduce(
+    const double2* __restrict__ P, int S, const int* __restrict__ cnt, int Np, int N, int nb,
+    double G, const double* __restrict__ Nk, double2* __restrict__ M)
 {
+    __shared__ double2 part[kRedParts][kRedElems];
     const int64_t NN = (int64_t)N * N, n = nb * NN;
-    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
-         x += (int64_t)gridDim.x * blockDim.x) {
-        const int b = (int)(x / NN);
-        const int64_t r = x - b * NN;
-        const int beta = (int)(r % N), alpha = (int)(r / N);
-        const double2* src = P + (int64_t)b * S * Np * Np + (int64_t)alpha * Np + beta;
-        const int ns = cnt ? cnt[(beta >> 4) + (alpha >> 4) * (Np >> 4)] : S;
+    const int e = threadIdx.x % kRedElems, j = threadIdx.x / kRedElems;
+    for (int64_t x0 = (int64_t)blockIdx.x * kRedElems; x0 < n; x0 += (int64_t)gridDim.x * kRedElems) {
+        const int64_t x = x0 + e;
         double sr = 0.0, si = 0.0;
-        for (int s = 0; s < ns; ++s) {
-            const double2 v = src[(int64_t)s * Np * Np];
-            sr += v.x;
-            si += v.y;
+        int beta = 0, alpha = 0;
+        if (x < n) {
+            const int b = (int)(x / NN);
+            const int64_t r = x - b * NN;
+            beta = (int)(r % N);
+            alpha = (int)(r / N);
+            const double2* src = P + (int64_t)b * S * Np * Np + (int64_t)alpha * Np + beta;
+            const int ns = cnt ? cnt[(beta >> 4) + (alpha >> 4) * (Np >> 4)] : S;
+            for (int s = j; s < ns; s += kRedParts) {
+                const double2 v = src[(int64_t)s * Np * Np];
+                sr += v.x;
+                si += v.y;
+            }
         }
-        const double g = Nk[alpha];
-        M[x] = make_double2(sr * G * g, si * G * g);
+        part[j][e] = make_double2(sr, si);
+        __syncthreads();
+        if (j == 0 && x < n) {
+            double tr = 0.0, ti = 0.0;
+#pragma unroll
+            for (int q = 0; q < kRedParts; ++q) {
+                tr += part[q][e].x;
+                ti += part[q][e].y;
+            }
+            const double g = Nk[alpha];
+            M[x] = make_double2(tr * G * g, ti * G * g);
+        }
+        __syncthreads();
     }
 }
 
@@ -1614,7 +1635,8 @@ static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
     if (ctx->prof) hipEventRecord(ctx->ev[1], ctx->stream);
     if ((rc = launch_contract(ctx, precision, nb, nf))) return rc;
     if (ctx->prof) hipEventRecord(ctx->ev[2], ctx->stream);
-    hipLaunchKernelGGL(k_reduce, dim3(grid_for((int64_t)nb * ctx->N * ctx->N)), dim3(256), 0,
+    hipLaunchKernelGGL(k_reduce, dim3(grid_for((int64_t)nb * ctx->N * ctx->N, kRedElems)),
+                       dim3(kRedElems * kRedParts), 0,
                        ctx->stream, ctx->d_P, ctx->last_S, ctx->last_count, ctx->Np, (int)ctx->N,
                        nb, ctx->G, ctx->d_Nk, ctx->d_M + (size_t)slot * ctx->N * ctx->N);
     if (ctx->prof) hipEventRecord(ctx->ev[3], ctx->stream);
