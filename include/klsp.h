@@ -279,16 +279,19 @@ typedef struct klsp_newton_result {
  *   1. low-precision passes: at a centre w_c one BF16x3 (MIXED_FP32: FP32)
  *      contraction builds the Taylor terms T_j = G N sum W c/d^j conj W
  *      (d = w_c - Omega_res, j = 1..4 for BF16x3, 1..3 for FP32), so
- *      M(w_c + s) = T_1 - s T_2 + s^2 T_3 - ...; Newton on det(I - M) of that
- *      model (FP64 LUs only) within |s| <= Im(w_c)/4 moves the centre; passes
- *      repeat until the move's predicted model error is small enough for
- *      the finish (one pass from a start 4e-3 from the root);
+ *      M(w_c + s) = T_1 - s T_2 + s^2 T_3 - ...; the zero of det(I - M) of
+ *      that model within |s| <= Im(w_c)/4 (FP64 LUs only: its values at 16
+ *      points on that circle, from one launch, give its Taylor coefficients;
+ *      model Newton steps where that fails) moves the centre; passes repeat
+ *      until the move's predicted model error is small enough for the finish
+ *      (one pass from a start 4e-3 from the root);
  *   2. FP64 finish: one FP64 build at the low-precision root w_a anchors
  *      the model, M(w) = M64(w_a) - (s - s_a) T_2 + (s - s_a)(s + s_a) T_3
  *      (s = w - w_c, s_a = w_a - w_c), in which the low-precision terms
  *      enter only multiplied by |s - s_a| <= 1e-6, so M near the root is
- *      FP64-accurate; Newton on it (FP64 LUs) runs until the step stops
- *      shrinking; converged if then |det| < tol.  A new anchor is built if
+ *      FP64-accurate; its zero near s_a (the same 16-point circle, radius
+ *      anchor_tol below) is confirmed by Newton steps on it (FP64 LUs) until
+ *      the step stops shrinking; converged if then |det| < tol.  A new anchor is built if
  *      the root moves so far from the last one that the low-precision
  *      terms' error (their model's slope error + ~1e-4 BF16x3 / 1e-5 FP32)
  *      times that move could exceed 5e-11.

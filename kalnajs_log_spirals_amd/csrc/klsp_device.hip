@@ -862,6 +862,8 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_det(const double2* __restrict
 // =====================================================================
 // Host side: context, buffers, drivers
 // =====================================================================
+constexpr int kCircle = 16;               // circle points of a model-det zero search
+constexpr int kModelPts = kCircle + 1;    // with the centre: matrices per model LU launch
 struct klsp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -881,7 +883,7 @@ struct klsp_ctx {
     bool have_W = false;
     // workspaces
     double* d_om = nullptr;
-    double2* d_coef = nullptr;  // [kMaxBatch][kMaxBatch] Taylor-model coefficients
+    double2* d_coef = nullptr;  // [kModelPts][kMaxW] Taylor-model coefficients
     double2* d_c = nullptr;   size_t c_cap = 0;
     float2* d_cf = nullptr;   size_t cf_cap = 0;   // FP32 copy of the weights
     double2* d_P = nullptr;   size_t P_cap = 0;
@@ -1211,7 +1213,7 @@ int klsp_create(int device, klsp_ctx** ctx_out)
     }
     for (auto& e : ctx->ev) hipEventCreate(&e);
     if (hipMalloc((void**)&ctx->d_om, sizeof(double) * 2 * kMaxBatch) != hipSuccess ||
-        hipMalloc((void**)&ctx->d_coef, sizeof(double2) * 3 * kMaxW) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_coef, sizeof(double2) * kModelPts * kMaxW) != hipSuccess ||
         hipMalloc((void**)&ctx->d_det, sizeof(double2) * kMaxBatch) != hipSuccess) {
         ctx->det_cap = 0;
         klsp_destroy(ctx);
@@ -1752,8 +1754,12 @@ static int model_dets(klsp_ctx* ctx, int nt, const std::complex<double>* cf, int
                       std::complex<double>* dets)
 {
     int rc;
-    if ((rc = ensure(ctx, ctx->d_A, ctx->A_cap, (size_t)kMaxBatch * ctx->N * ctx->N))) return rc;
-    double2 coef[3 * kMaxW];
+    if (nd > kModelPts || nt > kMaxW) return fail(ctx, KLSP_ERR_INVALID_ARG, "model_dets: too many points");
+    if ((rc = ensure(ctx, ctx->d_A, ctx->A_cap,
+                     (size_t)std::max(kMaxBatch, nd) * ctx->N * ctx->N)))
+        return rc;
+    if ((rc = ensure(ctx, ctx->d_det, ctx->det_cap, (size_t)std::max(kMaxBatch, nd)))) return rc;
+    double2 coef[kModelPts * kMaxW];
     for (int i = 0; i < nd * nt; ++i) coef[i] = make_double2(cf[i].real(), cf[i].imag());
     KLSP_HIP(ctx, hipMemcpyAsync(ctx->d_coef, coef, sizeof(double2) * nd * nt,
                                  hipMemcpyHostToDevice, ctx->stream));
@@ -1762,6 +1768,63 @@ static int model_dets(klsp_ctx* ctx, int nt, const std::complex<double>* cf, int
                                  ctx->stream));
     KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     add_lu_timing(ctx);
+    return KLSP_OK;
+}
+
+// Zero of a model det by one LU launch: its values at the kCircle points
+// s0 + r e^(2 pi i k / kCircle) and at s0 give, by a discrete Fourier
+// transform, its Taylor coefficients around s0,
+//   a_n = (1 / kCircle) sum_k f_k e^(-2 pi i n k / kCircle) / r^n,
+// exact up to the aliased terms ~(r / R)^kCircle (R >= Im w, the distance to
+// the det's nearest singularity); Newton on that polynomial from -a_0 / a_1
+// gives the zero t nearest s0 (t relative to s0).  coef_at(s, cf) fills the
+// nt model coefficients of the point s.  ok is false where the polynomial is
+// not to be trusted: a_0 differs from the centre value by more than 1e-8 of
+// the circle's values (aliasing), or no zero within the circle was found.
+template <typename CoefAt>
+static int circle_zero(klsp_ctx* ctx, int nt, CoefAt coef_at, std::complex<double> s0, double r,
+                       std::complex<double>* t_out, std::complex<double>* f0_out, bool* ok)
+{
+    typedef std::complex<double> cd;
+    const int P = kCircle;
+    cd cf[kModelPts * kMaxW], f[kModelPts];
+    cd e[kCircle];
+    const double two_pi = 6.283185307179586;
+    for (int k = 0; k < P; ++k) e[k] = std::polar(1.0, two_pi * k / P);
+    for (int k = 0; k < P; ++k) coef_at(s0 + r * e[k], cf + k * nt);
+    coef_at(s0, cf + P * nt);
+    int rc = model_dets(ctx, nt, cf, P + 1, f);
+    if (rc) return rc;
+    *f0_out = f[P];
+    *ok = false;
+    double fmax = 0.0;
+    for (int k = 0; k < P; ++k) fmax = std::max(fmax, std::abs(f[k]));
+    cd a[kCircle];
+    for (int n = 0; n < P; ++n) {
+        cd sum = 0.0;
+        for (int k = 0; k < P; ++k) sum += f[k] * std::conj(e[(n * k) % P]);
+        a[n] = sum / (double)P / std::pow(r, n);
+    }
+    if (!(fmax > 0.0) || !std::isfinite(fmax) || std::abs(a[0] - f[P]) > 1e-8 * fmax ||
+        !(std::abs(a[1]) > 0.0))
+        return KLSP_OK;
+    cd t = -a[0] / a[1];
+    for (int it = 0; it < 60; ++it) {
+        cd p = a[P - 1], dp = 0.0;
+        for (int n = P - 2; n >= 0; --n) {
+            dp = dp * t + p;
+            p = p * t + a[n];
+        }
+        if (!(std::abs(dp) > 0.0)) return KLSP_OK;
+        const cd dt = p / dp;
+        t -= dt;
+        if (!(std::abs(t) <= 2.0 * r)) return KLSP_OK;
+        if (std::abs(dt) <= 1e-15 * r) break;
+    }
+    if (std::abs(t) <= r) {
+        *t_out = t;
+        *ok = true;
+    }
     return KLSP_OK;
 }
 
@@ -1822,7 +1885,20 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
         nt_last = nt;
         const double rho = 0.25 * w.imag();
         cd s = 0.0, D0 = 0.0;
-        for (int it = 0; it < 12; ++it) {
+        auto model_at = [&](cd si, cd* c) {   // M(w_c + s) = sum_j (-s)^j T_{j+1}
+            cd p = 1.0;
+            for (int j = 0; j < nt; ++j, p *= -si) c[j] = p;
+        };
+        // the model's zero within rho from one launch of kCircle + 1 LUs;
+        // model Newton steps (three LUs a launch) where that fails
+        bool found = false;
+        {
+            cd t;
+            if ((rc = circle_zero(ctx, nt, model_at, 0.0, rho, &t, &D0, &found))) return rc;
+            acc_timing(acc, ctx);
+            if (found) s = t;
+        }
+        for (int it = 0; it < 12 && !found; ++it) {
             const cd pts[3] = {s, s + h, s - h};
             cd cf[3 * kMaxW], d[3];
             for (int i = 0; i < 3; ++i) {     // M(w_c + s) = sum_j (-s)^j T_{j+1}
@@ -1831,7 +1907,6 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
             }
             if ((rc = model_dets(ctx, nt, cf, 3, d))) return rc;
             acc_timing(acc, ctx);
-            if (it == 0) D0 = d[0];
             const cd Dp = (d[1] - d[2]) / (2.0 * h);
             if (!(std::abs(Dp) > 0.0) || !std::isfinite(std::abs(Dp))) break;
             cd sn = s - d[0] / Dp;
@@ -1884,24 +1959,35 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
         cd s = sa;
         double prev_step = INFINITY;
         bool far = false;
+        auto anchored_at = [&](cd si, cd* c) {
+            // (-s)^j - (-s_a)^j = (-1)^j (s - s_a) sum_{q<j} s^q s_a^(j-1-q), free
+            // of the cancellation of the difference of powers
+            const cd ds = si - sa;
+            c[0] = 1.0;
+            for (int j = 1; j < nt_last; ++j) {
+                cd sum = 0.0, sp = 1.0;
+                for (int q = 0; q < j; ++q, sp *= si) sum += sp * std::pow(sa, j - 1 - q);
+                c[j] = ((j & 1) ? -1.0 : 1.0) * ds * sum;
+            }
+        };
+        // the anchored model's zero within anchor_tol of s_a from one launch;
+        // the Newton steps below then confirm it (normally in one launch)
+        {
+            cd t, D_a;
+            bool found = false;
+            if ((rc = circle_zero(ctx, nt_last, anchored_at, sa, anchor_tol, &t, &D_a, &found)))
+                return rc;
+            acc_timing(acc, ctx);
+            record(w, std::abs(D_a));
+            if (found) s = sa + t;
+        }
         for (int it = 0; it < 30; ++it) {
             const cd pts[3] = {s, s + h64, s - h64};
             cd cf[3 * kMaxW], d[3];
-            for (int i = 0; i < 3; ++i) {
-                // (-s)^j - (-s_a)^j = (-1)^j (s - s_a) sum_{q<j} s^q s_a^(j-1-q), free
-                // of the cancellation of the difference of powers
-                const cd si = pts[i], ds = si - sa;
-                cf[nt_last * i] = 1.0;
-                for (int j = 1; j < nt_last; ++j) {
-                    cd sum = 0.0, sp = 1.0;
-                    for (int q = 0; q < j; ++q, sp *= si) sum += sp * std::pow(sa, j - 1 - q);
-                    cf[nt_last * i + j] = ((j & 1) ? -1.0 : 1.0) * ds * sum;
-                }
-            }
+            for (int i = 0; i < 3; ++i) anchored_at(pts[i], cf + nt_last * i);
             if ((rc = model_dets(ctx, nt_last, cf, 3, d))) return rc;
             acc_timing(acc, ctx);
             D = d[0];
-            if (it == 0) record(w, std::abs(D));
             const cd Dp = (d[1] - d[2]) / (2.0 * h64);
             if (!(std::abs(Dp) > 0.0) || !std::isfinite(std::abs(Dp))) break;
             const cd wn = bounded(wc + s - D / Dp);
