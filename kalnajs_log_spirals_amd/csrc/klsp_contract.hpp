@@ -129,7 +129,9 @@ __device__ __forceinline__ void cload(const S2* pa, const S2* px, const double2*
 }
 
 // One workgroup of a balanced launch: wave tile widx over rows [Kb, Ke) of
-// K (multiples of 8) into partial slice `slot`; widx < 0 is an idle slot.
+// K into partial slice `slot`; widx < 0 is an idle slot.  Kb and Ke are
+// multiples of the kernel's rows per loop trip (kRowsPerTrip), except Ke = Kp.
+template <bool F64, int NW> constexpr int kRowsPerTrip = (F64 && NW == 1) ? 16 : 8;
 struct WorkUnit {
     int64_t Kb, Ke;
     int widx, slot;
@@ -171,9 +173,48 @@ __device__ __forceinline__ void wave_f64(const ContractArgs& a, int64_t Kb, int6
     const double2* pa = W + (Kb + kk) * Np + b0 + i;
     const double2* px = W + (Kb + kk) * Np + a0 + i;
     const double2* pc = a.cw + Kb + kk;
+    const MfmaF64 mf;
+    if constexpr (NW == 1) {
+        // One omega (accumulators in AGPRs, VGPRs to spare): four operand
+        // sets, loads three K-steps ahead of the MFMAs (with one wave per
+        // SIMD nothing else hides their latency), four steps (16 rows) a
+        // trip.  [Kb, Ke) is a multiple of 16 rows except where Ke = Kp (a
+        // multiple of 8); there the last trip runs past Ke by at most 8 rows
+        // and its loads by at most 19, inside the kRowPad zero rows.
+        double2 A0[MB], X0[NB], C0[1], A1[MB], X1[NB], C1[1];
+        double2 A2[MB], X2[NB], C2[1], A3[MB], X3[NB], C3[1];
+        cload<MB, NB, 1>(pa, px, pc, Kpc, A0, X0, C0);
+        __builtin_amdgcn_sched_barrier(0);
+        cload<MB, NB, 1>(pa + rstep, px + rstep, pc + kKStep, Kpc, A1, X1, C1);
+        __builtin_amdgcn_sched_barrier(0);
+        cload<MB, NB, 1>(pa + 2 * rstep, px + 2 * rstep, pc + 2 * kKStep, Kpc, A2, X2, C2);
+        __builtin_amdgcn_sched_barrier(0);
+        // a 32-bit trip count keeps the loop test in scalar registers
+        const int ntrip = (int)((Ke - Kb + 4 * kKStep - 1) / (4 * kKStep));
+        for (int trip = 0; trip < ntrip; ++trip) {
+            cload<MB, NB, 1>(pa + 3 * rstep, px + 3 * rstep, pc + 3 * kKStep, Kpc, A3, X3, C3);
+            __builtin_amdgcn_sched_barrier(0);
+            cstep<MB, NB, 1, double>(A0, X0, C0, cr, ci, mf);
+            __builtin_amdgcn_sched_barrier(0);
+            cload<MB, NB, 1>(pa + 4 * rstep, px + 4 * rstep, pc + 4 * kKStep, Kpc, A0, X0, C0);
+            __builtin_amdgcn_sched_barrier(0);
+            cstep<MB, NB, 1, double>(A1, X1, C1, cr, ci, mf);
+            __builtin_amdgcn_sched_barrier(0);
+            cload<MB, NB, 1>(pa + 5 * rstep, px + 5 * rstep, pc + 5 * kKStep, Kpc, A1, X1, C1);
+            __builtin_amdgcn_sched_barrier(0);
+            cstep<MB, NB, 1, double>(A2, X2, C2, cr, ci, mf);
+            __builtin_amdgcn_sched_barrier(0);
+            cload<MB, NB, 1>(pa + 6 * rstep, px + 6 * rstep, pc + 6 * kKStep, Kpc, A2, X2, C2);
+            __builtin_amdgcn_sched_barrier(0);
+            cstep<MB, NB, 1, double>(A3, X3, C3, cr, ci, mf);
+            __builtin_amdgcn_sched_barrier(0);
+            pa += 4 * rstep;
+            px += 4 * rstep;
+            pc += 4 * kKStep;
+        }
+    } else {
     double2 A0[MB], X0[NB], C0[NW], A1[MB], X1[NB], C1[NW];
     cload<MB, NB, NW>(pa, px, pc, Kpc, A0, X0, C0);
-    const MfmaF64 mf;
     for (int64_t K0 = Kb; K0 < Ke; K0 += 2 * kKStep) {
         cload<MB, NB, NW>(pa + rstep, px + rstep, pc + kKStep, Kpc, A1, X1, C1);
         __builtin_amdgcn_sched_barrier(0);
@@ -186,6 +227,7 @@ __device__ __forceinline__ void wave_f64(const ContractArgs& a, int64_t Kb, int6
         __builtin_amdgcn_sched_barrier(0);
         cstep<MB, NB, NW, double>(A1, X1, C1, cr, ci, mf);
         __builtin_amdgcn_sched_barrier(0);
+    }
     }
 #pragma unroll
     for (int w = 0; w < NW; ++w) {

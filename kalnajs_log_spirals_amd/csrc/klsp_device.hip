@@ -955,13 +955,14 @@ struct GangPlan {
 // so no unit does more than bmax / pf blocks x K of work; pf is the largest
 // value whose units fit the resident wave slots of the 8 XCDs in one round
 // (with equal gangs, four rounds letting freed slots take the next gang's
-// waves measured 7% slower: the gangs lose L2 locality), with at least 32
-// loop trips (256 rows) per unit.  Units with the same piece count and index
+// waves measured 7% slower: the gangs lose L2 locality), with at least 256
+// rows per unit; piece boundaries are multiples of the kernel's rows per
+// loop trip (`quantum`), the last piece ending at Kp.  Units with the same piece count and index
 // cover the same K range and are packed onto one XCD where it has room
 // (largest groups first, least-loaded XCD).  With `uniform` every tile gets
 // the same number of pieces, 8 x (XCD slots / tiles): equal gangs of whole
 // tile sets, one K range each.
-static bool plan_balanced(const klsp_ctx* ctx, const ContractShape& sh, bool uniform,
+static bool plan_balanced(const klsp_ctx* ctx, const ContractShape& sh, bool uniform, int quantum,
                           klsp_ctx::Balanced& out, std::vector<WorkUnit>& units,
                           std::vector<int>& count)
 {
@@ -976,15 +977,16 @@ static bool plan_balanced(const klsp_ctx* ctx, const ContractShape& sh, bool uni
         bmax = std::max(bmax, blocks[t]);
     }
     const int slots = std::max(1, ctx->n_cu / 8) * std::max(1, sh.waves_per_cu);
-    const int64_t Kq = ctx->Kp / 8;       // K in units of the 8-row loop trip
+    const int64_t Kq = ctx->Kp / quantum;  // K in units of the loop trip (the last piece ends at Kp)
     std::vector<int> pieces(nt);
-    const int64_t pf_max = std::max<int64_t>(1, std::min<int64_t>(Kq / 32, 8 * slots));
-    const int64_t pu = std::max<int64_t>(1, std::min<int64_t>(Kq / 32, 8 * (slots / nt)));
+    const int64_t pf_max = std::max<int64_t>(1, std::min<int64_t>(ctx->Kp / 256, 8 * slots));
+    const int64_t pu = std::max<int64_t>(1, std::min<int64_t>(ctx->Kp / 256, 8 * (slots / nt)));
     for (int pf = (int)pf_max; pf >= 1; --pf) {
         int64_t total = 0;
         for (int t = 0; t < nt; ++t) {
             pieces[t] = uniform ? (int)std::min<int64_t>(pu, pf)
-                                : (int)std::min<int64_t>(Kq, ((int64_t)blocks[t] * pf + bmax - 1) / bmax);
+                                : (int)std::min<int64_t>(std::max<int64_t>(1, Kq),
+                                                         ((int64_t)blocks[t] * pf + bmax - 1) / bmax);
             total += pieces[t];
         }
         if (total > 8 * (int64_t)slots) continue;
@@ -1008,7 +1010,8 @@ static bool plan_balanced(const klsp_ctx* ctx, const ContractShape& sh, bool uni
         std::vector<std::vector<WorkUnit>> bins(8);
         for (const auto& g : groups) {
             const int pv = g.first.first, j = g.first.second;
-            const int64_t Kb = (int64_t)j * Kq / pv * 8, Ke = (int64_t)(j + 1) * Kq / pv * 8;
+            const int64_t Kb = (int64_t)j * Kq / pv * quantum;
+            const int64_t Ke = j + 1 == pv ? ctx->Kp : (int64_t)(j + 1) * Kq / pv * quantum;
             size_t next = 0;
             while (next < g.second.size()) {
                 // least-loaded XCD that holds the rest of the group, else the
@@ -1060,7 +1063,10 @@ static int get_balanced(klsp_ctx* ctx, int precision, int nw, const klsp_ctx::Ba
         // ms; more K ranges in flight per XCD), while the FP64 kernels gain 8%
         // (59.5 -> 54.8 ms) and the three-omega FP32 kernel 11%
         const bool uniform = precision == KLSP_PREC_FP32 && nw == 1;
-        if (!plan_balanced(ctx, shape_for(precision, nw), uniform, nbal, units, count))
+        const int quantum = precision == KLSP_PREC_FP64 ? (nw == 1 ? kRowsPerTrip<true, 1>
+                                                                   : kRowsPerTrip<true, 2>)
+                                                        : kRowsPerTrip<false, 1>;
+        if (!plan_balanced(ctx, shape_for(precision, nw), uniform, quantum, nbal, units, count))
             return fail(ctx, KLSP_ERR_INVALID_ARG, "contraction: no work plan fits the device");
         // stored in b before the copies, so that a failed call leaves nothing
         // unowned (a retry or free_balanced releases it)
