@@ -67,7 +67,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the omega-sharded multi-GPU form is described in DESIGN.md but not built",
+              "the omega-sharded and K-sharded multi-GPU forms are not built",
               file=sys.stderr)
         return 2
 
