@@ -282,9 +282,10 @@ typedef struct klsp_newton_result {
  *      M(w_c + s) = T_1 - s T_2 + s^2 T_3 - ...; the zero of det(I - M) of
  *      that model within |s| <= Im(w_c)/4 (FP64 LUs only: its values at 16
  *      points on that circle, from one launch, give its Taylor coefficients;
- *      model Newton steps where that fails) moves the centre; passes repeat
- *      until the move's predicted model error is small enough for the finish
- *      (one pass from a start 4e-3 from the root);
+ *      model Newton steps where that fails, moving at most Im(w_c)/2) moves
+ *      the centre; passes repeat until the move's predicted model error is
+ *      small enough for the finish (one pass from a start 4e-3 from the
+ *      root);
  *   2. FP64 finish: one FP64 build at the low-precision root w_a anchors
  *      the model, M(w) = M64(w_a) - (s - s_a) T_2 + (s - s_a)(s + s_a) T_3
  *      (s = w - w_c, s_a = w_a - w_c), in which the low-precision terms
@@ -294,7 +295,8 @@ typedef struct klsp_newton_result {
  *      the step stops shrinking; converged if then |det| < tol.  A new anchor is built if
  *      the root moves so far from the last one that the low-precision
  *      terms' error (their model's slope error + ~1e-4 BF16x3 / 1e-5 FP32)
- *      times that move could exceed 5e-11.
+ *      times that move could exceed 5e-11, at the anchored model's own zero
+ *      (converged on first).
  * iterations = low-precision passes + anchors + 1; history rows record each
  * centre with its low-precision |det|, each anchor with its FP64 |det|, and
  * the final point.  Bounds (gamma >= 1e-4, Omega_p in [0, 1]) apply to

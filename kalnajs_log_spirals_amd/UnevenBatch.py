@@ -163,7 +163,8 @@ def run_batch(draws: List[Draw], precision: str = "mixed", scale: float = 1.0,
                     f"Ne={cfgs[i].grid.Ne} N={cfgs[i].grid.N_alpha} l={cfgs[i].grid.l_max}: "
                     f"{'converged' if res.converged else 'not converged'} "
                     f"Op={res.Omega_p:.13f} g={res.gamma:.13f} host {t_host:.1f}s "
-                    f"W {t_W:.2f}s newton {t_n:.2f}s"
+                    f"W {t_W:.2f}s newton {t_n:.2f}s ({res.n_lp_builds} lp + "
+                    f"{res.n_fp64_builds} fp64 builds)"
                     + (f" polish move {res.polish_move:.2e}" if polish > 0 else ""))
     return results
 
