@@ -8,8 +8,8 @@ det(I - M(omega)) evaluation: resonance weights, the K-long complex
 contraction (FP64 MFMA), the G*N epilogue and the FP64 complex LU, with W
 resident in HBM.  Also reported: Newton roots/s (FP64, from
 omega0 = 0.8794 + 0.1231i), the complex64/FP32-MFMA det rate, the W
-quadrature time, the mixed-precision Newton search (FP32 Taylor-model passes,
-FP64 finish) with its distance to the FP64 root, and the CPU baseline
+quadrature time, the mixed-precision Newton search (BF16x3 Taylor-model passes,
+FP64 finish; also with FP32 passes) with its distance to the FP64 root, and the CPU baseline
 (numpy/OpenBLAS BLAS-form det: one det's worth of l-slab ZGEMMs over 8
 synthetic slabs plus one LU).
 
@@ -155,7 +155,12 @@ def main() -> int:
              "device_orbits_ms": orbit_ms, "precompute_device_orbits_s": t_pre_d,
              "device_orbits_max_rel_domega": d_om, "device_orbits_max_dsample": d_samples,
              "device_orbits_det_rel_diff": d_det,
-             "W_quadrature_tflops": 4.0 * n_l * g.nwa * N * NPh / (W_ms * 1e-3) / 1e12}
+             # algorithmic 4 n_l nwa N flop per orbit; on the symmetric alpha grid
+             # only the upper half of the columns is computed (W(-a) = conj W(a)),
+             # in workgroups of 64 columns
+             "W_quadrature_tflops_algorithmic": 4.0 * n_l * g.nwa * N * NPh / (W_ms * 1e-3) / 1e12,
+             "W_quadrature_tflops_executed": 4.0 * n_l * g.nwa * (-(-(N - N // 2) // 64) * 64)
+                                             * NPh / (W_ms * 1e-3) / 1e12}
 
     if not args.no_fp32:
         ctx.prepare(_ffi.PREC_FP32)

@@ -434,7 +434,7 @@ __device__ __forceinline__ void w_produce(
 template <int NQ>
 __device__ __forceinline__ void w_consume(
     const WStage* st, int64_t k, int nst, int ab, int lq0, int lc0, int ac0, int64_t NPh, int N,
-    int Np, int n_l, bool on, double2* __restrict__ W)
+    int Np, int n_l, bool on, bool mirror, double2* __restrict__ W)
 {
     const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
     v4d accr[NQ], acci[NQ];
@@ -473,12 +473,18 @@ __device__ __forceinline__ void w_consume(
     if (!on) return;
     const int a = ac0 + 16 * ab + i;
     if (a >= N) return;
+    // symmetric alpha grid: W(-alpha) = conj W(alpha) (the l-side factor is real)
+    const int am = N - 1 - a;
+    const bool wm = mirror && am < a;
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
         for (int v = 0; v < 4; ++v) {
             const int li = lc0 + 16 * (lq0 + q) + kk + 4 * v;   // f64 D layout: row = lane/16 + 4v
-            if (li < n_l)
-                W[((int64_t)li * NPh + k) * Np + a] = make_double2(accr[q][v], acci[q][v]);
+            if (li < n_l) {
+                double2* row = W + ((int64_t)li * NPh + k) * Np;
+                row[a] = make_double2(accr[q][v], acci[q][v]);
+                if (wm) row[am] = make_double2(accr[q][v], -acci[q][v]);
+            }
         }
 }
 
@@ -487,14 +493,14 @@ __global__ __launch_bounds__(64 * kWMmaWaves + 256, 1) void k_W_f64(
     const double* __restrict__ w1o, const double* __restrict__ mph,
     const int* __restrict__ valid, const double* __restrict__ alpha,
     const int* __restrict__ l_arr, int l_consec, double dalpha, int64_t NPh, int N, int Np,
-    int n_l, int nwa_p, double2* __restrict__ W)
+    int n_l, int nwa_p, int a_first, int mirror, double2* __restrict__ W)
 {
     __shared__ WStage st[2];
     const int64_t k = blockIdx.x;
     const int f = valid[k];
     if (!f) return;
     const int wave = threadIdx.x >> 6;
-    const int ac0 = blockIdx.y * kWAc, lc0 = blockIdx.z * kWLc;
+    const int ac0 = a_first + blockIdx.y * kWAc, lc0 = blockIdx.z * kWLc;
     const int nlb = min(kWLc / 16, (n_l - lc0 + 15) / 16);   // active l-blocks
     const int nst = (nwa_p + kWTk - 1) / kWTk;
     if (wave >= kWMmaWaves) {
@@ -511,10 +517,10 @@ __global__ __launch_bounds__(64 * kWMmaWaves + 256, 1) void k_W_f64(
     const int nq = max(0, min(4, nlb - lq0));
     const bool on = ac0 + 16 * ab < N && nq > 0;
     switch (nq) {
-    case 4: w_consume<4>(st, k, nst, ab, lq0, lc0, ac0, NPh, N, Np, n_l, on, W); break;
-    case 3: w_consume<3>(st, k, nst, ab, lq0, lc0, ac0, NPh, N, Np, n_l, on, W); break;
-    case 2: w_consume<2>(st, k, nst, ab, lq0, lc0, ac0, NPh, N, Np, n_l, on, W); break;
-    default: w_consume<1>(st, k, nst, ab, lq0, lc0, ac0, NPh, N, Np, n_l, on, W); break;
+    case 4: w_consume<4>(st, k, nst, ab, lq0, lc0, ac0, NPh, N, Np, n_l, on, mirror != 0, W); break;
+    case 3: w_consume<3>(st, k, nst, ab, lq0, lc0, ac0, NPh, N, Np, n_l, on, mirror != 0, W); break;
+    case 2: w_consume<2>(st, k, nst, ab, lq0, lc0, ac0, NPh, N, Np, n_l, on, mirror != 0, W); break;
+    default: w_consume<1>(st, k, nst, ab, lq0, lc0, ac0, NPh, N, Np, n_l, on, mirror != 0, W); break;
     }
 }
 
@@ -1378,12 +1384,21 @@ static int W_from_orbits(klsp_ctx* ctx, int64_t nwa, const OrbitBufs& ob, const 
     int l_consec = 1;
     for (int64_t j = 1; j < ctx->n_l; ++j)
         if (l_arr[j] != l_arr[0] + j) l_consec = 0;
+    // An exactly symmetric alpha grid (the reference's range(-alpha_max,
+    // alpha_max, N) is) needs only its upper half: W_l(-alpha) = conj W_l(alpha),
+    // since the l-side factor wt cos(l w1 + m phi) / sqrt(r) is real and the
+    // non-finite zeroing is the same for both signs.  The kernel computes
+    // columns a_first.. and writes each one's conjugate at N - 1 - a too.
+    int mirror = ctx->N >= 2;
+    for (int64_t j = 0; j < ctx->N && mirror; ++j)
+        if (!(alpha[j] == -alpha[ctx->N - 1 - j])) mirror = 0;
+    const int a_first = mirror ? (int)(ctx->N / 2) : 0;
     hipLaunchKernelGGL(k_W_f64,
-                       dim3((unsigned)NPh, (unsigned)((ctx->N + kWAc - 1) / kWAc),
+                       dim3((unsigned)NPh, (unsigned)((ctx->N - a_first + kWAc - 1) / kWAc),
                             (unsigned)((ctx->n_l + kWLc - 1) / kWLc)),
                        dim3(64 * kWMmaWaves + 256), 0, ctx->stream, d_u, d_wt, d_x1, d_mp, d_valid, d_al, d_l,
                        l_consec, dalpha, NPh, (int)ctx->N, ctx->Np, (int)ctx->n_l, (int)nwa_p,
-                       ctx->d_W64);
+                       a_first, mirror, ctx->d_W64);
     hipEventRecord(ctx->ev[1], ctx->stream);
     e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = hipGetLastError();
