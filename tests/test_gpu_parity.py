@@ -92,7 +92,9 @@ def test_W_quadrature_small_grid(oracle_small, small_cfg_paths):
         assert relnorm(W[:, :, il], oracle_small["W"][:, :, il]) <= 1e-12, il
 
 
-@pytest.mark.parametrize("case", ["gaps_large_l", "large_alpha", "uneven_alpha"])
+@pytest.mark.parametrize("case", ["gaps_large_l", "large_alpha", "uneven_alpha",
+                                  "even_symmetric", "wide_even_symmetric",
+                                  "uneven_symmetric"])
 def test_W_quadrature_general_l_and_large_arguments(case):
     """compute_W with l values that are not one consecutive run (every cos
     evaluated directly), with alphas that are not evenly spaced (every
@@ -102,7 +104,10 @@ def test_W_quadrature_general_l_and_large_arguments(case):
     one launch for large alpha).  With alpha up to 5e5 the phase alpha ln r
     inherits alpha times the one-ulp difference between the device's and the
     host's ln, so that case is held to 4 max|alpha| eps max|ln r| instead of
-    1e-12."""
+    1e-12.  The symmetric cases (alpha[j] == -alpha[N-1-j] exactly, N even)
+    take the kernel's mirrored form, which computes the columns N/2.. and
+    writes W(-alpha) = conj W(alpha) into the other half: 20 and 150 evenly
+    spaced alphas (one and two alpha-workgroups) and 36 unevenly spaced ones."""
     p = oracle.Params(NR=9, Ne=7, Rc_min=0.05, Rc_max=20.0, v_min=0.01, N_alpha=21,
                       alpha_max=8.0, l_min=-3, l_max=3, nw=401, nwa=41)
     g = oracle.compute_grids(p)
@@ -113,9 +118,21 @@ def test_W_quadrature_general_l_and_large_arguments(case):
     elif case == "uneven_alpha":
         l_arr = np.arange(-3, 4, dtype=np.int32)
         alpha = np.sort(np.random.default_rng(7).uniform(-8.0, 8.0, 37))
+    elif case == "even_symmetric":
+        l_arr = np.arange(-3, 4, dtype=np.int32)
+        alpha = oracle.linrange(-8.0, 8.0, 20)
+    elif case == "wide_even_symmetric":
+        l_arr = np.arange(-3, 4, dtype=np.int32)
+        alpha = oracle.linrange(-8.0, 8.0, 150)
+    elif case == "uneven_symmetric":
+        l_arr = np.arange(-3, 4, dtype=np.int32)
+        h = np.sort(np.random.default_rng(11).uniform(0.01, 8.0, 18))
+        alpha = np.concatenate([-h[::-1], h])
     else:
         l_arr = np.arange(-3, 4, dtype=np.int32)
         alpha = oracle.linrange(-5e5, 5e5, 23)
+    if case.endswith("_symmetric"):
+        assert alpha.size % 2 == 0 and np.array_equal(alpha, -alpha[::-1])
     Wo = oracle.compute_W(p, o, alpha, l_arr)
     NPh, N, n_l = p.NPh, alpha.size, l_arr.size
     ctx = _ffi.Context(0)
