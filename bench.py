@@ -44,6 +44,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -329,12 +340,9 @@ def main() -> int:
         args.cpu_slabs = n_l
     if args.cpu_slabs > 0:
         from oracle import blas_form
-        try:
-            from threadpoolctl import threadpool_info
-            cores = max((i.get("num_threads", 1) for i in threadpool_info()
-                         if i.get("internal_api") == "openblas"), default=os.cpu_count())
-        except Exception:  # pragma: no cover
-            cores = os.cpu_count()
+        from threadpoolctl import threadpool_info, threadpool_limits
+        blas = [i for i in threadpool_info() if i.get("user_api") == "blas"]
+        cores = max((i.get("num_threads", 1) for i in blas), default=os.cpu_count())
         t0 = time.perf_counter()
         s_det, s_slab, s_lu = blas_form.time_sample(NPh, N, n_l, args.cpu_slabs)
         log(f"[bench] CPU BLAS-form: {s_slab * 1e3:.1f} ms per l-slab, LU {s_lu * 1e3:.2f} ms -> "
@@ -342,7 +350,19 @@ def main() -> int:
         cpu = {"value": 1.0 / s_det, "unit": "det/s", "cores": int(cores), "kind": "port",
                "sample": f"numpy/OpenBLAS BLAS-form det: {args.cpu_slabs} per-l ZGEMMs "
                          f"W_l^T (c*conj W_l) over 8 synthetic [{NPh}x{N}] complex128 l-slabs + "
-                         f"one {N}x{N} LU, scaled to {n_l} slabs per det"}
+                         f"one {N}x{N} LU, scaled to {n_l} slabs per det",
+               "cpu_model": _cpu_model(),
+               "blas": [{k: i.get(k) for k in ("internal_api", "version", "architecture",
+                                               "num_threads")} for i in blas]}
+        # the reference's runner caps BLAS at 4 threads (configs/default.toml:28)
+        n4 = min(args.cpu_slabs, 48)
+        t0 = time.perf_counter()
+        with threadpool_limits(limits=4, user_api="blas"):
+            s4_det, s4_slab, _ = blas_form.time_sample(NPh, N, n_l, n4)
+        log(f"[bench] CPU BLAS-form at 4 threads: {s4_slab * 1e3:.1f} ms per l-slab -> "
+            f"{s4_det:.2f} s per det ({time.perf_counter() - t0:.1f} s of CPU work, {n4} slabs)")
+        cpu["value_4_threads"] = 1.0 / s4_det
+        cpu["sample_4_threads"] = f"{n4} l-slabs at 4 BLAS threads, scaled to {n_l} slabs per det"
         if "newton_det_evals" in extra:
             # the CPU path runs the same reference iteration: its roots/s is the
             # det rate over the builds of the FP64 search above (not run on the CPU)

@@ -183,14 +183,19 @@ __device__ __forceinline__ void wave_f64(const ContractArgs& a, int64_t Kb, int6
         // and its loads by at most 19, inside the kRowPad zero rows.
         double2 A0[MB], X0[NB], C0[1], A1[MB], X1[NB], C1[1];
         double2 A2[MB], X2[NB], C2[1], A3[MB], X3[NB], C3[1];
+        // a 32-bit trip count keeps the loop test in scalar registers.  The
+        // prologue sits behind the same test as the loop: otherwise the
+        // compiler sinks its loads past that test, reorders them, and the
+        // wait at the loop head (which must hold for the first trip too)
+        // then covers part of the second operand set.
+        const int ntrip = (int)((Ke - Kb + 4 * kKStep - 1) / (4 * kKStep));
+        if (ntrip > 0) {
         cload<MB, NB, 1>(pa, px, pc, Kpc, A0, X0, C0);
         __builtin_amdgcn_sched_barrier(0);
         cload<MB, NB, 1>(pa + rstep, px + rstep, pc + kKStep, Kpc, A1, X1, C1);
         __builtin_amdgcn_sched_barrier(0);
         cload<MB, NB, 1>(pa + 2 * rstep, px + 2 * rstep, pc + 2 * kKStep, Kpc, A2, X2, C2);
         __builtin_amdgcn_sched_barrier(0);
-        // a 32-bit trip count keeps the loop test in scalar registers
-        const int ntrip = (int)((Ke - Kb + 4 * kKStep - 1) / (4 * kKStep));
         for (int trip = 0; trip < ntrip; ++trip) {
             cload<MB, NB, 1>(pa + 3 * rstep, px + 3 * rstep, pc + 3 * kKStep, Kpc, A3, X3, C3);
             __builtin_amdgcn_sched_barrier(0);
@@ -211,6 +216,7 @@ __device__ __forceinline__ void wave_f64(const ContractArgs& a, int64_t Kb, int6
             pa += 4 * rstep;
             px += 4 * rstep;
             pc += 4 * kKStep;
+        }
         }
     } else {
     double2 A0[MB], X0[NB], C0[NW], A1[MB], X1[NB], C1[NW];
