@@ -41,7 +41,9 @@ enum klsp_status {
     KLSP_ERR_OOM = 2,          /* device or host allocation failed          */
     KLSP_ERR_HIP = 3,          /* a HIP runtime call failed                 */
     KLSP_ERR_NOT_READY = 4,    /* e.g. det before the problem/W is on device */
-    KLSP_ERR_NO_DEVICE = 5     /* no gfx950 device visible                  */
+    KLSP_ERR_NO_DEVICE = 5,    /* no gfx950 device visible                  */
+    KLSP_ERR_HOST = 6          /* another host-side failure (e.g. a worker
+                                  thread could not start)                  */
 };
 
 /* Arithmetic of an M(omega) build.  The determinant (LU) is FP64 always. */

@@ -20,6 +20,7 @@ KLSP_ERR_OOM = 2
 KLSP_ERR_HIP = 3
 KLSP_ERR_NOT_READY = 4
 KLSP_ERR_NO_DEVICE = 5
+KLSP_ERR_HOST = 6
 
 PREC_FP64 = 0
 PREC_FP32 = 1
@@ -33,6 +34,7 @@ _STATUS_NAMES = {
     KLSP_ERR_HIP: "HIP runtime error",
     KLSP_ERR_NOT_READY: "not ready",
     KLSP_ERR_NO_DEVICE: "no gfx950 device",
+    KLSP_ERR_HOST: "host failure",
 }
 
 # Every symbol include/klsp.h declares (checked by tests/test_abi.py).

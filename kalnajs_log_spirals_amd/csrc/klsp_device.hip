@@ -1216,7 +1216,8 @@ int klsp_create(int device, klsp_ctx** ctx_out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return KLSP_ERR_HIP;
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return KLSP_ERR_NO_DEVICE;
-    klsp_ctx* ctx = new klsp_ctx();
+    klsp_ctx* ctx = new (std::nothrow) klsp_ctx();
+    if (!ctx) return KLSP_ERR_OOM;
     ctx->device = device;
     ctx->n_cu = prop.multiProcessorCount;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
@@ -1253,7 +1254,7 @@ int klsp_destroy(klsp_ctx* ctx)
 const char* klsp_last_error(const klsp_ctx* ctx) { return ctx ? ctx->err.c_str() : g_noctx_err; }
 
 int klsp_upload_problem(klsp_ctx* ctx, const klsp_problem* pr)
-{
+try {
     if (!ctx) return KLSP_ERR_INVALID_ARG;
     const bool arrays = pr && pr->DJ && pr->F0l && pr->Omega_res;
     const bool none = pr && !pr->DJ && !pr->F0l && !pr->Omega_res;
@@ -1283,11 +1284,13 @@ int klsp_upload_problem(klsp_ctx* ctx, const klsp_problem* pr)
     if (arrays) return klsp_set_problem_arrays(ctx, pr->DJ, pr->F0l, pr->Omega_res);
     ctx->have_arrays = false;
     return KLSP_OK;
+} catch (...) {
+    return klsp::exception_status(ctx ? &ctx->err : nullptr);
 }
 
 int klsp_set_problem_arrays(klsp_ctx* ctx, const double* DJ, const double* F0l,
                             const double* Omega_res)
-{
+try {
     if (!ctx) return KLSP_ERR_INVALID_ARG;
     if (!ctx->have_problem)
         return fail(ctx, KLSP_ERR_NOT_READY, "klsp_set_problem_arrays: upload the problem first");
@@ -1300,6 +1303,8 @@ int klsp_set_problem_arrays(klsp_ctx* ctx, const double* DJ, const double* F0l,
     KLSP_HIP(ctx, hipMemcpy(ctx->d_Ores, Omega_res, sizeof(double) * nK, hipMemcpyHostToDevice));
     ctx->have_arrays = true;
     return KLSP_OK;
+} catch (...) {
+    return klsp::exception_status(ctx ? &ctx->err : nullptr);
 }
 
 static int alloc_W64(klsp_ctx* ctx)
@@ -1416,7 +1421,7 @@ extern "C" {
 int klsp_compute_W(klsp_ctx* ctx, int64_t nwa, const double* w1, const double* ra,
                    const double* pha, const double* Om1, const double* Om2,
                    const double* alpha, const int32_t* l_arr, double* W_out)
-{
+try {
     if (!ctx) return KLSP_ERR_INVALID_ARG;
     if (!ctx->have_problem)
         return fail(ctx, KLSP_ERR_NOT_READY, "klsp_compute_W: upload the problem first");
@@ -1439,6 +1444,8 @@ int klsp_compute_W(klsp_ctx* ctx, int64_t nwa, const double* w1, const double* r
     if (rc) return rc;
     if (W_out) return klsp_download_W(ctx, W_out);
     return KLSP_OK;
+} catch (...) {
+    return klsp::exception_status(ctx ? &ctx->err : nullptr);
 }
 
 int klsp_compute_orbits_W(klsp_ctx* ctx, const klsp_params* p, const double* Rc,
@@ -1446,7 +1453,7 @@ int klsp_compute_orbits_W(klsp_ctx* ctx, const klsp_params* p, const double* Rc,
                           const double* L, const double* L2, const double* SGNL,
                           const double* alpha, const int32_t* l_arr, double* Om1, double* Om2,
                           double* w1, double* ra, double* pha)
-{
+try {
     if (!ctx) return KLSP_ERR_INVALID_ARG;
     if (!ctx->have_problem)
         return fail(ctx, KLSP_ERR_NOT_READY, "klsp_compute_orbits_W: upload the problem first");
@@ -1499,10 +1506,12 @@ int klsp_compute_orbits_W(klsp_ctx* ctx, const klsp_params* p, const double* Rc,
     if (ra) KLSP_HIP(ctx, hipMemcpy(ra, ob.ra, no * 8, hipMemcpyDeviceToHost));
     if (pha) KLSP_HIP(ctx, hipMemcpy(pha, ob.pha, no * 8, hipMemcpyDeviceToHost));
     return KLSP_OK;
+} catch (...) {
+    return klsp::exception_status(ctx ? &ctx->err : nullptr);
 }
 
 int klsp_upload_W(klsp_ctx* ctx, const double* W)
-{
+try {
     if (!ctx) return KLSP_ERR_INVALID_ARG;
     if (!ctx->have_problem)
         return fail(ctx, KLSP_ERR_NOT_READY, "klsp_upload_W: upload the problem first");
@@ -1525,10 +1534,12 @@ int klsp_upload_W(klsp_ctx* ctx, const double* W)
     KLSP_HIP(ctx, e);
     ctx->have_W = true;
     return KLSP_OK;
+} catch (...) {
+    return klsp::exception_status(ctx ? &ctx->err : nullptr);
 }
 
 int klsp_download_W(klsp_ctx* ctx, double* W_out)
-{
+try {
     if (!ctx) return KLSP_ERR_INVALID_ARG;
     if (!ctx->have_W) return fail(ctx, KLSP_ERR_NOT_READY, "klsp_download_W: no W on device");
     if (!W_out) return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_download_W: null output");
@@ -1546,10 +1557,12 @@ int klsp_download_W(klsp_ctx* ctx, double* W_out)
     }
     hipFree(d_slab);
     return KLSP_OK;
+} catch (...) {
+    return klsp::exception_status(ctx ? &ctx->err : nullptr);
 }
 
 int klsp_prepare(klsp_ctx* ctx, int precision)
-{
+try {
     if (!ctx) return KLSP_ERR_INVALID_ARG;
     if (precision != KLSP_PREC_FP64 && precision != KLSP_PREC_FP32 &&
         precision != KLSP_PREC_BF16X3)
@@ -1572,6 +1585,8 @@ int klsp_prepare(klsp_ctx* ctx, int precision)
         KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
     return KLSP_OK;
+} catch (...) {
+    return klsp::exception_status(ctx ? &ctx->err : nullptr);
 }
 
 }  // extern "C"
@@ -2054,7 +2069,7 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
 extern "C" {
 
 int klsp_build_M(klsp_ctx* ctx, double om_re, double om_im, int precision, double* M_out)
-{
+try {
     if (!ctx) return KLSP_ERR_INVALID_ARG;
     if (!M_out) return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_build_M: null output");
     int rc = check_ready(ctx, precision, "klsp_build_M");
@@ -2067,11 +2082,13 @@ int klsp_build_M(klsp_ctx* ctx, double om_re, double om_im, int precision, doubl
                                  hipMemcpyDeviceToHost, ctx->stream));
     KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return KLSP_OK;
+} catch (...) {
+    return klsp::exception_status(ctx ? &ctx->err : nullptr);
 }
 
 int klsp_build_taylor(klsp_ctx* ctx, double om_re, double om_im, int n_terms, int precision,
                       double* T_out)
-{
+try {
     if (!ctx) return KLSP_ERR_INVALID_ARG;
     if (!T_out || n_terms < 1 ||
         n_terms > (precision == KLSP_PREC_BF16X3 ? kMaxTerms : kMaxBatch))
@@ -2086,23 +2103,27 @@ int klsp_build_taylor(klsp_ctx* ctx, double om_re, double om_im, int n_terms, in
                                  hipMemcpyDeviceToHost, ctx->stream));
     KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return KLSP_OK;
+} catch (...) {
+    return klsp::exception_status(ctx ? &ctx->err : nullptr);
 }
 
 int klsp_det(klsp_ctx* ctx, const double* omegas, int64_t n, int precision, double* dets_out)
-{
+try {
     if (!ctx) return KLSP_ERR_INVALID_ARG;
     if (n < 0 || (n > 0 && (!omegas || !dets_out)))
         return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_det: bad arguments");
     reset_timing(ctx);
     if (n == 0) return KLSP_OK;
     return det_many(ctx, omegas, n, precision, dets_out);
+} catch (...) {
+    return klsp::exception_status(ctx ? &ctx->err : nullptr);
 }
 
 // newton_search, NewtonSolver.jl:36-172, with the forward-difference pair
 // (omega + m*delta, omega + i*delta) evaluated as one two-omega batch.
 int klsp_newton(klsp_ctx* ctx, const klsp_newton_params* np, klsp_newton_result* res,
                 double* history)
-{
+try {
     if (!ctx) return KLSP_ERR_INVALID_ARG;
     if (!np || !res || np->max_iter < 0 || np->max_line_search < 0)
         return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_newton: bad arguments");
@@ -2193,11 +2214,13 @@ int klsp_newton(klsp_ctx* ctx, const klsp_newton_params* np, klsp_newton_result*
     res->n_lp_builds = prec == KLSP_PREC_FP64 ? 0 : n_eval;
     ctx->last = acc;
     return KLSP_OK;
+} catch (...) {
+    return klsp::exception_status(ctx ? &ctx->err : nullptr);
 }
 
 int klsp_grid_scan(klsp_ctx* ctx, const klsp_scan_params* sp, double* det_grid,
                    double* Op_init, double* g_init)
-{
+try {
     if (!ctx) return KLSP_ERR_INVALID_ARG;
     if (!sp || sp->n_Op < 1 || sp->n_gamma < 1 || !det_grid)
         return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_grid_scan: bad arguments");
@@ -2225,6 +2248,8 @@ int klsp_grid_scan(klsp_ctx* ctx, const klsp_scan_params* sp, double* det_grid,
     if (Op_init) *Op_init = Opg[best / sp->n_gamma];
     if (g_init) *g_init = gg[best % sp->n_gamma];
     return KLSP_OK;
+} catch (...) {
+    return klsp::exception_status(ctx ? &ctx->err : nullptr);
 }
 
 int klsp_set_profiling(klsp_ctx* ctx, int on)

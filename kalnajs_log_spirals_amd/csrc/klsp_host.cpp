@@ -233,7 +233,7 @@ int klsp_compute_N_kernel(int m, const double* alpha, const double* d_alpha, int
 int klsp_compute_grids(const klsp_params* p, double* Rc, double* e, double* v, double* R1,
                        double* R2, double* E, double* L, double* L2, double* SGNL,
                        double* S_RC, double* S_e)
-{
+try {
     if (!params_ok(p) || !Rc || !e || !v || !R1 || !R2 || !E || !L || !L2 || !SGNL || !S_RC ||
         !S_e)
         return KLSP_ERR_INVALID_ARG;
@@ -269,6 +269,8 @@ int klsp_compute_grids(const klsp_params* p, double* Rc, double* e, double* v, d
             S_e[k] = se[j]; E[k] = En; L2[k] = l2; L[k] = std::sqrt(l2) * sg;
         }
     return KLSP_OK;
+} catch (...) {
+    return klsp::exception_status(nullptr);
 }
 
 int klsp_compute_orbits(const klsp_params* p, const double* Rc, const double* e,
@@ -276,7 +278,7 @@ int klsp_compute_orbits(const klsp_params* p, const double* Rc, const double* e,
                         const double* L2, const double* SGNL, double* Om1, double* Om2,
                         double* w1, double* ra, double* pha, double* jac, double* F0,
                         double* FE, double* FL, int n_threads)
-{
+try {
     if (!params_ok(p) || !Rc || !e || !R1 || !R2 || !E || !L || !L2 || !SGNL || !Om1 || !Om2 ||
         !w1 || !ra || !pha || !jac || !F0 || !FE || !FL)
         return KLSP_ERR_INVALID_ARG;
@@ -380,6 +382,8 @@ int klsp_compute_orbits(const klsp_params* p, const double* Rc, const double* e,
     for (auto& th : pool) th.join();
 
     return klsp_compute_jacobian_df(p, Rc, R1, R2, E, L, Om1, jac, F0, FE, FL);
+} catch (...) {
+    return klsp::exception_status(nullptr);
 }
 
 int klsp_orbit_phase_grid(const klsp_params* p, double* w, double* cw, double* sw)
@@ -396,7 +400,7 @@ int klsp_orbit_phase_grid(const klsp_params* p, double* w, double* cw, double* s
 int klsp_compute_jacobian_df(const klsp_params* p, const double* Rc, const double* R1,
                              const double* R2, const double* E, const double* L,
                              const double* Om1, double* jac, double* F0, double* FE, double* FL)
-{
+try {
     if (!params_ok(p) || !Rc || !R1 || !R2 || !E || !L || !Om1 || !jac || !F0 || !FE || !FL)
         return KLSP_ERR_INVALID_ARG;
     const int64_t NPh = (int64_t)p->NR * p->Ne;
@@ -421,13 +425,15 @@ int klsp_compute_jacobian_df(const klsp_params* p, const double* Rc, const doubl
         FL[k] = std::isfinite(fl) ? fl : 0.0;
     }
     return KLSP_OK;
+} catch (...) {
+    return klsp::exception_status(nullptr);
 }
 
 int klsp_precompute_arrays(const klsp_params* p, const double* S_RC, const double* S_e,
                            const double* jac, const double* Om1, const double* Om2,
                            const double* FE, const double* FL, double* DJ, double* F0l,
                            double* Ores)
-{
+try {
     if (!p || p->NR < 1 || p->Ne < 1 || p->l_max < p->l_min || !S_RC || !S_e || !jac || !Om1 ||
         !Om2 || !FE || !FL || !DJ || !F0l || !Ores)
         return KLSP_ERR_INVALID_ARG;
@@ -442,6 +448,8 @@ int klsp_precompute_arrays(const klsp_params* p, const double* S_RC, const doubl
         }
     }
     return KLSP_OK;
+} catch (...) {
+    return klsp::exception_status(nullptr);
 }
 
 }  // extern "C"

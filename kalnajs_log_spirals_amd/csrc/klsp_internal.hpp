@@ -3,8 +3,42 @@
 #pragma once
 
 #include <cstdint>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "klsp.h"
 
 namespace klsp {
+
+// Status of the C++ exception being handled (call only inside a catch
+// clause): the C entry points end in a function-try-block whose handler
+// returns this, so no exception crosses the C ABI.  Host allocation failure
+// is KLSP_ERR_OOM, a size beyond the host's limits KLSP_ERR_INVALID_ARG,
+// anything else (e.g. a worker thread that could not start) KLSP_ERR_HOST.
+// The reason goes to *msg when msg is not null.
+inline int exception_status(std::string* msg) noexcept
+{
+    int code = KLSP_ERR_HOST;
+    try {
+        try {
+            throw;
+        } catch (const std::bad_alloc&) {
+            code = KLSP_ERR_OOM;
+            if (msg) *msg = "klsp: host allocation failed";
+        } catch (const std::length_error&) {
+            code = KLSP_ERR_INVALID_ARG;
+            if (msg) *msg = "klsp: size beyond the host's limits";
+        } catch (const std::exception& e) {
+            if (msg) *msg = std::string("klsp: ") + e.what();
+        } catch (...) {
+            if (msg) *msg = "klsp: unknown host failure";
+        }
+    } catch (...) {
+        // the message itself could not be stored; the status still says why
+    }
+    return code;
+}
 
 // Godfrey-Lanczos complex Gamma (host), out = {re, im}.
 void host_gamma(double re, double im, double* out);

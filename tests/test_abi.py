@@ -82,3 +82,38 @@ def test_host_precompute_matches_oracle(small_cfg_paths):
     assert np.array_equal(K.compute_N_kernel(2, alpha, da), oracle.N_kernel(2, alpha, da))
     for z in (0.3 + 2j, -1.3 + 0.2j, 1.0 + 0j, 2.0 + 0j):
         assert K.gamma_complex(z) == oracle.gamma_complex(z)
+
+
+_OOM_CHILD = r"""
+import ctypes as ct, resource, sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from kalnajs_log_spirals_amd import _ffi
+L = _ffi.lib()
+# Address space capped 1 GiB above what the process maps now: the grid
+# routine's first host vector (NR doubles = 1.6 GB) cannot be allocated, so
+# it must return KLSP_ERR_OOM before it writes any output.
+with open("/proc/self/statm") as fh:
+    mapped = int(fh.read().split()[0]) * resource.getpagesize()
+resource.setrlimit(resource.RLIMIT_AS, (mapped + (1 << 30), resource.RLIM_INFINITY))
+p = _ffi.klsp_params(m=2, G=1.0, L0=1.0, n_zang=4, q1=7, NR=200_000_000, Ne=3, Rc_min=0.1,
+                     Rc_max=10.0, v_min=0.05, N_alpha=5, alpha_max=5.0, l_min=-1, l_max=1,
+                     nw=401, nwa=41)
+outs = [np.zeros(1) for _ in range(11)]
+rc = L.klsp_compute_grids(ct.byref(p), *[o.ctypes.data_as(ct.POINTER(ct.c_double)) for o in outs])
+print("status", rc)
+sys.exit(0 if rc == _ffi.KLSP_ERR_OOM else 1)
+"""
+
+
+def test_host_allocation_failure_is_a_status():
+    """A failed host allocation inside an entry point comes back as
+    KLSP_ERR_OOM instead of a C++ exception through the C ABI (which would
+    terminate the caller).  Run in a child process with a capped address
+    space."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, "-c", _OOM_CHILD, ROOT], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stdout, r.stderr[-2000:])
+    assert "status 2" in r.stdout
