@@ -368,18 +368,36 @@ try {
     unsigned nt = n_threads > 0 ? (unsigned)n_threads : std::thread::hardware_concurrency();
     nt = std::max(1u, std::min<unsigned>(nt, 64u));
     std::atomic<int64_t> next{0};
-    auto worker = [&]() {
-        std::vector<double> xs(nw), rv(nw), sv(nw), cum(nw);
-        for (;;) {
-            const int64_t k0 = next.fetch_add(16);
-            if (k0 >= NPh) break;
-            for (int64_t k = k0; k < std::min<int64_t>(k0 + 16, NPh); ++k) orbit(k, xs, rv, sv, cum);
+    std::atomic<bool> failed{false};
+    // Orbits are taken 16 at a time from one shared counter.  A worker that
+    // cannot allocate its scratch drains the counter and flags the call,
+    // which fails once every worker has been joined (an exception must not
+    // leave a thread function, nor unwind past joinable threads).
+    auto worker = [&]() noexcept {
+        try {
+            std::vector<double> xs(nw), rv(nw), sv(nw), cum(nw);
+            for (;;) {
+                const int64_t k0 = next.fetch_add(16);
+                if (k0 >= NPh) break;
+                for (int64_t k = k0; k < std::min<int64_t>(k0 + 16, NPh); ++k)
+                    orbit(k, xs, rv, sv, cum);
+            }
+        } catch (...) {
+            failed = true;
+            next = NPh;
         }
     };
     std::vector<std::thread> pool;
-    for (unsigned t = 1; t < nt; ++t) pool.emplace_back(worker);
+    try {
+        pool.reserve(nt - 1);
+        for (unsigned t = 1; t < nt; ++t) pool.emplace_back(worker);
+    } catch (...) {
+        // fewer threads than asked for: the counter is shared, so the ones
+        // that started (and this thread) still cover every orbit
+    }
     worker();
     for (auto& th : pool) th.join();
+    if (failed) return KLSP_ERR_OOM;
 
     return klsp_compute_jacobian_df(p, Rc, R1, R2, E, L, Om1, jac, F0, FE, FL);
 } catch (...) {

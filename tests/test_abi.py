@@ -90,30 +90,48 @@ import numpy as np
 sys.path.insert(0, sys.argv[1])
 from kalnajs_log_spirals_amd import _ffi
 L = _ffi.lib()
-# Address space capped 1 GiB above what the process maps now: the grid
-# routine's first host vector (NR doubles = 1.6 GB) cannot be allocated, so
-# it must return KLSP_ERR_OOM before it writes any output.
+D = ct.POINTER(ct.c_double)
+ptr = lambda a: a.ctypes.data_as(D)
+mode = sys.argv[2]
+if mode == "grids":
+    # the grid routine's first host vector (NR doubles = 1.6 GB) cannot be
+    # allocated; it must fail before writing any output
+    p = _ffi.klsp_params(m=2, G=1.0, L0=1.0, n_zang=4, q1=7, NR=200_000_000, Ne=3, Rc_min=0.1,
+                         Rc_max=10.0, v_min=0.05, N_alpha=5, alpha_max=5.0, l_min=-1, l_max=1,
+                         nw=401, nwa=41)
+else:
+    # one orbit with nw = 3e7 phase samples: the shared phase tables (0.72 GB)
+    # fit under the cap, each worker's scratch (0.96 GB) does not
+    p = _ffi.klsp_params(m=2, G=1.0, L0=1.0, n_zang=4, q1=7, NR=1, Ne=1, Rc_min=1.0,
+                         Rc_max=1.0, v_min=0.5, N_alpha=5, alpha_max=5.0, l_min=-1, l_max=1,
+                         nw=30_000_001, nwa=2)
+    g = [np.zeros(1) for _ in range(11)]
+    assert L.klsp_compute_grids(ct.byref(p), *[ptr(a) for a in g]) == 0
+outs = [np.zeros(2) for _ in range(11)]
+# address space capped 1 GiB above what the process maps now
 with open("/proc/self/statm") as fh:
     mapped = int(fh.read().split()[0]) * resource.getpagesize()
 resource.setrlimit(resource.RLIMIT_AS, (mapped + (1 << 30), resource.RLIM_INFINITY))
-p = _ffi.klsp_params(m=2, G=1.0, L0=1.0, n_zang=4, q1=7, NR=200_000_000, Ne=3, Rc_min=0.1,
-                     Rc_max=10.0, v_min=0.05, N_alpha=5, alpha_max=5.0, l_min=-1, l_max=1,
-                     nw=401, nwa=41)
-outs = [np.zeros(1) for _ in range(11)]
-rc = L.klsp_compute_grids(ct.byref(p), *[o.ctypes.data_as(ct.POINTER(ct.c_double)) for o in outs])
+if mode == "grids":
+    rc = L.klsp_compute_grids(ct.byref(p), *[ptr(o) for o in outs])
+else:
+    rc = L.klsp_compute_orbits(ct.byref(p), *[ptr(a) for a in (g[0], g[1], g[3], g[4], g[5],
+                               g[6], g[7], g[8])], *[ptr(o) for o in outs[:9]], 2)
 print("status", rc)
 sys.exit(0 if rc == _ffi.KLSP_ERR_OOM else 1)
 """
 
 
-def test_host_allocation_failure_is_a_status():
+@pytest.mark.parametrize("mode", ["grids", "orbit_workers"])
+def test_host_allocation_failure_is_a_status(mode):
     """A failed host allocation inside an entry point comes back as
     KLSP_ERR_OOM instead of a C++ exception through the C ABI (which would
-    terminate the caller).  Run in a child process with a capped address
-    space."""
+    terminate the caller): in the entry point's own thread (grids) and in an
+    orbit worker thread (orbit_workers).  Run in a child process with a
+    capped address space."""
     import subprocess
     import sys
-    r = subprocess.run([sys.executable, "-c", _OOM_CHILD, ROOT], capture_output=True, text=True,
-                       timeout=120)
+    r = subprocess.run([sys.executable, "-c", _OOM_CHILD, ROOT, mode], capture_output=True,
+                       text=True, timeout=120)
     assert r.returncode == 0, (r.returncode, r.stdout, r.stderr[-2000:])
     assert "status 2" in r.stdout
