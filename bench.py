@@ -37,7 +37,7 @@ PEAK_HBM_GBS = 8000.0
 # rocprofv3 --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, calibrated on a
 # copy kernel of known size); profiles/det_default_grid.md holds the run.
 PROFILED_KERNEL = "klsp::k_contract<true, 4, 4, 1>"
-PROFILED_TRAFFIC_BYTES = 91.581e9 + 0.062e9
+PROFILED_TRAFFIC_BYTES = 96.916e9 + 0.062e9
 
 
 def log(*a):
