@@ -139,16 +139,20 @@ def precompute_host(config: KalnajsConfig, model, verbose: bool = False) -> dict
     return dict(od=od, alpha=alpha, d_alpha=d_alpha, Nk=Nk, DJ=DJ, F0l=F0l, Ores=Ores)
 
 
-def precompute_device(config: KalnajsConfig, host: dict, verbose: bool = False) -> PrecomputedData:
+def precompute_device(config: KalnajsConfig, host: dict, verbose: bool = False,
+                      ctx: _ffi.Context | None = None) -> PrecomputedData:
     """Device half of precompute_all: a context with the problem arrays and
-    W_l (BasisFunctions.jl:155-165) computed on the device."""
+    W_l (BasisFunctions.jl:155-165) computed on the device.  An existing
+    context passed as ctx is reused (its previous problem is replaced and its
+    W buffers are kept when the new W fits)."""
     m, G = int(config.physics.m), float(config.physics.G)
     N = int(config.grid.N_alpha)
     l_arr = np.arange(config.grid.l_min, config.grid.l_max + 1, dtype=np.int32)
     NPh = config.grid.NR * config.grid.Ne
     if verbose:
         print("Computing W_l basis functions (complex, on device)...")
-    ctx = _ffi.Context(GPUBackend.current_device())
+    if ctx is None:
+        ctx = _ffi.Context(GPUBackend.current_device())
     ctx.upload_problem(NPh, N, l_arr.size, m, G, host["DJ"], host["F0l"], host["Ores"],
                        host["Nk"])
     w1, ra, pha, o1, o2, _, _ = _orbit_inputs(host["od"])
@@ -161,13 +165,15 @@ def precompute_device(config: KalnajsConfig, host: dict, verbose: bool = False) 
 
 
 def precompute_device_orbits(config: KalnajsConfig, model, verbose: bool = False,
-                             download_orbits: bool = True) -> PrecomputedData:
+                             download_orbits: bool = True,
+                             ctx: _ffi.Context | None = None) -> PrecomputedData:
     """precompute_all with the orbits integrated on the device
     (OrbitIntegration.jl:181-302 as one workgroup per orbit) and fed to the W
     quadrature without leaving it.  Grids, N_m(alpha), the Jacobian / DF and
     DJ / F0l / Omega_res stay on the host (O(NPh) work).  The orbit samples
     are copied back into orbit_data unless download_orbits is False (then
-    w1 / ra / pha are None)."""
+    w1 / ra / pha are None).  ctx: an existing context to reuse, as in
+    precompute_device."""
     m, G = int(config.physics.m), float(config.physics.G)
     N = int(config.grid.N_alpha)
     l_arr = np.arange(config.grid.l_min, config.grid.l_max + 1, dtype=np.int32)
@@ -181,7 +187,9 @@ def precompute_device_orbits(config: KalnajsConfig, model, verbose: bool = False
     Nk = compute_N_kernel(m, alpha, d_alpha)
     if verbose:
         print("Computing orbits and W_l basis functions (on device)...")
-    ctx = _ffi.Context(GPUBackend.current_device())
+    own = ctx is None
+    if own:
+        ctx = _ffi.Context(GPUBackend.current_device())
     try:
         ctx.upload_problem(NPh, N, l_arr.size, m, G, None, None, None, Nk)
         g = dict(Rc=grids.Rc, e=grids.e, R1=grids.R1, R2=grids.R2, E=grids.E, L=grids.L_m,
@@ -196,7 +204,8 @@ def precompute_device_orbits(config: KalnajsConfig, model, verbose: bool = False
         DJ, F0l, Ores = _omega_independent(config, od)
         ctx.set_problem_arrays(DJ, F0l, Ores)
     except BaseException:
-        ctx.close()
+        if own:
+            ctx.close()
         raise
     if verbose:
         print("Precomputation complete.")

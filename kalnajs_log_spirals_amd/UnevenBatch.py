@@ -24,7 +24,7 @@ from typing import List, Optional
 
 import numpy as np
 
-from . import _ffi
+from . import GPUBackend, _ffi
 from .BasisFunctions import precompute_device, precompute_device_orbits, precompute_host, to_device
 from .Configuration import KalnajsConfig, load_config
 from .GPUBackend import GPUType
@@ -67,9 +67,15 @@ class DrawResult:
     host_s: float        # grids + orbits (worker thread, overlapped); 0 with device orbits
     W_s: float           # device W quadrature (wall, incl. orbit upload); with device
                          # orbits: the whole precompute (grids, orbits, W, DF arrays)
-    newton_s: float
+    newton_s: float      # to_device + the search (incl. a low-precision W copy's first build)
     polished: complex = complex("nan")   # root after `polish` FP64 Newton steps, tol = 0
     polish_move: float = float("nan")    # |polished - omega|
+    device_ms: float = float("nan")      # the search's contraction + LU device time
+    setup_s: float = float("nan")        # part of newton_s: to_device + the low-precision W copy
+
+
+# the W copy a mixed search builds its low-precision passes from
+_LP_COPY = {"mixed": _ffi.PREC_BF16X3, "mixed_fp32": _ffi.PREC_FP32}
 
 
 def draw_batch(n: int = 48, seed: int = 0) -> List[Draw]:
@@ -130,42 +136,56 @@ def run_batch(draws: List[Draw], precision: str = "mixed", scale: float = 1.0,
         return h, time.perf_counter() - t0
 
     results = []
-    with ThreadPoolExecutor(max_workers=1) as pool:
-        nxt = pool.submit(host, 0) if draws else None
-        for i, d in enumerate(draws):
-            h, t_host = nxt.result()
-            nxt = pool.submit(host, i + 1) if i + 1 < len(draws) else None
-            t0 = time.perf_counter()
-            if orbits == "device":
-                pre = precompute_device_orbits(cfgs[i], create_toomre_zang_model(cfgs[i]),
-                                               download_orbits=False)
-            else:
-                pre = precompute_device(cfgs[i], h)
-            t_W = time.perf_counter() - t0
-            del h
-            t0 = time.perf_counter()
-            dev = to_device(pre, GPUType.AMD, np.float64)
-            r = newton_search(cfgs[i], pre, dev, GPUType.AMD, np.float64, d.Omega_p0, d.gamma0,
-                              precision=precision)
-            t_n = time.perf_counter() - t0
-            res = DrawResult(d, precision, bool(r.converged), float(r.Omega_p), float(r.gamma),
-                             float(abs(r.det_val)), int(r.iterations), int(r.n_fp64_builds),
-                             int(r.n_lp_builds), t_host, t_W, t_n)
-            if polish > 0 and r.converged:
-                pr, _ = pre.ctx.newton(r.Omega_p, r.gamma, max_iter=polish, tol=0.0,
-                                       precision=_ffi.PREC_FP64)
-                res.polished = complex(pr.omega[0], pr.omega[1])
-                res.polish_move = abs(res.polished - complex(d.m * r.Omega_p, r.gamma))
-            pre.ctx.close()
-            results.append(res)
-            if log:
-                log(f"[uneven] {i:2d} m={d.m} n={d.n_zang} q1={d.q1} NR={cfgs[i].grid.NR} "
-                    f"Ne={cfgs[i].grid.Ne} N={cfgs[i].grid.N_alpha} l={cfgs[i].grid.l_max}: "
-                    f"{'converged' if res.converged else 'not converged'} "
-                    f"Op={res.Omega_p:.13f} g={res.gamma:.13f} host {t_host:.1f}s "
-                    f"W {t_W:.2f}s newton {t_n:.2f}s ({res.n_lp_builds} lp + "
-                    f"{res.n_fp64_builds} fp64 builds)"
-                    + (f" polish move {res.polish_move:.2e}" if polish > 0 else ""))
+    # one context for the whole batch: its W buffers grow to the largest
+    # draw's and are reused, instead of tens of GB allocated and freed per draw
+    ctx = _ffi.Context(GPUBackend.current_device())
+    try:
+        with ThreadPoolExecutor(max_workers=1) as pool:
+            nxt = pool.submit(host, 0) if draws else None
+            for i, d in enumerate(draws):
+                h, t_host = nxt.result()
+                nxt = pool.submit(host, i + 1) if i + 1 < len(draws) else None
+                t0 = time.perf_counter()
+                if orbits == "device":
+                    pre = precompute_device_orbits(cfgs[i], create_toomre_zang_model(cfgs[i]),
+                                                   download_orbits=False, ctx=ctx)
+                else:
+                    pre = precompute_device(cfgs[i], h, ctx=ctx)
+                t_W = time.perf_counter() - t0
+                del h
+                t0 = time.perf_counter()
+                dev = to_device(pre, GPUType.AMD, np.float64)
+                if precision in _LP_COPY:
+                    pre.ctx.prepare(_LP_COPY[precision])
+                t_setup = time.perf_counter() - t0
+                # device-time accounting (one event wait per build, microseconds)
+                pre.ctx.set_profiling(True)
+                r = newton_search(cfgs[i], pre, dev, GPUType.AMD, np.float64, d.Omega_p0, d.gamma0,
+                                  precision=precision)
+                t_n = time.perf_counter() - t0
+                tm = pre.ctx.last_timing()
+                res = DrawResult(d, precision, bool(r.converged), float(r.Omega_p), float(r.gamma),
+                                 float(abs(r.det_val)), int(r.iterations), int(r.n_fp64_builds),
+                                 int(r.n_lp_builds), t_host, t_W, t_n,
+                                 device_ms=float(tm["contract_ms"] + tm["lu_ms"]),
+                                 setup_s=t_setup)
+                if polish > 0 and r.converged:
+                    pr, _ = pre.ctx.newton(r.Omega_p, r.gamma, max_iter=polish, tol=0.0,
+                                           precision=_ffi.PREC_FP64)
+                    res.polished = complex(pr.omega[0], pr.omega[1])
+                    res.polish_move = abs(res.polished - complex(d.m * r.Omega_p, r.gamma))
+                results.append(res)
+                if log:
+                    log(f"[uneven] {i:2d} m={d.m} n={d.n_zang} q1={d.q1} NR={cfgs[i].grid.NR} "
+                        f"Ne={cfgs[i].grid.Ne} N={cfgs[i].grid.N_alpha} l={cfgs[i].grid.l_max}: "
+                        f"{'converged' if res.converged else 'not converged'} "
+                        f"Op={res.Omega_p:.13f} g={res.gamma:.13f} host {t_host:.1f}s "
+                        f"W {t_W:.2f}s newton {t_n:.2f}s ({res.n_lp_builds} lp + "
+                        f"{res.n_fp64_builds} fp64 builds, {res.device_ms / 1e3:.2f}s on device, "
+                        f"setup {t_setup:.2f}s)"
+                        + (f" polish move {res.polish_move:.2e}" if polish > 0 else ""))
+    finally:
+        ctx.close()
     return results
 
 

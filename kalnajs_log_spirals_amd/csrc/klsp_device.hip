@@ -780,7 +780,11 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_det(const double2* __restrict
                 }
                 const double2 piv = Pj[jj];
                 det = cmul(det, piv);
-                const double2 rp = d_inv(piv.x, piv.y);
+                // an exactly zero pivot (the whole column below is zero):
+                // zgetf2 leaves the column unscaled and LinearAlgebra's det
+                // returns 0 for that factorisation; det is 0 from here on
+                const bool zp = piv.x == 0.0 && piv.y == 0.0;
+                const double2 rp = zp ? make_double2(0.0, 0.0) : d_inv(piv.x, piv.y);
                 for (int r = jj + 1 + lane; r < m; r += 64) {
                     const double2 l = cmul(Pj[r], rp);
                     Pj[r] = l;
@@ -882,11 +886,14 @@ struct klsp_ctx {
     int Np = 0, m = 0;
     double G = 0.0;
     double *d_DJ = nullptr, *d_F0l = nullptr, *d_Ores = nullptr, *d_Nk = nullptr;
-    // W
-    double2* d_W64 = nullptr;
-    float2* d_W32 = nullptr;
-    void* d_Wb = nullptr;     // paired bf16 hi/lo rows (BF16x3 builds)
+    // W and its low-precision copies, grow-only (in elements): a new problem
+    // reuses them when it fits, so a batch of problems allocates its largest
+    // W once; klsp_destroy releases them
+    double2* d_W64 = nullptr; size_t W64_cap = 0;
+    float2* d_W32 = nullptr;  size_t W32_cap = 0;
+    uint4* d_Wb = nullptr;    size_t Wb_cap = 0;   // paired bf16 hi/lo rows (BF16x3 builds)
     bool have_W = false;
+    bool have_W32 = false, have_Wb = false;   // copies of the current W
     // workspaces
     double* d_om = nullptr;
     double2* d_coef = nullptr;  // [kModelPts][kMaxW] Taylor-model coefficients
@@ -1163,13 +1170,21 @@ static int free_problem(klsp_ctx* ctx)
 {
     free_balanced(ctx);
     hipFree(ctx->d_DJ); hipFree(ctx->d_F0l); hipFree(ctx->d_Ores); hipFree(ctx->d_Nk);
-    hipFree(ctx->d_W64); hipFree(ctx->d_W32); hipFree(ctx->d_Wb);
     ctx->d_DJ = ctx->d_F0l = ctx->d_Ores = ctx->d_Nk = nullptr;
+    ctx->have_problem = ctx->have_W = ctx->have_arrays = false;
+    ctx->have_W32 = ctx->have_Wb = false;
+    return KLSP_OK;
+}
+
+// The W buffers themselves (kept across problems by free_problem).
+static void release_W(klsp_ctx* ctx)
+{
+    hipFree(ctx->d_W64); hipFree(ctx->d_W32); hipFree(ctx->d_Wb);
     ctx->d_W64 = nullptr;
     ctx->d_W32 = nullptr;
     ctx->d_Wb = nullptr;
-    ctx->have_problem = ctx->have_W = ctx->have_arrays = false;
-    return KLSP_OK;
+    ctx->W64_cap = ctx->W32_cap = ctx->Wb_cap = 0;
+    ctx->have_W = ctx->have_W32 = ctx->have_Wb = false;
 }
 
 extern "C" {
@@ -1243,6 +1258,7 @@ int klsp_destroy(klsp_ctx* ctx)
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     free_problem(ctx);
+    release_W(ctx);
     hipFree(ctx->d_om); hipFree(ctx->d_c); hipFree(ctx->d_P); hipFree(ctx->d_M);
     hipFree(ctx->d_A); hipFree(ctx->d_det); hipFree(ctx->d_coef); hipFree(ctx->d_cf);
     for (auto& e : ctx->ev) hipEventDestroy(e);
@@ -1307,15 +1323,17 @@ try {
     return klsp::exception_status(ctx ? &ctx->err : nullptr);
 }
 
+// Device W of the current problem, zeroed (the padding rows and columns
+// stay zero).  A W that does not fit the buffer releases the low-precision
+// copies too before the larger one is allocated.
 static int alloc_W64(klsp_ctx* ctx)
 {
-    if (ctx->d_W32) { KLSP_HIP(ctx, hipFree(ctx->d_W32)); ctx->d_W32 = nullptr; }
-    if (ctx->d_Wb) { KLSP_HIP(ctx, hipFree(ctx->d_Wb)); ctx->d_Wb = nullptr; }
-    if (!ctx->d_W64)
-        KLSP_HIP(ctx, hipMalloc((void**)&ctx->d_W64,
-                                sizeof(double2) * (ctx->Kp + kRowPad) * ctx->Np));
-    KLSP_HIP(ctx, hipMemsetAsync(ctx->d_W64, 0, sizeof(double2) * (ctx->Kp + kRowPad) * ctx->Np,
-                                 ctx->stream));
+    const size_t n = (size_t)(ctx->Kp + kRowPad) * ctx->Np;
+    ctx->have_W = ctx->have_W32 = ctx->have_Wb = false;
+    if (n > ctx->W64_cap) release_W(ctx);
+    int rc = ensure(ctx, ctx->d_W64, ctx->W64_cap, n);
+    if (rc) return rc;
+    KLSP_HIP(ctx, hipMemsetAsync(ctx->d_W64, 0, sizeof(double2) * n, ctx->stream));
     return KLSP_OK;
 }
 
@@ -1568,21 +1586,25 @@ try {
         precision != KLSP_PREC_BF16X3)
         return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_prepare: unknown precision");
     if (!ctx->have_W) return fail(ctx, KLSP_ERR_NOT_READY, "klsp_prepare: no W on device");
-    if (precision == KLSP_PREC_FP32 && !ctx->d_W32) {
+    int rc;
+    if (precision == KLSP_PREC_FP32 && !ctx->have_W32) {
         KLSP_HIP(ctx, hipSetDevice(ctx->device));
         const int64_t n = (ctx->Kp + kRowPad) * ctx->Np;
-        KLSP_HIP(ctx, hipMalloc((void**)&ctx->d_W32, sizeof(float2) * n));
+        if ((rc = ensure(ctx, ctx->d_W32, ctx->W32_cap, (size_t)n))) return rc;
         hipLaunchKernelGGL(k_W64_to_32, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
                            ctx->d_W64, n, ctx->d_W32);
         KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->have_W32 = true;
     }
-    if (precision == KLSP_PREC_BF16X3 && !ctx->d_Wb) {
+    if (precision == KLSP_PREC_BF16X3 && !ctx->have_Wb) {
         KLSP_HIP(ctx, hipSetDevice(ctx->device));
         const int64_t nq = round_up(ctx->Kp, 16) / 4;
-        KLSP_HIP(ctx, hipMalloc(&ctx->d_Wb, (size_t)2 * 16 * nq * ctx->Np));
+        // two planes (hi, lo) of nq x Np 16-byte groups of four K rows
+        if ((rc = ensure(ctx, ctx->d_Wb, ctx->Wb_cap, (size_t)2 * nq * ctx->Np))) return rc;
         KLSP_HIP(ctx, split_W_bf16(ctx->d_W64, ctx->Kp + kRowPad, ctx->Np, nq, ctx->d_Wb,
                                    ctx->stream));
         KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->have_Wb = true;
     }
     return KLSP_OK;
 } catch (...) {
@@ -1736,8 +1758,8 @@ static int check_ready(klsp_ctx* ctx, int precision, const char* who)
     if (precision != KLSP_PREC_FP64 && precision != KLSP_PREC_FP32 &&
         precision != KLSP_PREC_BF16X3)
         return fail(ctx, KLSP_ERR_INVALID_ARG, std::string(who) + ": unknown precision");
-    if (precision == KLSP_PREC_FP32 && !ctx->d_W32) return klsp_prepare(ctx, precision);
-    if (precision == KLSP_PREC_BF16X3 && !ctx->d_Wb) return klsp_prepare(ctx, precision);
+    if (precision == KLSP_PREC_FP32 && !ctx->have_W32) return klsp_prepare(ctx, precision);
+    if (precision == KLSP_PREC_BF16X3 && !ctx->have_Wb) return klsp_prepare(ctx, precision);
     return KLSP_OK;
 }
 

@@ -279,6 +279,52 @@ def test_lu_dense_pivoting(N):
     assert np.count_nonzero(piv != np.arange(N)) >= N // 3
 
 
+@pytest.mark.parametrize("N", [8, 37, 301])
+def test_lu_identity_diagonal_and_singular(N):
+    """Special cases of the device LU on matrices built through the normal
+    path: F0l = 0 gives M = 0 and det(I) = 1 exactly; W diagonal
+    (W[k, a] = w_a delta_ka, one l) gives a diagonal M and
+    det(I - M) = prod_a (1 - G N_a c_a w_a^2) in closed form; with
+    c = N = G = w = 1 on one diagonal entry (every operation exact) I - M
+    has an exactly zero column, where LAPACK's zgetrf reports a zero pivot
+    and LinearAlgebra's det returns exactly 0 (MatrixBuilder.jl:85)."""
+    rng = np.random.default_rng(N)
+    om = 1.5 + 0.0j
+    G = 1.0
+    Ores = rng.uniform(-2.0, 0.0, (N, 1))
+    DJ = rng.uniform(0.2, 1.0, N)
+    Nk = rng.uniform(0.5, 1.5, N)
+    w = rng.uniform(0.5, 1.5, N)
+    W = np.zeros((N, N, 1), dtype=np.complex128)
+    W[np.arange(N), np.arange(N), 0] = w
+    # identity: F0l = 0
+    pre = from_arrays(2, G, W, DJ, np.zeros((N, 1)), Ores, Nk)
+    d = pre.ctx.det([om], _ffi.PREC_FP64)[0]
+    pre.ctx.close()
+    assert d == 1.0, d
+    # diagonal, closed form
+    F0l = rng.uniform(-0.5, 0.5, (N, 1))
+    pre = from_arrays(2, G, W, DJ, F0l, Ores, Nk)
+    M = pre.ctx.build_M(om, _ffi.PREC_FP64)
+    c = DJ * F0l[:, 0] / (om - Ores[:, 0])
+    diag = G * Nk * c * w * w
+    assert np.count_nonzero(M - np.diag(np.diag(M))) == 0
+    np.testing.assert_allclose(np.diag(M), diag, rtol=1e-14, atol=0)
+    d = pre.ctx.det([om], _ffi.PREC_FP64)[0]
+    pre.ctx.close()
+    ref = np.prod(1.0 - diag)
+    assert abs(d - ref) <= 4 * N * np.finfo(float).eps * abs(ref), (d, ref)
+    # exactly singular: entry a has d = om - Ores = 1, DJ = F0l = N = w = 1
+    a = N // 3
+    Ores[a, 0], DJ[a], F0l[a, 0], Nk[a], W[a, a, 0] = 0.5, 1.0, 1.0, 1.0, 1.0
+    pre = from_arrays(2, G, W, DJ, F0l, Ores, Nk)
+    M = pre.ctx.build_M(om, _ffi.PREC_FP64)
+    assert M[a, a] == 1.0
+    d = pre.ctx.det([om], _ffi.PREC_FP64)[0]
+    pre.ctx.close()
+    assert d == 0.0, d
+
+
 def test_resonance_clamp():
     """|omega - Omega_res| < 1e-12 -> denominator 1e-12 + 1e-12 i (MatrixBuilder.jl:47)."""
     rng = np.random.default_rng(7)
@@ -571,3 +617,46 @@ def test_bf16x3_four_taylor_terms(oracle_small):
     assert relnorm(Tb[3], T4) <= 1e-4
     with pytest.raises(_ffi.KlspError):
         pre.ctx.build_taylor(wc, 4, _ffi.PREC_FP64)
+
+
+def test_context_reuse_across_problems():
+    """One context given problems of different sizes in turn (as the uneven
+    batch does): its W buffers and low-precision copies are reused when the
+    new W fits and replaced when it does not, and every det equals the one a
+    fresh context gives (same kernels, same data: bit-identical), in FP64,
+    FP32 and BF16x3."""
+    rng = np.random.default_rng(5)
+
+    def problem(NPh, N, n_l):
+        W = (rng.standard_normal((NPh, N, n_l)) + 1j * rng.standard_normal((NPh, N, n_l)))
+        W *= 0.5 / np.sqrt(NPh * n_l)
+        return dict(NPh=NPh, N=N, n_l=n_l, W=W, DJ=rng.uniform(0.1, 1, NPh),
+                    F0l=rng.standard_normal((NPh, n_l)),
+                    Ores=rng.uniform(-2, 2, (NPh, n_l)), Nk=rng.uniform(0.5, 1.5, N))
+
+    def load(ctx, q):
+        ctx.upload_problem(q["NPh"], q["N"], q["n_l"], 2, 0.7, q["DJ"], q["F0l"], q["Ores"],
+                           q["Nk"])
+        ctx.upload_W(q["W"])
+
+    oms = [0.3 + 0.4j, 0.9 + 0.1j, 0.5 + 0.25j]
+    precs = (_ffi.PREC_FP64, _ffi.PREC_FP32, _ffi.PREC_BF16X3)
+    # large, smaller (reuse), larger than both (new buffers), small again
+    probs = [problem(700, 96, 5), problem(300, 40, 3), problem(900, 130, 6),
+             problem(65, 17, 2)]
+    shared = _ffi.Context(0)
+    try:
+        for q in probs:
+            load(shared, q)
+            got = {p: shared.det(oms, p) for p in precs}
+            fresh = _ffi.Context(0)
+            try:
+                load(fresh, q)
+                for p in precs:
+                    ref = fresh.det(oms, p)
+                    print("problem", q["NPh"], q["N"], q["n_l"], "precision", p, flush=True)
+                    assert np.array_equal(got[p], ref), (p, got[p], ref)
+            finally:
+                fresh.close()
+    finally:
+        shared.close()
