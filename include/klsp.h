@@ -180,6 +180,9 @@ typedef struct klsp_problem {
     const double* N_kernel; /* [N]                                          */
 } klsp_problem;
 
+/* Replaces the context's problem (dimensions, arrays; W must be given
+ * again).  The W buffers are kept and reused when the new W fits (a batch of
+ * problems then allocates its largest W once); klsp_destroy releases them. */
 int klsp_upload_problem(klsp_ctx* ctx, const klsp_problem* prob);
 
 /* DJ [NPh], F0l and Omega_res [NPh, n_l] of an uploaded problem (dimensions
