@@ -2261,10 +2261,13 @@ try {
             om[2 * x + 1] = gg[i];
         }
     if ((rc = det_many(ctx, om.data(), n, sp->precision, det_grid))) return rc;
+    // find_minimum (GridScan.jl:234-246): argmin |det| in column-major
+    // order, first occurrence; Julia's argmin takes NaN as the minimum
     int64_t best = 0;
     double bv = INFINITY;
     for (int64_t x = 0; x < n; ++x) {
         const double v = std::hypot(det_grid[2 * x], det_grid[2 * x + 1]);
+        if (std::isnan(v)) { best = x; break; }
         if (v < bv) { bv = v; best = x; }
     }
     if (Op_init) *Op_init = Opg[best / sp->n_gamma];

@@ -88,3 +88,21 @@ def test_julia_value_formatting():
     j = run_kalnajs._jl
     assert (j(1e-12), j(100.0), j(0.5), j(2.5e20), j(True), j(False), j(7)) == (
         "1.0e-12", "100.0", "0.5", "2.5e20", "true", "false", "7")
+
+
+def test_find_minimum_column_major_and_nan():
+    """find_minimum (GridScan.jl:234-246): argmin |det| over det_grid[n_gamma,
+    n_Op] in column-major order, first occurrence on ties, and NaN counted as
+    the minimum, as Julia's argmin does ("NaN is treated as less than all
+    other values except missing")."""
+    import numpy as np
+    from kalnajs_log_spirals_amd.GridScan import find_minimum
+    Op = np.array([0.2, 0.3, 0.4])
+    g = np.array([0.01, 0.02])
+    d = np.array([[3.0, 1.0, 1.0], [2.0, 5.0, 0.5]], dtype=complex)
+    assert find_minimum(d, Op, g) == (0.4, 0.02)
+    d[1, 2] = 1.0                      # tie at (0, 1), (0, 2), (1, 2): first column-major
+    assert find_minimum(d, Op, g) == (0.3, 0.01)
+    d[1, 1] = complex(np.nan, 0.0)     # NaN wins over the smaller finite values
+    d[0, 2] = 0.1
+    assert find_minimum(d, Op, g) == (0.3, 0.02)
