@@ -266,9 +266,19 @@ typedef struct klsp_newton_params {
 typedef struct klsp_newton_result {
     double Omega_p, gamma;
     double omega[2];       /* m*Omega_p + i*gamma                           */
-    double det[2];         /* det at the final (Omega_p, gamma)             */
+    double det[2];         /* det at the final (Omega_p, gamma): a fresh
+                              det in that precision for FP64 / FP32 /
+                              BF16X3; for MIXED / MIXED_FP32 the anchored
+                              model's det there (M64 at the last anchor plus
+                              the low-precision Taylor corrections, which
+                              enter multiplied by |s - s_a|), not a new
+                              FP64 build                                   */
     int iterations;
-    int converged;
+    int converged;         /* FP64 / FP32 / BF16X3: |det| < tol (the
+                              reference's test); MIXED: the last anchor's
+                              Newton loop ended on its step test, the zero
+                              lay within that anchor's trust radius, and
+                              |det| < tol                                  */
     int n_det_evals;       /* number of M(omega) builds performed           */
     int n_fp64_builds;     /* ... of which FP64 single-omega builds         */
     int n_lp_builds;       /* ... of which FP32 builds (MIXED: Taylor passes) */

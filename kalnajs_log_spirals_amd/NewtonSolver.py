@@ -9,7 +9,11 @@ difference pair of each step evaluated as one two-omega batch.
 mixed-precision search: BF16x3 passes ("mixed_fp32": FP32 passes) that each
 build the first four (FP32: three) Taylor terms of M around a centre and run Newton on
 that model, then one FP64 build that anchors the model for the FP64 finish;
-it returns the FP64 root to |d omega| <= 1e-10 with one FP64 build.
+it returns the FP64 root to |d omega| <= 1e-10 with one FP64 build.  In
+that mode ``det_val`` is the anchored model's det at the returned omega (the
+last FP64 build plus the low-precision Taylor corrections, which enter
+multiplied by the distance from that anchor), not a fresh FP64 build; the
+reference recomputes it with a full evaluation (NewtonSolver.jl:157-158).
 """
 from __future__ import annotations
 

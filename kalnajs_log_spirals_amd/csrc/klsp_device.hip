@@ -751,6 +751,10 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_det(const double2* __restrict
         if (wid == 0) {
             for (int jj = 0; jj < kb; ++jj) {
                 double2* Pj = P + jj * m;
+                // izamax: the first entry (the diagonal) seeds the search and
+                // a later entry replaces it only if strictly larger, so NaN
+                // entries below the diagonal are never chosen, while a NaN
+                // diagonal is the pivot and propagates into det
                 double bv = -1.0;
                 int bi = m;
                 for (int r = jj + lane; r < m; r += 64) {
@@ -763,7 +767,13 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_det(const double2* __restrict
                     const int oi = __shfl_xor(bi, off);
                     argmax_merge(bv, bi, ov, oi);
                 }
-                if (!(bv > 0.0)) {
+                {
+                    const double2 d = Pj[jj];
+                    if (isnan(fabs(d.x) + fabs(d.y))) { bv = NAN; bi = jj; }
+                }
+                if (bv == 0.0) {
+                    // an exactly zero pivot: zgetf2 reports info > 0 and
+                    // LinearAlgebra's det(::LU) returns 0 for that factorisation
                     singular = true;
                     if (lane == 0) pv[jj] = jj;
                     continue;
@@ -778,13 +788,15 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_det(const double2* __restrict
                         P[lane * m + p] = tmp;
                     }
                 }
+                // lane t swapped column t; every lane reads the swapped rows
+                // below.  The wave's LDS accesses complete in order, but
+                // without this fence the compiler may hand a lane that took
+                // no part in the swap a value it loaded before it (Pj[jj]).
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
                 const double2 piv = Pj[jj];
                 det = cmul(det, piv);
-                // an exactly zero pivot (the whole column below is zero):
-                // zgetf2 leaves the column unscaled and LinearAlgebra's det
-                // returns 0 for that factorisation; det is 0 from here on
-                const bool zp = piv.x == 0.0 && piv.y == 0.0;
-                const double2 rp = zp ? make_double2(0.0, 0.0) : d_inv(piv.x, piv.y);
+                const double2 rp = d_inv(piv.x, piv.y);
                 for (int r = jj + 1 + lane; r < m; r += 64) {
                     const double2 l = cmul(Pj[r], rp);
                     Pj[r] = l;
@@ -2010,6 +2022,10 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
     cd D(NAN, NAN);
     int conv = 0;
     for (int anchor = 0; anchor < std::max(1, np->max_iter); ++anchor) {
+        // converged is decided by the last anchor alone: its Newton loop must
+        // end on the step test with the zero inside anchor_tol of the anchor
+        conv = 0;
+        bool stepped = false;
         {
             const double om[2] = {w.real(), w.imag()};
             if ((rc = build_batch(ctx, om, 1, KLSP_PREC_FP64))) return rc;   // d_M[0]
@@ -2057,7 +2073,7 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
             // converged: a step below 1e-13 |w|, or no longer shrinking (the
             // rounding floor of the FP64 dets)
             if (step <= 1e-13 * std::max(1.0, std::abs(wn)) || (it >= 2 && step > 0.5 * prev_step)) {
-                conv = std::abs(D) < np->tol;
+                stepped = true;
                 break;
             }
             prev_step = step;
@@ -2070,7 +2086,10 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
             if (std::abs(s - sa) > 0.25 * wc.imag()) break;
         }
         w = wc + s;
-        if (!far) break;
+        if (!far) {
+            conv = stepped && std::abs(D) < np->tol;
+            break;
+        }
     }
     record(w, std::abs(D));
     res->Omega_p = w.real() / m;

@@ -622,11 +622,15 @@ int or_build_M(long NPh, long N, long n_l, const double* W, const double* DJ,
 /* det(A) by LU with partial pivoting, LAPACK zgetf2 semantics: pivot =      */
 /* first row maximising |re|+|im| (izamax), reciprocal-scaled column,        */
 /* rank-1 update; det = prod(diag U) * (-1)^swaps in Julia's order           */
-/* (LinearAlgebra det(::LU)).  A is complex [N, N] column-major, in place.   */
+/* (LinearAlgebra det(::LU)), and exactly 0 when a pivot is exactly zero     */
+/* (zgetf2's info > 0, for which det(::LU) returns zero(T)).  A NaN on the   */
+/* diagonal is the pivot (izamax seeds its search with the first entry and   */
+/* takes a later one only if strictly larger), so it reaches det.            */
+/* A is complex [N, N] column-major, in place.                               */
 /* ------------------------------------------------------------------------ */
 int or_det_inplace(long N, double* A, double* det_re, double* det_im)
 {
-    int swaps = 0;
+    int swaps = 0, singular = 0;
 #define AR(i, j) A[2 * ((i) + N * (j))]
 #define AI(i, j) A[2 * ((i) + N * (j)) + 1]
     for (long j = 0; j < N; ++j) {
@@ -636,7 +640,7 @@ int or_det_inplace(long N, double* A, double* det_re, double* det_im)
             double v = fabs(AR(i, j)) + fabs(AI(i, j));
             if (v > best) { best = v; p = i; }
         }
-        if (AR(p, j) == 0.0 && AI(p, j) == 0.0) continue;   /* singular column */
+        if (AR(p, j) == 0.0 && AI(p, j) == 0.0) { singular = 1; continue; }
         if (p != j) {
             ++swaps;
             for (long c = 0; c < N; ++c) {
@@ -662,6 +666,7 @@ int or_det_inplace(long N, double* A, double* det_re, double* det_im)
     cplx d = c_mk(1.0, 0.0);
     for (long i = 0; i < N; ++i) d = c_mul(d, c_mk(AR(i, i), AI(i, i)));
     if (swaps & 1) d = c_mk(-d.re, -d.im);
+    if (singular) d = c_mk(0.0, 0.0);
 #undef AR
 #undef AI
     *det_re = d.re;

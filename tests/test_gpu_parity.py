@@ -660,3 +660,58 @@ def test_context_reuse_across_problems():
                 fresh.close()
     finally:
         shared.close()
+
+
+# --------------------------------------------------------------------------
+# Non-finite input and the convergence flag of the mixed search
+# --------------------------------------------------------------------------
+def test_nan_in_W_gives_nan_det_and_no_convergence():
+    """A NaN in W reaches row and column a of M, so I - M has a NaN on its
+    diagonal: zgetrf's izamax takes it as the pivot and det is NaN (as
+    LinearAlgebra/numpy and the oracle give), not 0; neither the FP64 nor the
+    mixed Newton search may report convergence there."""
+    rng = np.random.default_rng(11)
+    NPh, N, n_l = 40, 17, 2
+    W = (rng.standard_normal((NPh, N, n_l)) + 1j * rng.standard_normal((NPh, N, n_l))) / 10
+    W[5, 3, 1] = complex(np.nan, 0.0)
+    DJ, F0l = rng.uniform(0.5, 1.5, NPh), rng.standard_normal((NPh, n_l))
+    Ores, Nk = rng.uniform(-1, 1, (NPh, n_l)), rng.uniform(0.5, 1.0, N)
+    pre_o = dict(m=2, G=1.0, W=W, DJ=DJ, F0l=F0l, Omega_res=Ores, N_kernel=Nk)
+    om = 0.8 + 0.12j
+    Mo = oracle.build_M(pre_o, om)
+    do = oracle.det_matrix(np.eye(N) - Mo)
+    assert np.isnan(do) and np.isnan(np.linalg.det(np.eye(N) - Mo))
+    pre = device_pre(pre_o)
+    try:
+        d = pre.ctx.det([om], _ffi.PREC_FP64)[0]
+        print("det with a NaN in W:", d, "oracle:", do, flush=True)
+        assert np.isnan(d)
+        for prec in (_ffi.PREC_FP64, _ffi.PREC_MIXED):
+            res, _ = pre.ctx.newton(0.4, 0.12, max_iter=5, tol=1e-12, precision=prec)
+            assert res.converged == 0, prec
+    finally:
+        pre.ctx.close()
+
+
+@pytest.mark.parametrize("max_iter", [1, 2, 3, 50])
+@pytest.mark.parametrize("start", [(0.30, 0.20), (0.55, 0.05), (0.44, 0.125)])
+def test_mixed_newton_converged_flag_is_honest(oracle_small, start, max_iter):
+    """converged from the mixed search means the returned omega is an FP64
+    root: three FP64 Newton steps with tol = 0 move it by at most 1e-10 and
+    its FP64 |det| is below 10 tol (the det's rounding floor there is
+    ~1e-13).  Far starts with small iteration budgets
+    end between anchors, where the flag must be 0."""
+    pre = device_pre(oracle_small)
+    try:
+        res, _ = pre.ctx.newton(*start, max_iter=max_iter, tol=1e-12, precision=_ffi.PREC_MIXED)
+        w = complex(res.omega[0], res.omega[1])
+        print("start", start, "max_iter", max_iter, "converged", res.converged, "omega", w,
+              flush=True)
+        if res.converged:
+            d64 = pre.ctx.det([w], _ffi.PREC_FP64)[0]
+            pol, _ = pre.ctx.newton(res.Omega_p, res.gamma, max_iter=3, tol=0.0,
+                                    precision=_ffi.PREC_FP64)
+            assert abs(complex(pol.omega[0], pol.omega[1]) - w) <= 1e-10
+            assert abs(d64) < 1e-12 * 10
+    finally:
+        pre.ctx.close()

@@ -307,6 +307,26 @@ def test_det_against_lapack_and_mpmath():
     assert oracle.det_matrix(S) == 0
 
 
+def test_det_non_finite_entries_follow_lapack():
+    """izamax seeds its search with the first (diagonal) entry and takes a
+    later one only if strictly larger, so a NaN is never chosen from below the
+    diagonal but is the pivot on it; either way the NaN reaches det, as
+    LAPACK (numpy) gives.  An exactly zero pivot returns exactly 0 even when
+    the rest of the matrix is non-finite (zgetf2's info > 0: LinearAlgebra's
+    det(::LU) returns zero(T)), as numpy does too."""
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((6, 6)) + 1j * rng.standard_normal((6, 6))
+    for (i, j) in ((0, 0), (3, 3), (4, 1), (5, 0), (0, 5)):
+        B = A.copy()
+        B[i, j] = complex(np.nan, 0.0)
+        assert np.isnan(np.linalg.det(B)), (i, j)
+        assert np.isnan(oracle.det_matrix(B)), (i, j)
+    B = A.copy()
+    B[:, 0] = 0.0
+    B[2, 4] = complex(np.inf, np.nan)
+    assert np.linalg.det(B) == 0 and oracle.det_matrix(B) == 0
+
+
 # --------------------------------------------------------------------------
 # Newton driver (NewtonSolver.jl:36-172) and grid scan (GridScan.jl)
 # --------------------------------------------------------------------------
