@@ -332,6 +332,45 @@ typedef struct klsp_scan_params {
 int klsp_grid_scan(klsp_ctx* ctx, const klsp_scan_params* sp, double* det_grid_out,
                    double* Omega_p_init_out, double* gamma_init_out);
 
+/* Real basis of a symmetric alpha grid (no counterpart in the reference;
+ * the arithmetic of MatrixBuilder.jl:119-141 rearranged).  Where W[:, N-1-a]
+ * = conj W[:, a] for every column a and the centre column of an odd N is
+ * real -- as on the reference's grid range(-alpha_max, alpha_max, N), where
+ * W_l(-alpha) = conj W_l(alpha) -- W = R T with R = [Re W(alpha >= 0) |
+ * Im W(alpha > 0)] real (K x N) and T fixed, so that one-omega FP64 builds
+ * take two real symmetric products R^T diag(Re c) R and R^T diag(Im c) R
+ * (about a quarter of the complex product's MFMA work; same results to
+ * rounding).  W from klsp_compute_W / klsp_compute_orbits_W on such a grid
+ * is symmetric by construction; an uploaded W is checked value by value
+ * when first needed.  on = 0 makes every build use the complex product
+ * (default 1). */
+int klsp_set_symmetric_basis(klsp_ctx* ctx, int on);
+
+/* *active_out = 1 if one-omega FP64 builds of the current W use the real
+ * basis (checks the symmetry of an uploaded W; needs W on the device). */
+int klsp_symmetric_basis(klsp_ctx* ctx, int* active_out);
+
+/* Autotuner (no counterpart in the reference).  Times every compiled
+ * variant of the contraction that one-omega `precision` builds of the current
+ * problem use -- FP64 on a symmetric alpha grid: the real-basis kernel's
+ * prefetch depth and the form of its diagonal tiles; one form elsewhere --
+ * with one untimed and `reps` timed builds each, keeps the fastest for the
+ * context's later builds (and for later problems of the same padded width
+ * on this context), and returns the variant count in *n_variants, their
+ * mean contraction times in ms_out[0 .. min(count, n_max)) and the index of
+ * the fastest in *chosen (ms_out and chosen may be NULL with n_max = 0). */
+int klsp_autotune(klsp_ctx* ctx, int precision, int reps, int* n_variants, double* ms_out,
+                  int n_max, int* chosen);
+
+/* Name of variant `variant` of `precision` builds ("" if none). */
+const char* klsp_variant_name(int precision, int variant);
+
+/* Pins variant `variant` (as numbered by klsp_autotune) for the context's
+ * later `precision` builds, e.g. a choice saved from an earlier tuning run.
+ * Only FP64 (real-basis) builds have variants; anything else is
+ * KLSP_ERR_INVALID_ARG. */
+int klsp_set_variant(klsp_ctx* ctx, int precision, int variant);
+
 /* ------------------------------------------------------------------ */
 /* Measurement                                                         */
 /* ------------------------------------------------------------------ */

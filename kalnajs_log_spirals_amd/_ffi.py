@@ -45,7 +45,10 @@ EXPORTED = [
     "klsp_compute_jacobian_df", "klsp_orbit_phase_grid", "klsp_precompute_arrays",
     "klsp_upload_problem", "klsp_set_problem_arrays", "klsp_compute_orbits_W",
     "klsp_compute_W", "klsp_upload_W", "klsp_download_W", "klsp_prepare", "klsp_build_M",
-    "klsp_build_taylor", "klsp_det", "klsp_newton", "klsp_grid_scan", "klsp_set_profiling",
+    "klsp_build_taylor", "klsp_det", "klsp_newton", "klsp_grid_scan",
+    "klsp_set_symmetric_basis", "klsp_symmetric_basis", "klsp_autotune", "klsp_variant_name",
+    "klsp_set_variant",
+    "klsp_set_profiling",
     "klsp_last_timing", "klsp_last_W_ms", "klsp_last_orbit_ms",
 ]
 
@@ -149,12 +152,19 @@ def lib() -> ct.CDLL:
     L.klsp_newton.argtypes = [P, ct.POINTER(klsp_newton_params),
                               ct.POINTER(klsp_newton_result), _D]
     L.klsp_grid_scan.argtypes = [P, ct.POINTER(klsp_scan_params), _D, _D, _D]
+    L.klsp_autotune.argtypes = [P, ct.c_int, ct.c_int, ct.POINTER(ct.c_int), _D, ct.c_int,
+                                ct.POINTER(ct.c_int)]
+    L.klsp_set_variant.argtypes = [P, ct.c_int, ct.c_int]
+    L.klsp_variant_name.restype = ct.c_char_p
+    L.klsp_variant_name.argtypes = [ct.c_int, ct.c_int]
+    L.klsp_set_symmetric_basis.argtypes = [P, ct.c_int]
+    L.klsp_symmetric_basis.argtypes = [P, ct.POINTER(ct.c_int)]
     L.klsp_set_profiling.argtypes = [P, ct.c_int]
     L.klsp_last_timing.argtypes = [P, ct.POINTER(klsp_timing)]
     L.klsp_last_W_ms.argtypes = [P, _D]
     L.klsp_last_orbit_ms.argtypes = [P, _D]
     for name in EXPORTED:
-        if name not in ("klsp_version", "klsp_last_error"):
+        if name not in ("klsp_version", "klsp_last_error", "klsp_variant_name"):
             getattr(L, name).restype = ct.c_int
     _lib = L
     return L
@@ -361,6 +371,31 @@ class Context:
         self._chk(lib().klsp_grid_scan(self.handle, ct.byref(sp), dptr(dg), ct.byref(o1),
                                        ct.byref(o2)))
         return dg.view(np.complex128).reshape((n_gamma, n_Op), order="F"), o1.value, o2.value
+
+    def set_symmetric_basis(self, on: bool):
+        """Real-basis FP64 builds where W is alpha-mirror symmetric (default on)."""
+        self._chk(lib().klsp_set_symmetric_basis(self.handle, int(bool(on))))
+
+    def autotune(self, precision: int = PREC_FP64, reps: int = 2):
+        """Times the contraction variants of one-omega `precision` builds and
+        keeps the fastest; returns ({name: mean ms}, chosen name)."""
+        n = ct.c_int(0)
+        ch = ct.c_int(0)
+        ms = np.zeros(16)
+        self._chk(lib().klsp_autotune(self.handle, int(precision), int(reps), ct.byref(n),
+                                      dptr(ms), ms.size, ct.byref(ch)))
+        names = [lib().klsp_variant_name(int(precision), v).decode() or "default"
+                 for v in range(n.value)]
+        return {names[v]: float(ms[v]) for v in range(n.value)}, names[ch.value]
+
+    def set_variant(self, variant: int, precision: int = PREC_FP64):
+        self._chk(lib().klsp_set_variant(self.handle, int(precision), int(variant)))
+
+    def symmetric_basis(self) -> bool:
+        """True if one-omega FP64 builds of the current W use the real basis."""
+        v = ct.c_int(0)
+        self._chk(lib().klsp_symmetric_basis(self.handle, ct.byref(v)))
+        return bool(v.value)
 
     # -- measurement ---------------------------------------------------------
     def set_profiling(self, on: bool):

@@ -31,12 +31,14 @@ DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-pa
 # ~8 accumulator blocks per wave keep them in VGPRs (-amdgpu-mfma-vgpr-form):
 # with the default AGPR form the compiler shuffles some of them between the
 # two register files every loop trip, and those moves take issue slots from
-# the MFMAs.  The one-omega FP64 contraction (16 blocks) fills the AGPRs and
-# keeps the default form.
+# the MFMAs.  The one-omega FP64 contraction (16 complex blocks) and the
+# symmetric-basis FP64 contraction (32 real blocks: P and Q of a 4 x 4 tile)
+# fill the AGPRs and keep the default form.
 SOURCES = [
     ("klsp_host.cpp", False, []),
     ("klsp_device.hip", True, []),
     ("klsp_contract_f64.hip", True, []),
+    ("klsp_contract_sym64.hip", True, []),
     ("klsp_contract_f64_batch.hip", True, ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
     ("klsp_contract_f32.hip", True, ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
     ("klsp_contract_bf16.hip", True, ["-mllvm", "-amdgpu-mfma-vgpr-form"]),

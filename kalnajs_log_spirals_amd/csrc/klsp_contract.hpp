@@ -384,6 +384,20 @@ ContractShape contract_f32_shape(int nw);
 hipError_t contract_f64_launch(int nw, const ContractArgs& a, hipStream_t stream);
 hipError_t contract_f32_launch(int nw, const ContractArgs& a, hipStream_t stream);
 
+// FP64 contraction in the real basis of a symmetric alpha grid
+// (klsp_contract_sym64.hip): the upper-triangle tiles of P = R^T diag(Re c) R
+// and Q = R^T diag(Im c) R for one omega, partial slice P + iQ per K piece;
+// square tiles of shape().MB blocks, work units from a balanced table whose
+// K pieces are multiples of quantum() rows.  `variant` (0 .. variants() - 1)
+// selects a compiled form: prefetch depth and whether diagonal tiles compute
+// all their blocks (full_diag) or the upper triangle only.
+int contract_sym64_variants();
+ContractShape contract_sym64_shape(int variant);
+int contract_sym64_quantum(int variant);
+bool contract_sym64_full_diag(int variant);
+const char* contract_sym64_name(int variant);
+hipError_t contract_sym64_launch(int variant, const ContractArgs& a, hipStream_t stream);
+
 // BF16 three-term contraction (klsp_contract_bf16.hip): workgroups of
 // kBf16Waves waves stacked along beta, kBf16MB blocks per wave, kBf16NB alpha
 // blocks per workgroup; K ranges per gang are multiples of 16 rows.

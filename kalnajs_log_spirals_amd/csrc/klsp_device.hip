@@ -21,6 +21,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -473,16 +474,19 @@ __device__ __forceinline__ void w_consume(
     if (!on) return;
     const int a = ac0 + 16 * ab + i;
     if (a >= N) return;
-    // symmetric alpha grid: W(-alpha) = conj W(alpha) (the l-side factor is real)
+    // symmetric alpha grid: W(-alpha) = conj W(alpha) (the l-side factor is
+    // real); its centre column is alpha = 0 exactly, where e^(-i alpha ln r)
+    // = 1 and W is real (the rotated sines only approximate that 0)
     const int am = N - 1 - a;
     const bool wm = mirror && am < a;
+    const bool centre = mirror && am == a;
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
         for (int v = 0; v < 4; ++v) {
             const int li = lc0 + 16 * (lq0 + q) + kk + 4 * v;   // f64 D layout: row = lane/16 + 4v
             if (li < n_l) {
                 double2* row = W + ((int64_t)li * NPh + k) * Np;
-                row[a] = make_double2(accr[q][v], acci[q][v]);
+                row[a] = make_double2(accr[q][v], centre ? 0.0 : acci[q][v]);
                 if (wm) row[am] = make_double2(accr[q][v], -acci[q][v]);
             }
         }
@@ -553,6 +557,39 @@ __global__ void k_W64_to_32(const double2* __restrict__ W, int64_t n, float2* __
          x += (int64_t)gridDim.x * blockDim.x) {
         const double2 w = W[x];
         Wf[x] = make_float2((float)w.x, (float)w.y);
+    }
+}
+
+// Mirror symmetry of W over its alpha columns, W[:, N-1-a] = conj W[:, a]
+// with a real centre column (odd N), checked value by value over `rows`
+// rows: any violation (NaN included) stores 1 in *flag.
+__global__ void k_W_mirror_check(const double2* __restrict__ W, int64_t rows, int N, int Np,
+                                 int* __restrict__ flag)
+{
+    const int h = N >> 1;
+    const int64_t n = rows * (N - h);
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
+         x += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = x / (N - h);
+        const int a = h + (int)(x - k * (N - h));   // upper half, centre included
+        const double2 w = W[k * Np + a], wm = W[k * Np + (N - 1 - a)];
+        if (!(wm.x == w.x && wm.y == -w.y)) *flag = 1;   // the centre: w.y == -w.y iff 0
+    }
+}
+
+// The real basis of a symmetric alpha grid: R[k][j] = Re W[k][h + j] for
+// j < nU = N - h, Im W[k][j] for nU <= j < N, 0 in the padding columns.
+__global__ void k_W64_to_R(const double2* __restrict__ W, int64_t rows, int N, int Np,
+                           double* __restrict__ R)
+{
+    const int h = N >> 1, nU = N - h;
+    const int64_t n = rows * Np;
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
+         x += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = x / Np;
+        const int j = (int)(x - k * Np);
+        const double2* row = W + k * Np;
+        R[x] = j < nU ? row[h + j].x : (j < N ? row[j].y : 0.0);
     }
 }
 
@@ -659,6 +696,86 @@ __global__ __launch_bounds__(kRedElems * kRedParts) void k_reduce(
             M[x] = make_double2(tr * G * g, ti * G * g);
         }
         __syncthreads();
+    }
+}
+
+// Symmetric-basis launches: S = P + iQ from the partial slices of the
+// upper-triangle tiles (element (i, j), i <= j, sums the first
+// cnt[i/16 + (j/16) Np/16] slices of its block; within a diagonal block the
+// element above the diagonal stands for both), stored symmetric as
+// S[i + N j] = S[j + N i].  Work split as in k_reduce.
+__global__ __launch_bounds__(kRedElems * kRedParts) void k_reduce_sym(
+    const double2* __restrict__ P, const int* __restrict__ cnt, int Np, int N,
+    double2* __restrict__ S)
+{
+    __shared__ double2 part[kRedParts][kRedElems];
+    const int64_t n = (int64_t)N * N;
+    const int e = threadIdx.x % kRedElems, q = threadIdx.x / kRedElems;
+    for (int64_t x0 = (int64_t)blockIdx.x * kRedElems; x0 < n; x0 += (int64_t)gridDim.x * kRedElems) {
+        const int64_t x = x0 + e;
+        const int i = (int)(x % N), j = (int)(x / N);
+        const bool on = x < n && i <= j;
+        double sr = 0.0, si = 0.0;
+        if (on) {
+            const double2* src = P + (int64_t)j * Np + i;
+            const int ns = cnt[(i >> 4) + (j >> 4) * (Np >> 4)];
+            for (int s = q; s < ns; s += kRedParts) {
+                const double2 v = src[(int64_t)s * Np * Np];
+                sr += v.x;
+                si += v.y;
+            }
+        }
+        part[q][e] = make_double2(sr, si);
+        __syncthreads();
+        if (q == 0 && on) {
+            double tr = 0.0, ti = 0.0;
+#pragma unroll
+            for (int p = 0; p < kRedParts; ++p) {
+                tr += part[p][e].x;
+                ti += part[p][e].y;
+            }
+            S[x] = make_double2(tr, ti);
+            S[(int64_t)i * N + j] = make_double2(tr, ti);
+        }
+        __syncthreads();
+    }
+}
+
+// M from S (see klsp_contract_sym64.hip): with a' = max(a, N-1-a),
+// u = a' - N/2, v = a' if a' >= N - N/2, s = +1 (a = a') or -1,
+//   H[b, a] = S[ub, ua] + i sb S[vb, ua] - i sa S[ub, va] + sb sa S[vb, va]
+// (terms without a v dropped), then M = H * G * N[a] as k_reduce does.
+__global__ void k_expand_sym(const double2* __restrict__ S, int N, double G,
+                             const double* __restrict__ Nk, double2* __restrict__ M)
+{
+    const int h = N >> 1, nU = N - h;
+    const int64_t n = (int64_t)N * N;
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
+         x += (int64_t)gridDim.x * blockDim.x) {
+        const int b = (int)(x % N), a = (int)(x / N);
+        const int bp = b >= h ? b : N - 1 - b, ap = a >= h ? a : N - 1 - a;
+        const double sb = bp == b ? 1.0 : -1.0, sa = ap == a ? 1.0 : -1.0;
+        const int ub = bp - h, ua = ap - h;
+        const bool hb = bp >= nU, ha = ap >= nU;
+        const double2 t0 = S[(int64_t)ua * N + ub];
+        double hr = t0.x, hi = t0.y;
+        if (hb) {                       // + i sb S[vb, ua]
+            const double2 t = S[(int64_t)ua * N + bp];
+            hr -= sb * t.y;
+            hi += sb * t.x;
+        }
+        if (ha) {                       // - i sa S[ub, va]
+            const double2 t = S[(int64_t)ap * N + ub];
+            hr += sa * t.y;
+            hi -= sa * t.x;
+        }
+        if (hb && ha) {                 // + sb sa S[vb, va]
+            const double2 t = S[(int64_t)ap * N + bp];
+            hr += sb * sa * t.x;
+            hi += sb * sa * t.y;
+        }
+        const double g = Nk[a];
+        M[x] = make_double2(hr * G * g, hi * G * g);
     }
 }
 
@@ -884,6 +1001,9 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_det(const double2* __restrict
 // =====================================================================
 // Host side: context, buffers, drivers
 // =====================================================================
+// Variant of the symmetric-basis FP64 contraction used until the autotuner
+// has measured the problem's width (index into contract_sym64_name).
+constexpr int kSymDefaultVariant = 2;   // depth 8, upper-triangle diagonal tiles
 constexpr int kCircle = 16;               // circle points of a model-det zero search
 constexpr int kModelPts = kCircle + 1;    // with the centre: matrices per model LU launch
 struct klsp_ctx {
@@ -925,6 +1045,20 @@ struct klsp_ctx {
         WorkUnit* d_units = nullptr;
         int* d_count = nullptr;
     } bal[2][kMaxBatch + 1];
+    Balanced bal_sym;            // ... of the symmetric-basis FP64 contraction
+    int bal_sym_variant = -1;    // the kernel variant bal_sym was planned for
+    int sym_variant = kSymDefaultVariant;   // kernel variant of symmetric-basis builds
+    std::map<int, int> sym_tuned;           // autotuned variant per padded width Np
+    // real basis R of a symmetric alpha grid (FP64 one-omega builds): sym is
+    // -1 until W has been checked, then 1 (W[:, N-1-a] = conj W[:, a]) or 0;
+    // use_sym = 0 turns the real-basis builds off (klsp_set_symmetric_basis)
+    double* d_R = nullptr;    size_t R_cap = 0;
+    bool have_R = false;
+    int sym = -1;
+    int use_sym = 1;
+    bool last_sym = false;       // the last contraction launch was symmetric-basis
+    double2* d_S = nullptr;   size_t S_cap = 0;   // S = P + iQ of that launch
+    int* d_flag = nullptr;
     // profiling
     bool prof = false;
     hipEvent_t ev[6] = {};
@@ -987,23 +1121,20 @@ struct GangPlan {
 // (largest groups first, least-loaded XCD).  With `uniform` every tile gets
 // the same number of pieces, 8 x (XCD slots / tiles): equal gangs of whole
 // tile sets, one K range each.
-static bool plan_balanced(const klsp_ctx* ctx, const ContractShape& sh, bool uniform, int quantum,
-                          klsp_ctx::Balanced& out, std::vector<WorkUnit>& units,
-                          std::vector<int>& count)
+// Tiles of a launch: work-table index (tile row * wcols + tile column) and
+// the number of 16 x 16 output blocks the tile computes.
+struct TileDesc {
+    int widx, blocks;
+};
+
+static bool plan_units(const klsp_ctx* ctx, const std::vector<TileDesc>& tiles, int bmax,
+                       int waves_per_cu, bool uniform, int quantum, klsp_ctx::Balanced& out,
+                       std::vector<WorkUnit>& units, std::vector<int>& pieces)
 {
-    const int nb = ctx->Np / 16;
-    const int wrows = (nb + sh.MB - 1) / sh.MB, wcols = (nb + sh.NB - 1) / sh.NB;
-    const int nt = wrows * wcols;
-    std::vector<int> blocks(nt);
-    int bmax = 1;
-    for (int t = 0; t < nt; ++t) {
-        const int r = t / wcols, c = t % wcols;
-        blocks[t] = std::min(sh.MB, nb - r * sh.MB) * std::min(sh.NB, nb - c * sh.NB);
-        bmax = std::max(bmax, blocks[t]);
-    }
-    const int slots = std::max(1, ctx->n_cu / 8) * std::max(1, sh.waves_per_cu);
+    const int nt = (int)tiles.size();
+    const int slots = std::max(1, ctx->n_cu / 8) * std::max(1, waves_per_cu);
     const int64_t Kq = ctx->Kp / quantum;  // K in units of the loop trip (the last piece ends at Kp)
-    std::vector<int> pieces(nt);
+    pieces.assign(nt, 1);
     const int64_t pf_max = std::max<int64_t>(1, std::min<int64_t>(ctx->Kp / 256, 8 * slots));
     const int64_t pu = std::max<int64_t>(1, std::min<int64_t>(ctx->Kp / 256, 8 * (slots / nt)));
     for (int pf = (int)pf_max; pf >= 1; --pf) {
@@ -1011,7 +1142,7 @@ static bool plan_balanced(const klsp_ctx* ctx, const ContractShape& sh, bool uni
         for (int t = 0; t < nt; ++t) {
             pieces[t] = uniform ? (int)std::min<int64_t>(pu, pf)
                                 : (int)std::min<int64_t>(std::max<int64_t>(1, Kq),
-                                                         ((int64_t)blocks[t] * pf + bmax - 1) / bmax);
+                                                         ((int64_t)tiles[t].blocks * pf + bmax - 1) / bmax);
             total += pieces[t];
         }
         if (total > 8 * (int64_t)slots) continue;
@@ -1025,7 +1156,7 @@ static bool plan_balanced(const klsp_ctx* ctx, const ContractShape& sh, bool uni
                 for (int j = 0; j < pv; ++j) {
                     std::vector<int> ts;
                     for (int t = 0; t < nt; ++t)
-                        if (pieces[t] == pv) ts.push_back(t);
+                        if (pieces[t] == pv) ts.push_back(tiles[t].widx);
                     groups.push_back({{pv, j}, ts});
                 }
         }
@@ -1060,12 +1191,6 @@ static bool plan_balanced(const klsp_ctx* ctx, const ContractShape& sh, bool uni
         units.assign(8 * per, WorkUnit{0, 0, -1, 0});
         for (int x = 0; x < 8; ++x)
             for (size_t k = 0; k < bins[x].size(); ++k) units[8 * k + x] = bins[x][k];
-        count.assign((size_t)nb * nb, 1);
-        for (int bi = 0; bi < nb; ++bi)
-            for (int bj = 0; bj < nb; ++bj)
-                count[bi + (size_t)bj * nb] = pieces[(bi / sh.MB) * wcols + bj / sh.NB];
-        out.wrows = wrows;
-        out.wcols = wcols;
         out.n_slots = *std::max_element(pieces.begin(), pieces.end());
         out.n_units = (int)units.size();
         return true;
@@ -1073,7 +1198,93 @@ static bool plan_balanced(const klsp_ctx* ctx, const ContractShape& sh, bool uni
     return false;
 }
 
+// Balanced work table of an FP64 / FP32 contraction shape over the whole
+// output (see klsp_contract.hpp); count[bi + bj nb] = the piece count of the
+// tile holding block (bi, bj).
+static bool plan_balanced(const klsp_ctx* ctx, const ContractShape& sh, bool uniform, int quantum,
+                          klsp_ctx::Balanced& out, std::vector<WorkUnit>& units,
+                          std::vector<int>& count)
+{
+    const int nb = ctx->Np / 16;
+    const int wrows = (nb + sh.MB - 1) / sh.MB, wcols = (nb + sh.NB - 1) / sh.NB;
+    std::vector<TileDesc> tiles;
+    int bmax = 1;
+    for (int t = 0; t < wrows * wcols; ++t) {
+        const int r = t / wcols, c = t % wcols;
+        const int b = std::min(sh.MB, nb - r * sh.MB) * std::min(sh.NB, nb - c * sh.NB);
+        tiles.push_back({t, b});
+        bmax = std::max(bmax, b);
+    }
+    std::vector<int> pieces;
+    if (!plan_units(ctx, tiles, bmax, sh.waves_per_cu, uniform, quantum, out, units, pieces))
+        return false;
+    count.assign((size_t)nb * nb, 1);
+    for (int bi = 0; bi < nb; ++bi)
+        for (int bj = 0; bj < nb; ++bj)
+            count[bi + (size_t)bj * nb] = pieces[(bi / sh.MB) * wcols + bj / sh.NB];
+    out.wrows = wrows;
+    out.wcols = wcols;
+    return true;
+}
+
+// The same for the symmetric-basis FP64 contraction: square tiles on or above
+// the diagonal only, a diagonal tile of m x m blocks computing m (m + 1) / 2
+// of them (all m^2 in the full-diagonal variants); count covers blocks
+// bi <= bj.
+static bool plan_balanced_sym(const klsp_ctx* ctx, int variant, klsp_ctx::Balanced& out,
+                              std::vector<WorkUnit>& units, std::vector<int>& count)
+{
+    const ContractShape sh = contract_sym64_shape(variant);
+    const bool full = contract_sym64_full_diag(variant);
+    const int nb = ctx->Np / 16;
+    const int wt = (nb + sh.MB - 1) / sh.MB;
+    std::vector<TileDesc> tiles;
+    std::vector<int> index(wt * wt, -1);
+    for (int r = 0; r < wt; ++r)
+        for (int c = r; c < wt; ++c) {
+            const int mv = std::min(sh.MB, nb - r * sh.MB), nv = std::min(sh.MB, nb - c * sh.MB);
+            index[r * wt + c] = (int)tiles.size();
+            tiles.push_back({r * wt + c, r == c && !full ? mv * (mv + 1) / 2 : mv * nv});
+        }
+    std::vector<int> pieces;
+    if (!plan_units(ctx, tiles, sh.MB * sh.MB, sh.waves_per_cu, false,
+                    contract_sym64_quantum(variant), out, units, pieces))
+        return false;
+    count.assign((size_t)nb * nb, 0);
+    for (int bi = 0; bi < nb; ++bi)
+        for (int bj = bi; bj < nb; ++bj)
+            count[bi + (size_t)bj * nb] = pieces[index[(bi / sh.MB) * wt + bj / sh.MB]];
+    out.wrows = wt;
+    out.wcols = wt;
+    return true;
+}
+
 static ContractShape shape_for(int precision, int nw);
+
+// The balanced table for (precision, nw), built and uploaded on first use.
+// Uploads a planned work table into b (stored in b before the copies, so
+// that a failed call leaves nothing unowned: a retry or free_balanced
+// releases it).
+static int store_balanced(klsp_ctx* ctx, klsp_ctx::Balanced& b, const klsp_ctx::Balanced& nbal,
+                          const std::vector<WorkUnit>& units, const std::vector<int>& count)
+{
+    hipFree(b.d_units);
+    hipFree(b.d_count);
+    b.d_units = nullptr;
+    b.d_count = nullptr;
+    KLSP_HIP(ctx, hipMalloc((void**)&b.d_units, units.size() * sizeof(WorkUnit)));
+    KLSP_HIP(ctx, hipMalloc((void**)&b.d_count, count.size() * sizeof(int)));
+    KLSP_HIP(ctx, hipMemcpy(b.d_units, units.data(), units.size() * sizeof(WorkUnit),
+                            hipMemcpyHostToDevice));
+    KLSP_HIP(ctx, hipMemcpy(b.d_count, count.data(), count.size() * sizeof(int),
+                            hipMemcpyHostToDevice));
+    b.wrows = nbal.wrows;
+    b.wcols = nbal.wcols;
+    b.n_slots = nbal.n_slots;
+    b.n_units = nbal.n_units;
+    b.built = true;
+    return KLSP_OK;
+}
 
 // The balanced table for (precision, nw), built and uploaded on first use.
 static int get_balanced(klsp_ctx* ctx, int precision, int nw, const klsp_ctx::Balanced** out)
@@ -1093,23 +1304,26 @@ static int get_balanced(klsp_ctx* ctx, int precision, int nw, const klsp_ctx::Ba
                                                         : kRowsPerTrip<false, 1>;
         if (!plan_balanced(ctx, shape_for(precision, nw), uniform, quantum, nbal, units, count))
             return fail(ctx, KLSP_ERR_INVALID_ARG, "contraction: no work plan fits the device");
-        // stored in b before the copies, so that a failed call leaves nothing
-        // unowned (a retry or free_balanced releases it)
-        hipFree(b.d_units);
-        hipFree(b.d_count);
-        b.d_units = nullptr;
-        b.d_count = nullptr;
-        KLSP_HIP(ctx, hipMalloc((void**)&b.d_units, units.size() * sizeof(WorkUnit)));
-        KLSP_HIP(ctx, hipMalloc((void**)&b.d_count, count.size() * sizeof(int)));
-        KLSP_HIP(ctx, hipMemcpy(b.d_units, units.data(), units.size() * sizeof(WorkUnit),
-                                hipMemcpyHostToDevice));
-        KLSP_HIP(ctx, hipMemcpy(b.d_count, count.data(), count.size() * sizeof(int),
-                                hipMemcpyHostToDevice));
-        b.wrows = nbal.wrows;
-        b.wcols = nbal.wcols;
-        b.n_slots = nbal.n_slots;
-        b.n_units = nbal.n_units;
-        b.built = true;
+        int rc = store_balanced(ctx, b, nbal, units, count);
+        if (rc) return rc;
+    }
+    *out = &b;
+    return KLSP_OK;
+}
+
+// The balanced table of the symmetric-basis FP64 contraction.
+static int get_balanced_sym(klsp_ctx* ctx, const klsp_ctx::Balanced** out)
+{
+    klsp_ctx::Balanced& b = ctx->bal_sym;
+    if (!b.built || ctx->bal_sym_variant != ctx->sym_variant) {
+        std::vector<WorkUnit> units;
+        std::vector<int> count;
+        klsp_ctx::Balanced nbal;
+        if (!plan_balanced_sym(ctx, ctx->sym_variant, nbal, units, count))
+            return fail(ctx, KLSP_ERR_INVALID_ARG, "contraction: no work plan fits the device");
+        int rc = store_balanced(ctx, b, nbal, units, count);
+        if (rc) return rc;
+        ctx->bal_sym_variant = ctx->sym_variant;
     }
     *out = &b;
     return KLSP_OK;
@@ -1176,6 +1390,9 @@ static void free_balanced(klsp_ctx* ctx)
             hipFree(b.d_count);
             b = klsp_ctx::Balanced();
         }
+    hipFree(ctx->bal_sym.d_units);
+    hipFree(ctx->bal_sym.d_count);
+    ctx->bal_sym = klsp_ctx::Balanced();
 }
 
 static int free_problem(klsp_ctx* ctx)
@@ -1184,19 +1401,22 @@ static int free_problem(klsp_ctx* ctx)
     hipFree(ctx->d_DJ); hipFree(ctx->d_F0l); hipFree(ctx->d_Ores); hipFree(ctx->d_Nk);
     ctx->d_DJ = ctx->d_F0l = ctx->d_Ores = ctx->d_Nk = nullptr;
     ctx->have_problem = ctx->have_W = ctx->have_arrays = false;
-    ctx->have_W32 = ctx->have_Wb = false;
+    ctx->have_W32 = ctx->have_Wb = ctx->have_R = false;
+    ctx->sym = -1;
     return KLSP_OK;
 }
 
 // The W buffers themselves (kept across problems by free_problem).
 static void release_W(klsp_ctx* ctx)
 {
-    hipFree(ctx->d_W64); hipFree(ctx->d_W32); hipFree(ctx->d_Wb);
+    hipFree(ctx->d_W64); hipFree(ctx->d_W32); hipFree(ctx->d_Wb); hipFree(ctx->d_R);
     ctx->d_W64 = nullptr;
     ctx->d_W32 = nullptr;
     ctx->d_Wb = nullptr;
-    ctx->W64_cap = ctx->W32_cap = ctx->Wb_cap = 0;
-    ctx->have_W = ctx->have_W32 = ctx->have_Wb = false;
+    ctx->d_R = nullptr;
+    ctx->W64_cap = ctx->W32_cap = ctx->Wb_cap = ctx->R_cap = 0;
+    ctx->have_W = ctx->have_W32 = ctx->have_Wb = ctx->have_R = false;
+    ctx->sym = -1;
 }
 
 extern "C" {
@@ -1254,7 +1474,8 @@ int klsp_create(int device, klsp_ctx** ctx_out)
     for (auto& e : ctx->ev) hipEventCreate(&e);
     if (hipMalloc((void**)&ctx->d_om, sizeof(double) * 2 * kMaxBatch) != hipSuccess ||
         hipMalloc((void**)&ctx->d_coef, sizeof(double2) * kModelPts * kMaxW) != hipSuccess ||
-        hipMalloc((void**)&ctx->d_det, sizeof(double2) * kMaxBatch) != hipSuccess) {
+        hipMalloc((void**)&ctx->d_det, sizeof(double2) * kMaxBatch) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_flag, sizeof(int)) != hipSuccess) {
         ctx->det_cap = 0;
         klsp_destroy(ctx);
         return KLSP_ERR_OOM;
@@ -1273,6 +1494,7 @@ int klsp_destroy(klsp_ctx* ctx)
     release_W(ctx);
     hipFree(ctx->d_om); hipFree(ctx->d_c); hipFree(ctx->d_P); hipFree(ctx->d_M);
     hipFree(ctx->d_A); hipFree(ctx->d_det); hipFree(ctx->d_coef); hipFree(ctx->d_cf);
+    hipFree(ctx->d_S); hipFree(ctx->d_flag);
     for (auto& e : ctx->ev) hipEventDestroy(e);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1302,6 +1524,10 @@ try {
     ctx->Np = (int)round_up(ctx->N, kColPad);
     ctx->m = pr->m;
     ctx->G = pr->G;
+    {
+        const auto t = ctx->sym_tuned.find(ctx->Np);
+        ctx->sym_variant = t != ctx->sym_tuned.end() ? t->second : kSymDefaultVariant;
+    }
     const size_t nK = (size_t)ctx->K;
     KLSP_HIP(ctx, hipMalloc((void**)&ctx->d_DJ, sizeof(double) * ctx->NPh));
     KLSP_HIP(ctx, hipMalloc((void**)&ctx->d_F0l, sizeof(double) * nK));
@@ -1341,7 +1567,8 @@ try {
 static int alloc_W64(klsp_ctx* ctx)
 {
     const size_t n = (size_t)(ctx->Kp + kRowPad) * ctx->Np;
-    ctx->have_W = ctx->have_W32 = ctx->have_Wb = false;
+    ctx->have_W = ctx->have_W32 = ctx->have_Wb = ctx->have_R = false;
+    ctx->sym = -1;
     if (n > ctx->W64_cap) release_W(ctx);
     int rc = ensure(ctx, ctx->d_W64, ctx->W64_cap, n);
     if (rc) return rc;
@@ -1443,6 +1670,40 @@ static int W_from_orbits(klsp_ctx* ctx, int64_t nwa, const OrbitBufs& ob, const 
     cleanup();
     if (e != hipSuccess) return fail(ctx, KLSP_ERR_HIP, std::string(who) + ": " + hipGetErrorString(e));
     ctx->have_W = true;
+    // the mirrored quadrature writes W(-alpha) = conj W(alpha) and a real
+    // centre column exactly; other grids are checked when first needed
+    if (mirror) ctx->sym = 1;
+    return KLSP_OK;
+}
+
+// The real basis of the current W for FP64 one-omega builds, made on first
+// use: the mirror symmetry is checked (unless the quadrature guaranteed it)
+// and, where it holds, R is filled from W (12.5 GB written, 25 GB read at the
+// default grid: a few ms, once per W).
+static int prepare_sym(klsp_ctx* ctx)
+{
+    if (!ctx->use_sym || ctx->sym == 0 || ctx->have_R || !ctx->have_W) return KLSP_OK;
+    const int64_t rows = ctx->Kp + kRowPad;
+    if (ctx->sym < 0) {
+        int flag = 1;
+        KLSP_HIP(ctx, hipMemsetAsync(ctx->d_flag, 0, sizeof(int), ctx->stream));
+        hipLaunchKernelGGL(k_W_mirror_check, dim3(grid_for(rows * (ctx->N - ctx->N / 2))),
+                           dim3(256), 0, ctx->stream, ctx->d_W64, rows, (int)ctx->N, ctx->Np,
+                           ctx->d_flag);
+        KLSP_HIP(ctx, hipGetLastError());
+        KLSP_HIP(ctx, hipMemcpyAsync(&flag, ctx->d_flag, sizeof(int), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+        KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->sym = flag ? 0 : 1;
+        if (!ctx->sym) return KLSP_OK;
+    }
+    int rc = ensure(ctx, ctx->d_R, ctx->R_cap, (size_t)rows * ctx->Np);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_W64_to_R, dim3(grid_for(rows * ctx->Np)), dim3(256), 0, ctx->stream,
+                       ctx->d_W64, rows, (int)ctx->N, ctx->Np, ctx->d_R);
+    KLSP_HIP(ctx, hipGetLastError());
+    KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->have_R = true;
     return KLSP_OK;
 }
 
@@ -1599,6 +1860,8 @@ try {
         return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_prepare: unknown precision");
     if (!ctx->have_W) return fail(ctx, KLSP_ERR_NOT_READY, "klsp_prepare: no W on device");
     int rc;
+    KLSP_HIP(ctx, hipSetDevice(ctx->device));
+    if (precision == KLSP_PREC_FP64 && (rc = prepare_sym(ctx))) return rc;
     if (precision == KLSP_PREC_FP32 && !ctx->have_W32) {
         KLSP_HIP(ctx, hipSetDevice(ctx->device));
         const int64_t n = (ctx->Kp + kRowPad) * ctx->Np;
@@ -1632,11 +1895,36 @@ static int launch_contract(klsp_ctx* ctx, int precision, int nb, int nf)
     const bool bf = precision == KLSP_PREC_BF16X3;
     GangPlan gp{};
     const klsp_ctx::Balanced* bal = nullptr;
+    int rc;
+    if (precision == KLSP_PREC_FP64 && nb == 1 && (rc = prepare_sym(ctx))) return rc;
+    const bool symb = precision == KLSP_PREC_FP64 && nb == 1 && ctx->use_sym && ctx->sym == 1 &&
+                      ctx->have_R;
+    if (symb) {
+        if ((rc = get_balanced_sym(ctx, &bal))) return rc;
+        ContractArgs a{};
+        a.units = bal->d_units;
+        a.n_units = bal->n_units;
+        a.W = ctx->d_R;
+        a.cw = ctx->d_c;
+        a.P = ctx->d_P;
+        a.Kp = ctx->Kp;
+        a.Np = ctx->Np;
+        a.n_gangs = bal->n_slots;
+        a.wrows = bal->wrows;
+        a.wcols = bal->wcols;
+        const hipError_t e = contract_sym64_launch(ctx->sym_variant, a, ctx->stream);
+        if (e != hipSuccess)
+            return fail(ctx, KLSP_ERR_HIP, std::string("contraction launch: ") + hipGetErrorString(e));
+        ctx->last_S = bal->n_slots;
+        ctx->last_count = bal->d_count;
+        ctx->last_sym = true;
+        return KLSP_OK;
+    }
+    ctx->last_sym = false;
     if (bf) {
         gp = plan_gangs_bf16(ctx, nb, nf);
     } else {
-        int rc = get_balanced(ctx, precision, nb, &bal);
-        if (rc) return rc;
+        if ((rc = get_balanced(ctx, precision, nb, &bal))) return rc;
         gp.wrows = bal->wrows;
         gp.wcols = bal->wcols;
         gp.n_gangs = bal->n_slots;
@@ -1685,8 +1973,17 @@ static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
         if ((rc = ensure(ctx, ctx->d_cf, ctx->cf_cap, (size_t)kMaxW * Kpc))) return rc;
         cf = ctx->d_cf;
     }
-    const int mg = max_gangs(ctx, &rc);
+    int mg = max_gangs(ctx, &rc);
     if (rc) return rc;
+    if (precision == KLSP_PREC_FP64 && nb == 1) {
+        if ((rc = prepare_sym(ctx))) return rc;
+        if (ctx->use_sym && ctx->sym == 1) {
+            const klsp_ctx::Balanced* bs = nullptr;
+            if ((rc = get_balanced_sym(ctx, &bs))) return rc;
+            mg = std::max(mg, bs->n_slots);
+            if ((rc = ensure(ctx, ctx->d_S, ctx->S_cap, (size_t)ctx->N * ctx->N))) return rc;
+        }
+    }
     if ((rc = ensure(ctx, ctx->d_P, ctx->P_cap, (size_t)kMaxW * mg * ctx->Np * ctx->Np)))
         return rc;
     if ((rc = ensure(ctx, ctx->d_M, ctx->M_cap,
@@ -1707,10 +2004,20 @@ static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
     if (ctx->prof) hipEventRecord(ctx->ev[1], ctx->stream);
     if ((rc = launch_contract(ctx, precision, nb, nf))) return rc;
     if (ctx->prof) hipEventRecord(ctx->ev[2], ctx->stream);
-    hipLaunchKernelGGL(k_reduce, dim3(grid_for((int64_t)nb * ctx->N * ctx->N, kRedElems)),
-                       dim3(kRedElems * kRedParts), 0,
-                       ctx->stream, ctx->d_P, ctx->last_S, ctx->last_count, ctx->Np, (int)ctx->N,
-                       nb, ctx->G, ctx->d_Nk, ctx->d_M + (size_t)slot * ctx->N * ctx->N);
+    if (ctx->last_sym) {
+        const int64_t NN = ctx->N * ctx->N;
+        hipLaunchKernelGGL(k_reduce_sym, dim3(grid_for(NN, kRedElems)), dim3(kRedElems * kRedParts),
+                           0, ctx->stream, ctx->d_P, ctx->last_count, ctx->Np, (int)ctx->N,
+                           ctx->d_S);
+        hipLaunchKernelGGL(k_expand_sym, dim3(grid_for(NN)), dim3(256), 0, ctx->stream, ctx->d_S,
+                           (int)ctx->N, ctx->G, ctx->d_Nk, ctx->d_M + (size_t)slot * NN);
+    } else {
+        hipLaunchKernelGGL(k_reduce, dim3(grid_for((int64_t)nb * ctx->N * ctx->N, kRedElems)),
+                           dim3(kRedElems * kRedParts), 0,
+                           ctx->stream, ctx->d_P, ctx->last_S, ctx->last_count, ctx->Np,
+                           (int)ctx->N, nb, ctx->G, ctx->d_Nk,
+                           ctx->d_M + (size_t)slot * ctx->N * ctx->N);
+    }
     if (ctx->prof) hipEventRecord(ctx->ev[3], ctx->stream);
     KLSP_HIP(ctx, hipGetLastError());
     add_build_timing(ctx, nb);
@@ -2291,6 +2598,107 @@ try {
     }
     if (Op_init) *Op_init = Opg[best / sp->n_gamma];
     if (g_init) *g_init = gg[best % sp->n_gamma];
+    return KLSP_OK;
+} catch (...) {
+    return klsp::exception_status(ctx ? &ctx->err : nullptr);
+}
+
+int klsp_autotune(klsp_ctx* ctx, int precision, int reps, int* n_variants, double* ms_out,
+                  int n_max, int* chosen)
+try {
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    if (reps < 1 || !n_variants || (n_max > 0 && !ms_out))
+        return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_autotune: bad arguments");
+    int rc = check_ready(ctx, precision, "klsp_autotune");
+    if (rc) return rc;
+    KLSP_HIP(ctx, hipSetDevice(ctx->device));
+    *n_variants = 1;
+    if (chosen) *chosen = 0;
+    if (precision != KLSP_PREC_FP64 || (rc = prepare_sym(ctx)) || !(ctx->use_sym && ctx->sym == 1)) {
+        // one form only: its time
+        if (rc) return rc;
+        const bool prof = ctx->prof;
+        ctx->prof = true;
+        const double om[2] = {1.0, 0.1};
+        double t = 0.0;
+        for (int r = 0; r <= reps && rc == KLSP_OK; ++r) {
+            reset_timing(ctx);
+            rc = build_batch(ctx, om, 1, precision);
+            if (r > 0) t += ctx->last.contract_ms;
+        }
+        ctx->prof = prof;
+        reset_timing(ctx);
+        if (rc) return rc;
+        if (n_max > 0) ms_out[0] = t / reps;
+        return KLSP_OK;
+    }
+    // FP64 on a symmetric grid: every compiled variant of the real-basis
+    // contraction, each with one untimed launch and `reps` timed ones
+    const int nv = contract_sym64_variants();
+    const bool prof = ctx->prof;
+    const int keep = ctx->sym_variant;
+    ctx->prof = true;
+    const double om[2] = {1.0, 0.1};
+    int best = keep;
+    double best_ms = INFINITY;
+    for (int v = 0; v < nv && rc == KLSP_OK; ++v) {
+        ctx->sym_variant = v;
+        double t = 0.0;
+        for (int r = 0; r <= reps && rc == KLSP_OK; ++r) {
+            reset_timing(ctx);
+            rc = build_batch(ctx, om, 1, KLSP_PREC_FP64);
+            if (r > 0) t += ctx->last.contract_ms;
+        }
+        t /= reps;
+        if (v < n_max) ms_out[v] = t;
+        if (t < best_ms) { best_ms = t; best = v; }
+    }
+    ctx->prof = prof;
+    reset_timing(ctx);
+    if (rc) {
+        ctx->sym_variant = keep;
+        return rc;
+    }
+    ctx->sym_variant = best;
+    ctx->sym_tuned[ctx->Np] = best;
+    *n_variants = nv;
+    if (chosen) *chosen = best;
+    return KLSP_OK;
+} catch (...) {
+    return klsp::exception_status(ctx ? &ctx->err : nullptr);
+}
+
+int klsp_set_variant(klsp_ctx* ctx, int precision, int variant)
+{
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    if (precision != KLSP_PREC_FP64 || variant < 0 || variant >= contract_sym64_variants())
+        return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_set_variant: no such variant");
+    ctx->sym_variant = variant;
+    return KLSP_OK;
+}
+
+const char* klsp_variant_name(int precision, int variant)
+{
+    if (precision == KLSP_PREC_FP64 && variant >= 0 && variant < contract_sym64_variants())
+        return contract_sym64_name(variant);
+    return "";
+}
+
+int klsp_set_symmetric_basis(klsp_ctx* ctx, int on)
+{
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    ctx->use_sym = on != 0;
+    return KLSP_OK;
+}
+
+int klsp_symmetric_basis(klsp_ctx* ctx, int* active_out)
+try {
+    if (!ctx || !active_out) return KLSP_ERR_INVALID_ARG;
+    if (!ctx->have_W) return fail(ctx, KLSP_ERR_NOT_READY, "klsp_symmetric_basis: no W on device");
+    KLSP_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = prepare_sym(ctx);
+    if (rc) return rc;
+    *active_out = ctx->use_sym && ctx->sym == 1 && ctx->have_R;
     return KLSP_OK;
 } catch (...) {
     return klsp::exception_status(ctx ? &ctx->err : nullptr);

@@ -50,7 +50,7 @@ constexpr int kMaxN = 1024;
 constexpr int kTile = 64;       // output tile (beta x alpha) per workgroup
 constexpr int kColPad = 16;     // alpha padding of the device W rows
 constexpr int kKStep = 4;       // K per MFMA step (16x16x4 f64/f32 MFMA)
-constexpr int kRowPad = 32;     // zero rows past Kp in W and c (prefetch slack)
+constexpr int kRowPad = 64;     // zero rows past Kp in W and c (prefetch slack)
 constexpr int kMaxBatch = 3;    // omegas sharing one pass over W
 constexpr int kMaxTerms = 4;    // Taylor terms of one BF16x3 pass
 constexpr int kMaxW = kMaxTerms > kMaxBatch ? kMaxTerms : kMaxBatch;  // weight sets per pass

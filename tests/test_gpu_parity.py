@@ -715,3 +715,123 @@ def test_mixed_newton_converged_flag_is_honest(oracle_small, start, max_iter):
             assert abs(d64) < 1e-12 * 10
     finally:
         pre.ctx.close()
+
+
+# --------------------------------------------------------------------------
+# Real basis of a symmetric alpha grid (one-omega FP64 builds): the same M
+# as the complex product and the BLAS-form oracle, on every tile shape
+# --------------------------------------------------------------------------
+def mirror_symmetric_W(rng, NPh, N, n_l, scale=1.0):
+    """Random W with W[:, N-1-a] = conj W[:, a] and a real centre column."""
+    W = (rng.standard_normal((NPh, N, n_l)) + 1j * rng.standard_normal((NPh, N, n_l))) * scale
+    h = N // 2
+    for a in range(h):
+        W[:, N - 1 - a, :] = np.conj(W[:, a, :])
+    if N % 2:
+        W[:, h, :] = W[:, h, :].real
+    return W
+
+
+@pytest.mark.parametrize("N", [1, 2, 5, 16, 17, 31, 32, 33, 64, 65, 79, 130, 301])
+def test_symmetric_basis_matches_complex_product(N):
+    """Symmetric W: the real-basis build equals the complex product and the
+    oracle's BLAS-form build to 1e-12 normwise, and the dets agree (N covers
+    odd and even grids, tiles cut by the edge of the 16-column blocks, and
+    one to five tile rows)."""
+    from oracle import blas_form
+    rng = np.random.default_rng(1000 + N)
+    NPh, n_l = 97, 3
+    W = mirror_symmetric_W(rng, NPh, N, n_l, 1.0 / np.sqrt(NPh * n_l))
+    DJ, F0l = rng.uniform(0.2, 1.0, NPh), rng.standard_normal((NPh, n_l))
+    Ores, Nk = rng.uniform(-2, 2, (NPh, n_l)), rng.uniform(0.5, 1.5, N)
+    pre = from_arrays(2, 0.8, W, DJ, F0l, Ores, Nk)
+    try:
+        assert pre.ctx.symmetric_basis()
+        for om in (0.3 + 0.4j, 1.1 - 0.2j):
+            Ms = pre.ctx.build_M(om, _ffi.PREC_FP64)
+            ds = pre.ctx.det([om], _ffi.PREC_FP64)[0]
+            pre.ctx.set_symmetric_basis(False)
+            Mc = pre.ctx.build_M(om, _ffi.PREC_FP64)
+            dc = pre.ctx.det([om], _ffi.PREC_FP64)[0]
+            pre.ctx.set_symmetric_basis(True)
+            Mb = blas_form.build_M(W, DJ, F0l, Ores, Nk, 0.8, om)
+            assert relnorm(Ms, Mb) <= 1e-12, relnorm(Ms, Mb)
+            assert relnorm(Ms, Mc) <= 1e-12
+            assert abs(ds - dc) <= 1e-12 * max(abs(dc), 1.0)
+    finally:
+        pre.ctx.close()
+
+
+def test_symmetric_basis_detection():
+    """An uploaded W is used in the real basis only if it is exactly mirror
+    symmetric: one entry off by one ulp, or an imaginary part in the centre
+    column, keeps the complex product (which then still matches the oracle)."""
+    from oracle import blas_form
+    rng = np.random.default_rng(21)
+    NPh, N, n_l = 50, 21, 2
+    W = mirror_symmetric_W(rng, NPh, N, n_l, 0.1)
+    DJ, F0l = rng.uniform(0.2, 1.0, NPh), rng.standard_normal((NPh, n_l))
+    Ores, Nk = rng.uniform(-2, 2, (NPh, n_l)), rng.uniform(0.5, 1.5, N)
+    om = 0.5 + 0.3j
+    for case in ("symmetric", "ulp", "centre"):
+        V = W.copy()
+        if case == "ulp":
+            V[7, 3, 1] = complex(np.nextafter(V[7, 3, 1].real, 1.0), V[7, 3, 1].imag)
+        elif case == "centre":
+            V[11, N // 2, 0] += 1e-300j
+        pre = from_arrays(2, 1.0, V, DJ, F0l, Ores, Nk)
+        try:
+            assert pre.ctx.symmetric_basis() == (case == "symmetric"), case
+            M = pre.ctx.build_M(om, _ffi.PREC_FP64)
+            assert relnorm(M, blas_form.build_M(V, DJ, F0l, Ores, Nk, 1.0, om)) <= 1e-12
+        finally:
+            pre.ctx.close()
+
+
+def test_quadrature_W_is_symmetric_on_the_reference_grid(oracle_tiny):
+    """W from the device quadrature on range(-alpha_max, alpha_max, N) is
+    mirror symmetric with a real alpha = 0 column (as the reference's
+    e^(-i 0 ln r) = 1 makes it), so FP64 builds take the real basis."""
+    p = oracle_tiny["params"]
+    cfg = cfg_from_params(p)
+    pre = K.precompute_all(cfg, K.create_toomre_zang_model(cfg), K.GPUType.AMD)
+    try:
+        W = pre.W_l_mat
+        N = W.shape[1]
+        assert np.array_equal(W[:, ::-1, :], np.conj(W))
+        if N % 2:
+            assert np.all(W[:, N // 2, :].imag == 0)
+        assert pre.ctx.symmetric_basis()
+        om = 0.8 + 0.1j
+        Mo = oracle.build_M(oracle_tiny, om)
+        assert relnorm(pre.ctx.build_M(om, _ffi.PREC_FP64), Mo) <= 1e-12
+    finally:
+        pre.ctx.close()
+
+
+@pytest.mark.parametrize("N", [33, 130, 301])
+def test_symmetric_basis_every_variant(N):
+    """Every compiled variant of the real-basis kernel (prefetch depth,
+    diagonal-tile form) gives the oracle's M to 1e-12; the autotuner times
+    them all and keeps one of them."""
+    from oracle import blas_form
+    rng = np.random.default_rng(2000 + N)
+    NPh, n_l = 211, 2
+    W = mirror_symmetric_W(rng, NPh, N, n_l, 1.0 / np.sqrt(NPh * n_l))
+    DJ, F0l = rng.uniform(0.2, 1.0, NPh), rng.standard_normal((NPh, n_l))
+    Ores, Nk = rng.uniform(-2, 2, (NPh, n_l)), rng.uniform(0.5, 1.5, N)
+    pre = from_arrays(2, 0.8, W, DJ, F0l, Ores, Nk)
+    try:
+        om = 0.45 + 0.15j
+        Mb = blas_form.build_M(W, DJ, F0l, Ores, Nk, 0.8, om)
+        times, chosen = pre.ctx.autotune(_ffi.PREC_FP64, reps=1)
+        print("variants", times, "chosen", chosen, flush=True)
+        assert len(times) >= 2 and chosen in times
+        assert relnorm(pre.ctx.build_M(om, _ffi.PREC_FP64), Mb) <= 1e-12
+        for v in range(len(times)):
+            pre.ctx.set_variant(v)
+            assert relnorm(pre.ctx.build_M(om, _ffi.PREC_FP64), Mb) <= 1e-12, v
+        with pytest.raises(_ffi.KlspError):
+            pre.ctx.set_variant(len(times))
+    finally:
+        pre.ctx.close()
