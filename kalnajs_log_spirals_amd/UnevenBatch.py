@@ -118,12 +118,14 @@ def config_for(d: Draw, scale: float = 1.0, base: Optional[KalnajsConfig] = None
 
 def run_batch(draws: List[Draw], precision: str = "mixed", scale: float = 1.0,
               base: Optional[KalnajsConfig] = None, log=None, polish: int = 0,
-              orbits: str = "host") -> List[DrawResult]:
+              orbits: str = "host", symmetric_basis: bool = True) -> List[DrawResult]:
     """Newton search (NewtonSolver.jl:36 semantics, `precision` as in
     newton_search) from each draw's starting omega.  With polish > 0 every
     converged root is then refined by that many FP64 Newton steps with
     tol = 0 (an a-posteriori check: the move is the root's FP64 error).
-    orbits = "host" | "device" chooses where the orbits are integrated."""
+    orbits = "host" | "device" chooses where the orbits are integrated;
+    symmetric_basis = False makes every build use the complex product
+    (klsp_set_symmetric_basis)."""
     if orbits not in ("host", "device"):
         raise ValueError(f"run_batch: orbits must be 'host' or 'device', not {orbits!r}")
     cfgs = [config_for(d, scale, base) for d in draws]
@@ -139,6 +141,7 @@ def run_batch(draws: List[Draw], precision: str = "mixed", scale: float = 1.0,
     # one context for the whole batch: its W buffers grow to the largest
     # draw's and are reused, instead of tens of GB allocated and freed per draw
     ctx = _ffi.Context(GPUBackend.current_device())
+    ctx.set_symmetric_basis(symmetric_basis)
     try:
         with ThreadPoolExecutor(max_workers=1) as pool:
             nxt = pool.submit(host, 0) if draws else None

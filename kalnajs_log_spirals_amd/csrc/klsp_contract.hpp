@@ -404,6 +404,7 @@ hipError_t contract_sym64_launch(int variant, const ContractArgs& a, hipStream_t
 constexpr int kBf16Waves = 4, kBf16MB = 5, kBf16NB = 1;
 struct Bf16Geometry {
     int wrows, wcols, wg_per_cu;   // workgroup tiles per gang, resident workgroups per CU
+    int mb = 0;                    // block slots per wave (real-basis kernel)
 };
 // nf: the first nf of the nw weight sets take the three-term products, the
 // others the single hi x hi product (nf = nw except in the mixed search).
@@ -412,6 +413,20 @@ hipError_t contract_bf16_launch(int nw, int nf, const ContractArgs& a, hipStream
 // W complex128 rows of K (`rows` present) -> bf16 hi and lo planes of nq
 // rows of four K (2 x nq x Np x 16 bytes at Wb).
 hipError_t split_W_bf16(const double2* W64, int64_t rows, int Np, int64_t nq, void* Wb,
+                        hipStream_t stream);
+
+// BF16 three-term contraction in the real basis of a symmetric alpha grid
+// (klsp_contract_symbf16.hip): folded column-block pairs of the upper
+// triangle (wcols pairs x wrows workgroups each, mb block slots per wave),
+// partial slice P + iQ per gang and weight set; K ranges per gang are
+// multiples of 32 rows.  The real basis is held as bf16 hi and lo planes of
+// symbf16_records(Kp) records of eight K rows (2 x records x Np x 16 bytes),
+// made from W by split_R_bf16.
+Bf16Geometry contract_symbf16_geometry(int Np, int nw, int nf);
+hipError_t contract_symbf16_launch(int nw, int nf, int mb, const ContractArgs& a,
+                                   hipStream_t stream);
+int64_t symbf16_records(int64_t Kp);
+hipError_t split_R_bf16(const double2* W64, int64_t rows, int N, int Np, int64_t nq, void* Rb,
                         hipStream_t stream);
 
 }  // namespace klsp

@@ -1,0 +1,397 @@
+// klsp_contract_symbf16.hip -- the M(omega) contraction in the real basis of
+// a symmetric alpha grid on BF16 MFMA with three-term split products, for
+// low-precision builds (Taylor passes of the mixed-precision Newton search,
+// BF16x3 dets and scans).
+//
+// Real basis (see klsp_contract_sym64.hip): S = P + iQ with
+// P = R^T diag(Re c) R and Q = R^T diag(Im c) R over K, upper-triangle blocks
+// only; the reduce expands S into M.
+//
+// Storage.  R is split once into bf16 hi/lo parts, x = hi + lo, hi = bf16(x),
+// lo = bf16(x - hi), in two planes of records of eight K rows:
+//   Rh[K/8][Np] = {hi(8q), ..., hi(8q+7)}   (16 bytes), Rl alike,
+// so one 16-byte load is one 16x16x32 bf16 MFMA operand (lane group g of a
+// stage of 32 K rows holds rows 8g..8g+7).
+//
+// Products.  B = Re c * x and Im c * x are formed in FP32 from hi + lo of the
+// B-side columns and split the same way; a real product x y is taken as
+// xh yh + xl yh + xh yl (the dropped xl yl is ~2^-16 relative), so a block
+// and K-step of 32 rows take three MFMAs for P and three for Q.  Weight sets
+// from NF on take the single product xh yh (the mixed search's Taylor term
+// T_4, as in the complex BF16 kernel).  Accumulation is FP32 over a gang's K
+// range; gang partials are summed in FP64 by the reduce.
+//
+// Work decomposition.  The upper triangle of nb x nb blocks is folded: column
+// blocks c and nb - 1 - c form a pair holding (c + 1) + (nb - c) = nb + 1
+// blocks (the middle column of an odd nb alone holds (nb + 1) / 2), so equal
+// slot counts cover it with little idle work.  A workgroup is 4 waves of MB
+// block slots each over one pair (a pair with more than 4 MB blocks takes
+// several workgroups); per stage of 32 K rows its 256 threads form the B
+// operands of the pair's two column blocks for NW weight sets once and stage
+// them in LDS in operand order; each wave reads its A operands (the rows of
+// its slots) from global memory one stage ahead.  Gangs (the workgroups
+// covering every pair) walk contiguous K ranges on one XCD.
+#include <algorithm>
+
+#include "klsp_contract.hpp"
+
+namespace klsp {
+namespace {
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+typedef unsigned v2u __attribute__((ext_vector_type(2)));
+
+constexpr int kWaves = 4;
+constexpr int kThreads = kWaves * 64;
+constexpr int kCols = 32;          // B-side columns staged per workgroup (two blocks)
+constexpr int kMaxMB = 5;          // block slots per wave
+constexpr int kMaxNW = kMaxW;
+
+__device__ __forceinline__ unsigned pk_bf16(float a, float b)
+{
+    const v2f f = {a, b};
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(f, v2bf));
+}
+__device__ __forceinline__ float lo_f(unsigned u) { return __uint_as_float(u << 16); }
+__device__ __forceinline__ float hi_f(unsigned u) { return __uint_as_float(u & 0xffff0000u); }
+
+__device__ __forceinline__ v4f mfma(v4u a, v4u b, v4f c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a),
+                                                   __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
+}
+
+// B operands of one stage in operand order: rec[w][part][g][col], part =
+// P hi, P lo, Q hi, Q lo; K rows 8g..8g+7 of column col.
+template <int NW>
+struct StageB {
+    v4u rec[NW][4][4][kCols];
+};
+
+// The pair of column blocks of workgroup slice `pair` and its slot list:
+// slot s < na is block (s, ca), then (s - na, cb) for the second column.
+struct Pair {
+    int ca, cb, na, nslots;
+};
+
+__device__ __forceinline__ Pair pair_of(int p, int nb)
+{
+    Pair q;
+    q.ca = p;
+    q.cb = nb - 1 - p;
+    q.na = p + 1;
+    q.nslots = q.ca == q.cb ? q.na : nb + 1;
+    return q;
+}
+
+// B inputs of one stage for this thread: rows 4 rg .. 4 rg + 3 (rg = tid / 32)
+// of staged column tid % 32, hi and lo (4 bf16 each), and their weights.
+template <int NW>
+struct BIn {
+    v2u h, l;
+    v2f c[NW][4];
+};
+
+template <int NW>
+__device__ __forceinline__ void load_B_in(const unsigned* __restrict__ Rh32,
+                                          const unsigned* __restrict__ Rl32,
+                                          const float2* __restrict__ cwf, int64_t Kpc,
+                                          int64_t K0, int Np, int col_g, bool cvalid, int rg,
+                                          BIn<NW>& o)
+{
+    const int64_t q = (K0 >> 3) + (rg >> 1);
+    if (cvalid) {
+        const int64_t x = (q * Np + col_g) * 4 + 2 * (rg & 1);   // dwords
+        o.h = (v2u){Rh32[x], Rh32[x + 1]};
+        o.l = (v2u){Rl32[x], Rl32[x + 1]};
+    } else {
+        o.h = (v2u){0, 0};
+        o.l = (v2u){0, 0};
+    }
+#pragma unroll
+    for (int w = 0; w < NW; ++w)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const float2 c = cwf[w * Kpc + K0 + 4 * rg + t];
+            o.c[w][t] = (v2f){c.x, c.y};
+        }
+}
+
+template <int NW, int NF>
+__device__ __forceinline__ void produce_B(StageB<NW>& sb, const BIn<NW>& in, int rg, int col)
+{
+    float x[4];
+    x[0] = lo_f(in.h.x) + lo_f(in.l.x);
+    x[1] = hi_f(in.h.x) + hi_f(in.l.x);
+    x[2] = lo_f(in.h.y) + lo_f(in.l.y);
+    x[3] = hi_f(in.h.y) + hi_f(in.l.y);
+    const int g = rg >> 1, d = 2 * (rg & 1);
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        float bp[4], bq[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            bp[t] = in.c[w][t].x * x[t];
+            bq[t] = in.c[w][t].y * x[t];
+        }
+        const unsigned ph0 = pk_bf16(bp[0], bp[1]), ph1 = pk_bf16(bp[2], bp[3]);
+        const unsigned qh0 = pk_bf16(bq[0], bq[1]), qh1 = pk_bf16(bq[2], bq[3]);
+        unsigned* r0 = (unsigned*)&sb.rec[w][0][g][col];
+        unsigned* r2 = (unsigned*)&sb.rec[w][2][g][col];
+        r0[d] = ph0;
+        r0[d + 1] = ph1;
+        r2[d] = qh0;
+        r2[d + 1] = qh1;
+        if (w < NF) {
+            unsigned* r1 = (unsigned*)&sb.rec[w][1][g][col];
+            unsigned* r3 = (unsigned*)&sb.rec[w][3][g][col];
+            r1[d] = pk_bf16(bp[0] - lo_f(ph0), bp[1] - hi_f(ph0));
+            r1[d + 1] = pk_bf16(bp[2] - lo_f(ph1), bp[3] - hi_f(ph1));
+            r3[d] = pk_bf16(bq[0] - lo_f(qh0), bq[1] - hi_f(qh0));
+            r3[d + 1] = pk_bf16(bq[2] - lo_f(qh1), bq[3] - hi_f(qh1));
+        }
+    }
+}
+
+// Stores to LDS done and visible, then the workgroup barrier.  Global loads
+// stay in flight (a __syncthreads fence would wait for them).
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+template <int MB>
+__device__ __forceinline__ void load_A(const v4u* __restrict__ Rh, const v4u* __restrict__ Rl,
+                                       int64_t q0, int Np, const int (&row0)[MB], int g, int i,
+                                       v4u (&Ah)[MB], v4u (&Al)[MB])
+{
+#pragma unroll
+    for (int s = 0; s < MB; ++s) {
+        if (row0[s] < 0) {
+            Ah[s] = (v4u){0, 0, 0, 0};
+            Al[s] = Ah[s];
+            continue;
+        }
+        const int64_t x = (q0 + g) * Np + row0[s] + i;
+        Ah[s] = Rh[x];
+        Al[s] = Rl[x];
+    }
+}
+
+template <int MB, int NW, int NF>
+__device__ __forceinline__ void mfma_stage(const StageB<NW>& sb, const v4u (&Ah)[MB],
+                                           const v4u (&Al)[MB], const int (&csel)[MB],
+                                           v4f (&P)[NW][MB], v4f (&Q)[NW][MB], int g, int i)
+{
+#pragma unroll
+    for (int s = 0; s < MB; ++s) {
+        if (csel[s] < 0) continue;
+        const int col = 16 * csel[s] + i;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const v4u ph = sb.rec[w][0][g][col], qh = sb.rec[w][2][g][col];
+            P[w][s] = mfma(Ah[s], ph, P[w][s]);
+            Q[w][s] = mfma(Ah[s], qh, Q[w][s]);
+            if (w < NF) {
+                const v4u pl = sb.rec[w][1][g][col], ql = sb.rec[w][3][g][col];
+                P[w][s] = mfma(Al[s], ph, P[w][s]);
+                Q[w][s] = mfma(Al[s], qh, Q[w][s]);
+                P[w][s] = mfma(Ah[s], pl, P[w][s]);
+                Q[w][s] = mfma(Ah[s], ql, Q[w][s]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+template <int MB, int NW, int NF>
+__global__ __launch_bounds__(kThreads) void k_contract_symbf16(ContractArgs a)
+{
+    __shared__ StageB<NW> sb[2];
+    const int L = blockIdx.x;
+    const int xcd = L & 7, j = L >> 3;
+    const int gw = a.wrows * a.wcols;         // workgroups per gang: wcols pairs x wrows
+    const int gang = xcd + 8 * (j / gw), widx = j % gw;
+    if (gang >= a.n_gangs) return;            // whole workgroup
+    const int Np = a.Np, nb = Np / 16;
+    const Pair pr = pair_of(widx / a.wrows, nb);
+    const int sub = widx % a.wrows;           // workgroup of the pair
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, i = lane & 15, g = lane >> 4;
+    // this wave's slots: row-block offsets and staged column block (-1: idle)
+    int row0[MB], csel[MB];
+#pragma unroll
+    for (int s = 0; s < MB; ++s) {
+        const int sl = (sub * kWaves + wave) * MB + s;
+        if (sl < pr.na) { row0[s] = 16 * sl; csel[s] = 0; }
+        else if (sl < pr.nslots) { row0[s] = 16 * (sl - pr.na); csel[s] = 1; }
+        else { row0[s] = -1; csel[s] = -1; }
+    }
+    const int64_t Kp32 = (a.Kp + 31) / 32 * 32;
+    const int64_t Kb = min((int64_t)gang * a.Kg, Kp32);
+    const int64_t Ke = min(Kp32, Kb + a.Kg);
+    const int64_t nq = Kp32 / 8;              // records per plane column
+    const v4u* __restrict__ Rh = (const v4u*)a.W;
+    const v4u* __restrict__ Rl = Rh + nq * Np;
+    const int64_t Kpc = a.Kp + kRowPad;
+    // B-producing role: staged column col (two blocks: ca, cb), rows 4 rg..
+    const int col = tid % kCols, rg = tid / kCols;
+    const int col_g = (col < 16 ? 16 * pr.ca : 16 * pr.cb) + (col & 15);
+    const bool cvalid = col < 16 || pr.cb != pr.ca;
+    v4f P[NW][MB], Q[NW][MB];
+#pragma unroll
+    for (int w = 0; w < NW; ++w)
+#pragma unroll
+        for (int s = 0; s < MB; ++s) {
+            P[w][s] = (v4f){0, 0, 0, 0};
+            Q[w][s] = (v4f){0, 0, 0, 0};
+        }
+    v4u Ah[MB], Al[MB], Ahn[MB], Aln[MB];
+    BIn<NW> bin, binn;
+    if (Kb < Ke) {
+        load_A<MB>(Rh, Rl, Kb >> 3, Np, row0, g, i, Ah, Al);
+        load_B_in<NW>((const unsigned*)Rh, (const unsigned*)Rl, a.cwf, Kpc, Kb, Np, col_g,
+                      cvalid, rg, bin);
+    }
+    int buf = 0;
+    for (int64_t K0 = Kb; K0 < Ke; K0 += 32) {
+        const int64_t Kn = K0 + 32 < Ke ? K0 + 32 : K0;   // the next stage (the last repeats)
+        produce_B<NW, NF>(sb[buf], bin, rg, col);
+        load_B_in<NW>((const unsigned*)Rh, (const unsigned*)Rl, a.cwf, Kpc, Kn, Np, col_g,
+                      cvalid, rg, binn);
+        load_A<MB>(Rh, Rl, Kn >> 3, Np, row0, g, i, Ahn, Aln);
+        lds_barrier();
+        mfma_stage<MB, NW, NF>(sb[buf], Ah, Al, csel, P, Q, g, i);
+        bin = binn;
+#pragma unroll
+        for (int s = 0; s < MB; ++s) {
+            Ah[s] = Ahn[s];
+            Al[s] = Aln[s];
+        }
+        buf ^= 1;
+    }
+    // FP32 accumulator lanes hold rows 4g + v of a block (column i); partial
+    // slices are column-major: column = B-side real column
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        double2* Pw = a.P + ((int64_t)w * a.n_gangs + gang) * (int64_t)Np * Np;
+#pragma unroll
+        for (int s = 0; s < MB; ++s) {
+            if (csel[s] < 0) continue;
+            const int c0 = 16 * (csel[s] ? pr.cb : pr.ca);
+            double2* dst = Pw + (int64_t)(c0 + i) * Np + row0[s] + 4 * g;
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                dst[v] = make_double2((double)P[w][s][v], (double)Q[w][s][v]);
+        }
+    }
+}
+
+// W (complex128 rows of K, `rows` rows present) -> the real basis's bf16 hi
+// and lo planes of nq records of eight K rows (columns j < nU from Re W[h+j],
+// nU <= j < N from Im W[j], zero beyond N).
+__global__ void k_W64_to_Rbf16(const double2* __restrict__ W, int64_t rows, int N, int Np,
+                               int64_t nq, v4u* __restrict__ Rb)
+{
+    const int h = N >> 1, nU = N - h;
+    const int64_t n = nq * Np;
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
+         x += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = x / Np;
+        const int col = (int)(x - q * Np);
+        double d[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int64_t k = 8 * q + t;
+            d[t] = 0.0;
+            if (k < rows && col < N) d[t] = col < nU ? W[k * Np + h + col].x : W[k * Np + col].y;
+        }
+        unsigned hw[4], lw[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            hw[t] = pk_bf16((float)d[2 * t], (float)d[2 * t + 1]);
+            lw[t] = pk_bf16((float)(d[2 * t] - (double)lo_f(hw[t])),
+                            (float)(d[2 * t + 1] - (double)hi_f(hw[t])));
+        }
+        Rb[x] = (v4u){hw[0], hw[1], hw[2], hw[3]};
+        Rb[n + x] = (v4u){lw[0], lw[1], lw[2], lw[3]};
+    }
+}
+
+template <int NW, int NF>
+const void* kernel_nw(int mb)
+{
+    switch (mb) {
+    case 1: return (const void*)k_contract_symbf16<1, NW, NF>;
+    case 2: return (const void*)k_contract_symbf16<2, NW, NF>;
+    case 3: return (const void*)k_contract_symbf16<3, NW, NF>;
+    case 4: return (const void*)k_contract_symbf16<4, NW, NF>;
+    default: return (const void*)k_contract_symbf16<5, NW, NF>;
+    }
+}
+
+// nf < nw only for the four Taylor terms of the mixed search (three full)
+const void* kernel_for(int nw, int nf, int mb)
+{
+    if (nw >= 4 && nf < 4) return kernel_nw<4, 3>(mb);
+    return nw <= 1 ? kernel_nw<1, 1>(mb) : nw == 2 ? kernel_nw<2, 2>(mb)
+         : nw == 3 ? kernel_nw<3, 3>(mb) : kernel_nw<4, 4>(mb);
+}
+
+}  // namespace
+
+// Slots per wave and workgroups per column pair: the fewest workgroups per
+// pair with at most max_mb slots per wave, then the fewest slots that hold
+// the pair's nb + 1 blocks.  max_mb keeps the accumulators and the prefetched
+// A operands within 256 VGPRs (two workgroups per CU): 5 slots for one or two
+// weight sets, 4 for three, 3 for four.
+Bf16Geometry contract_symbf16_geometry(int Np, int nw, int nf)
+{
+    const int nb = Np / 16;
+    const int need = nb + 1;                  // blocks of a full pair
+    const int max_mb = nw <= 2 ? kMaxMB : nw == 3 ? 4 : 3;
+    const int per_pair = (need + kWaves * max_mb - 1) / (kWaves * max_mb);
+    const int mb = (need + kWaves * per_pair - 1) / (kWaves * per_pair);
+    Bf16Geometry g;
+    g.wrows = per_pair;                       // workgroups per pair
+    g.wcols = (nb + 1) / 2;                   // pairs (the middle column of an odd nb alone)
+    g.mb = mb;
+    static int occ[kMaxNW + 1][2][kMaxMB + 1] = {};
+    const int i = nw < 1 ? 1 : (nw > kMaxNW ? kMaxNW : nw);
+    const int hh = (i >= 4 && nf < 4) ? 1 : 0;
+    if (occ[i][hh][mb] == 0) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_for(i, nf, mb), kThreads,
+                                                         0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        occ[i][hh][mb] = per_cu;
+    }
+    g.wg_per_cu = occ[i][hh][mb];
+    return g;
+}
+
+hipError_t contract_symbf16_launch(int nw, int nf, int mb, const ContractArgs& a,
+                                   hipStream_t stream)
+{
+    const unsigned grid = (unsigned)(a.n_gangs * a.wrows * a.wcols);
+    ContractArgs args = a;
+    void* params[] = {(void*)&args};
+    return hipLaunchKernel(kernel_for(nw, nf, mb), dim3(grid), dim3(kThreads), params, 0, stream);
+}
+
+int64_t symbf16_records(int64_t Kp) { return (Kp + 31) / 32 * 32 / 8; }
+
+hipError_t split_R_bf16(const double2* W64, int64_t rows, int N, int Np, int64_t nq, void* Rb,
+                        hipStream_t stream)
+{
+    const int64_t n = nq * Np;
+    const int grid = (int)std::min<int64_t>((n + 255) / 256, 1 << 20);
+    hipLaunchKernelGGL(k_W64_to_Rbf16, dim3(grid), dim3(256), 0, stream, W64, rows, N, Np, nq,
+                       (v4u*)Rb);
+    return hipGetLastError();
+}
+
+}  // namespace klsp

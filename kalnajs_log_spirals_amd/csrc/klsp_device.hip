@@ -699,26 +699,29 @@ __global__ __launch_bounds__(kRedElems * kRedParts) void k_reduce(
     }
 }
 
-// Symmetric-basis launches: S = P + iQ from the partial slices of the
-// upper-triangle tiles (element (i, j), i <= j, sums the first
-// cnt[i/16 + (j/16) Np/16] slices of its block; within a diagonal block the
-// element above the diagonal stands for both), stored symmetric as
-// S[i + N j] = S[j + N i].  Work split as in k_reduce.
+// Symmetric-basis launches: S_b = P + iQ for weight sets b < nb from the
+// partial slices ([nb][nsl][Np][Np]) of the upper-triangle tiles (element
+// (i, j), i <= j, sums the first cnt[i/16 + (j/16) Np/16] slices of its
+// block, or all nsl without cnt; within a diagonal block the element above
+// the diagonal stands for both), stored symmetric as S[i + N j] = S[j + N i].
+// Work split as in k_reduce.
 __global__ __launch_bounds__(kRedElems * kRedParts) void k_reduce_sym(
-    const double2* __restrict__ P, const int* __restrict__ cnt, int Np, int N,
+    const double2* __restrict__ P, int nsl, const int* __restrict__ cnt, int Np, int N, int nb,
     double2* __restrict__ S)
 {
     __shared__ double2 part[kRedParts][kRedElems];
-    const int64_t n = (int64_t)N * N;
+    const int64_t NN = (int64_t)N * N, n = nb * NN;
     const int e = threadIdx.x % kRedElems, q = threadIdx.x / kRedElems;
     for (int64_t x0 = (int64_t)blockIdx.x * kRedElems; x0 < n; x0 += (int64_t)gridDim.x * kRedElems) {
         const int64_t x = x0 + e;
-        const int i = (int)(x % N), j = (int)(x / N);
+        const int b = (int)(x / NN);
+        const int64_t r = x - b * NN;
+        const int i = (int)(r % N), j = (int)(r / N);
         const bool on = x < n && i <= j;
         double sr = 0.0, si = 0.0;
         if (on) {
-            const double2* src = P + (int64_t)j * Np + i;
-            const int ns = cnt[(i >> 4) + (j >> 4) * (Np >> 4)];
+            const double2* src = P + (int64_t)b * nsl * Np * Np + (int64_t)j * Np + i;
+            const int ns = cnt ? cnt[(i >> 4) + (j >> 4) * (Np >> 4)] : nsl;
             for (int s = q; s < ns; s += kRedParts) {
                 const double2 v = src[(int64_t)s * Np * Np];
                 sr += v.x;
@@ -735,23 +738,27 @@ __global__ __launch_bounds__(kRedElems * kRedParts) void k_reduce_sym(
                 ti += part[p][e].y;
             }
             S[x] = make_double2(tr, ti);
-            S[(int64_t)i * N + j] = make_double2(tr, ti);
+            S[b * NN + (int64_t)i * N + j] = make_double2(tr, ti);
         }
         __syncthreads();
     }
 }
 
-// M from S (see klsp_contract_sym64.hip): with a' = max(a, N-1-a),
+// M_m from S_m for nb matrices (see klsp_contract_sym64.hip): with
+// a' = max(a, N-1-a),
 // u = a' - N/2, v = a' if a' >= N - N/2, s = +1 (a = a') or -1,
 //   H[b, a] = S[ub, ua] + i sb S[vb, ua] - i sa S[ub, va] + sb sa S[vb, va]
 // (terms without a v dropped), then M = H * G * N[a] as k_reduce does.
-__global__ void k_expand_sym(const double2* __restrict__ S, int N, double G,
-                             const double* __restrict__ Nk, double2* __restrict__ M)
+__global__ void k_expand_sym(const double2* __restrict__ Sall, int N, int nb, double G,
+                             const double* __restrict__ Nk, double2* __restrict__ Mall)
 {
     const int h = N >> 1, nU = N - h;
-    const int64_t n = (int64_t)N * N;
-    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
-         x += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t NN = (int64_t)N * N, n = nb * NN;
+    for (int64_t xx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; xx < n;
+         xx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t m = xx / NN, x = xx - m * NN;
+        const double2* S = Sall + m * NN;
+        double2* M = Mall + m * NN;
         const int b = (int)(x % N), a = (int)(x / N);
         const int bp = b >= h ? b : N - 1 - b, ap = a >= h ? a : N - 1 - a;
         const double sb = bp == b ? 1.0 : -1.0, sa = ap == a ? 1.0 : -1.0;
@@ -1058,6 +1065,8 @@ struct klsp_ctx {
     int use_sym = 1;
     bool last_sym = false;       // the last contraction launch was symmetric-basis
     double2* d_S = nullptr;   size_t S_cap = 0;   // S = P + iQ of that launch
+    uint4* d_Rb = nullptr;    size_t Rb_cap = 0;  // bf16 hi/lo planes of R (BF16x3 builds)
+    bool have_Rb = false;
     int* d_flag = nullptr;
     // profiling
     bool prof = false;
@@ -1360,6 +1369,35 @@ static GangPlan plan_gangs_bf16(const klsp_ctx* ctx, int nw, int nf)
     return gp;
 }
 
+// Gang geometry of the real-basis BF16 contraction: folded column pairs
+// (wcols) x workgroups per pair (wrows); gangs per XCD as for the complex
+// kernel; K ranges per gang in multiples of 32 rows.
+static GangPlan plan_gangs_symbf16(const klsp_ctx* ctx, int nw, int nf, int* mb)
+{
+    const Bf16Geometry geo = contract_symbf16_geometry(ctx->Np, nw, nf);
+    GangPlan gp;
+    gp.wrows = geo.wrows;
+    gp.wcols = geo.wcols;
+    *mb = geo.mb;
+    const int gw = gp.wrows * gp.wcols;
+    const int slots = std::max(1, ctx->n_cu / 8) * std::max(1, geo.wg_per_cu);
+    int gpx = 0, xmin = 0;
+    double best = 2.0;
+    for (int x = std::max(1, (2 * slots) / gw); x <= std::max(1, (12 * slots) / gw); ++x) {
+        const double r = (double)x * gw / slots, rounds = std::ceil(r);
+        const double tail = (rounds - r) / rounds;
+        if (tail <= 0.02) { gpx = x; break; }
+        if (tail < best) { best = tail; xmin = x; }
+    }
+    if (gpx == 0) gpx = std::max(1, xmin);
+    const int64_t Kp32 = round_up(ctx->Kp, 32);
+    while (gpx > 1 && (int64_t)gpx * 8 * 256 > Kp32) --gpx;
+    gp.gangs_per_xcd = gpx;
+    gp.n_gangs = 8 * gpx;
+    gp.Kg = round_up((Kp32 + gp.n_gangs - 1) / gp.n_gangs, 32);
+    return gp;
+}
+
 static ContractShape shape_for(int precision, int nw)
 {
     return precision == KLSP_PREC_FP64 ? contract_f64_shape(nw) : contract_f32_shape(nw);
@@ -1377,8 +1415,13 @@ static int max_gangs(klsp_ctx* ctx, int* rc)
             if ((*rc = get_balanced(ctx, prec, nw, &b))) return 0;
             g = std::max(g, b->n_slots);
         }
-    for (int nw = 1; nw <= kMaxW; ++nw) g = std::max(g, plan_gangs_bf16(ctx, nw, nw).n_gangs);
+    int mb = 0;
+    for (int nw = 1; nw <= kMaxW; ++nw) {
+        g = std::max(g, plan_gangs_bf16(ctx, nw, nw).n_gangs);
+        g = std::max(g, plan_gangs_symbf16(ctx, nw, nw, &mb).n_gangs);
+    }
     g = std::max(g, plan_gangs_bf16(ctx, kMaxW, kMaxW - 1).n_gangs);
+    g = std::max(g, plan_gangs_symbf16(ctx, kMaxW, kMaxW - 1, &mb).n_gangs);
     return g;
 }
 
@@ -1401,7 +1444,7 @@ static int free_problem(klsp_ctx* ctx)
     hipFree(ctx->d_DJ); hipFree(ctx->d_F0l); hipFree(ctx->d_Ores); hipFree(ctx->d_Nk);
     ctx->d_DJ = ctx->d_F0l = ctx->d_Ores = ctx->d_Nk = nullptr;
     ctx->have_problem = ctx->have_W = ctx->have_arrays = false;
-    ctx->have_W32 = ctx->have_Wb = ctx->have_R = false;
+    ctx->have_W32 = ctx->have_Wb = ctx->have_R = ctx->have_Rb = false;
     ctx->sym = -1;
     return KLSP_OK;
 }
@@ -1410,12 +1453,14 @@ static int free_problem(klsp_ctx* ctx)
 static void release_W(klsp_ctx* ctx)
 {
     hipFree(ctx->d_W64); hipFree(ctx->d_W32); hipFree(ctx->d_Wb); hipFree(ctx->d_R);
+    hipFree(ctx->d_Rb);
     ctx->d_W64 = nullptr;
     ctx->d_W32 = nullptr;
     ctx->d_Wb = nullptr;
     ctx->d_R = nullptr;
-    ctx->W64_cap = ctx->W32_cap = ctx->Wb_cap = ctx->R_cap = 0;
-    ctx->have_W = ctx->have_W32 = ctx->have_Wb = ctx->have_R = false;
+    ctx->d_Rb = nullptr;
+    ctx->W64_cap = ctx->W32_cap = ctx->Wb_cap = ctx->R_cap = ctx->Rb_cap = 0;
+    ctx->have_W = ctx->have_W32 = ctx->have_Wb = ctx->have_R = ctx->have_Rb = false;
     ctx->sym = -1;
 }
 
@@ -1567,7 +1612,7 @@ try {
 static int alloc_W64(klsp_ctx* ctx)
 {
     const size_t n = (size_t)(ctx->Kp + kRowPad) * ctx->Np;
-    ctx->have_W = ctx->have_W32 = ctx->have_Wb = ctx->have_R = false;
+    ctx->have_W = ctx->have_W32 = ctx->have_Wb = ctx->have_R = ctx->have_Rb = false;
     ctx->sym = -1;
     if (n > ctx->W64_cap) release_W(ctx);
     int rc = ensure(ctx, ctx->d_W64, ctx->W64_cap, n);
@@ -1676,15 +1721,13 @@ static int W_from_orbits(klsp_ctx* ctx, int64_t nwa, const OrbitBufs& ob, const 
     return KLSP_OK;
 }
 
-// The real basis of the current W for FP64 one-omega builds, made on first
-// use: the mirror symmetry is checked (unless the quadrature guaranteed it)
-// and, where it holds, R is filled from W (12.5 GB written, 25 GB read at the
-// default grid: a few ms, once per W).
-static int prepare_sym(klsp_ctx* ctx)
+// Mirror symmetry of the current W, checked once value by value (W from the
+// mirrored quadrature is symmetric by construction and is not checked).
+static int check_sym(klsp_ctx* ctx)
 {
-    if (!ctx->use_sym || ctx->sym == 0 || ctx->have_R || !ctx->have_W) return KLSP_OK;
+    if (ctx->sym >= 0 || !ctx->have_W) return KLSP_OK;
     const int64_t rows = ctx->Kp + kRowPad;
-    if (ctx->sym < 0) {
+    {
         int flag = 1;
         KLSP_HIP(ctx, hipMemsetAsync(ctx->d_flag, 0, sizeof(int), ctx->stream));
         hipLaunchKernelGGL(k_W_mirror_check, dim3(grid_for(rows * (ctx->N - ctx->N / 2))),
@@ -1695,9 +1738,20 @@ static int prepare_sym(klsp_ctx* ctx)
                                      ctx->stream));
         KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
         ctx->sym = flag ? 0 : 1;
-        if (!ctx->sym) return KLSP_OK;
     }
-    int rc = ensure(ctx, ctx->d_R, ctx->R_cap, (size_t)rows * ctx->Np);
+    return KLSP_OK;
+}
+
+// The real basis of the current W for FP64 one-omega builds, made on first
+// use where W is symmetric (12.5 GB written, 25 GB read at the default grid:
+// a few ms, once per W).
+static int prepare_sym(klsp_ctx* ctx)
+{
+    if (!ctx->use_sym || ctx->sym == 0 || ctx->have_R || !ctx->have_W) return KLSP_OK;
+    int rc = check_sym(ctx);
+    if (rc || ctx->sym != 1) return rc;
+    const int64_t rows = ctx->Kp + kRowPad;
+    rc = ensure(ctx, ctx->d_R, ctx->R_cap, (size_t)rows * ctx->Np);
     if (rc) return rc;
     hipLaunchKernelGGL(k_W64_to_R, dim3(grid_for(rows * ctx->Np)), dim3(256), 0, ctx->stream,
                        ctx->d_W64, rows, (int)ctx->N, ctx->Np, ctx->d_R);
@@ -1871,7 +1925,17 @@ try {
         KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
         ctx->have_W32 = true;
     }
-    if (precision == KLSP_PREC_BF16X3 && !ctx->have_Wb) {
+    if (precision == KLSP_PREC_BF16X3 && (rc = check_sym(ctx))) return rc;
+    if (precision == KLSP_PREC_BF16X3 && ctx->use_sym && ctx->sym == 1 && !ctx->have_Rb) {
+        // bf16 hi and lo planes of the real basis
+        const int64_t nq = symbf16_records(ctx->Kp);
+        if ((rc = ensure(ctx, ctx->d_Rb, ctx->Rb_cap, (size_t)2 * nq * ctx->Np))) return rc;
+        KLSP_HIP(ctx, split_R_bf16(ctx->d_W64, ctx->Kp + kRowPad, (int)ctx->N, ctx->Np, nq,
+                                   ctx->d_Rb, ctx->stream));
+        KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->have_Rb = true;
+    }
+    if (precision == KLSP_PREC_BF16X3 && !(ctx->use_sym && ctx->sym == 1) && !ctx->have_Wb) {
         KLSP_HIP(ctx, hipSetDevice(ctx->device));
         const int64_t nq = round_up(ctx->Kp, 16) / 4;
         // two planes (hi, lo) of nq x Np 16-byte groups of four K rows
@@ -1917,6 +1981,28 @@ static int launch_contract(klsp_ctx* ctx, int precision, int nb, int nf)
             return fail(ctx, KLSP_ERR_HIP, std::string("contraction launch: ") + hipGetErrorString(e));
         ctx->last_S = bal->n_slots;
         ctx->last_count = bal->d_count;
+        ctx->last_sym = true;
+        return KLSP_OK;
+    }
+    if (bf && ctx->use_sym && ctx->sym == 1 && ctx->have_Rb) {
+        int mb = 0;
+        gp = plan_gangs_symbf16(ctx, nb, nf, &mb);
+        ContractArgs a{};
+        a.W = ctx->d_Rb;
+        a.cw = ctx->d_c;
+        a.cwf = ctx->d_cf;
+        a.P = ctx->d_P;
+        a.Kp = ctx->Kp;
+        a.Np = ctx->Np;
+        a.Kg = gp.Kg;
+        a.n_gangs = gp.n_gangs;
+        a.wrows = gp.wrows;
+        a.wcols = gp.wcols;
+        const hipError_t e = contract_symbf16_launch(nb, nf, mb, a, ctx->stream);
+        if (e != hipSuccess)
+            return fail(ctx, KLSP_ERR_HIP, std::string("contraction launch: ") + hipGetErrorString(e));
+        ctx->last_S = gp.n_gangs;
+        ctx->last_count = nullptr;
         ctx->last_sym = true;
         return KLSP_OK;
     }
@@ -1981,9 +2067,11 @@ static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
             const klsp_ctx::Balanced* bs = nullptr;
             if ((rc = get_balanced_sym(ctx, &bs))) return rc;
             mg = std::max(mg, bs->n_slots);
-            if ((rc = ensure(ctx, ctx->d_S, ctx->S_cap, (size_t)ctx->N * ctx->N))) return rc;
         }
     }
+    if (ctx->use_sym && ctx->sym == 1 &&
+        (rc = ensure(ctx, ctx->d_S, ctx->S_cap, (size_t)kMaxW * ctx->N * ctx->N)))
+        return rc;
     if ((rc = ensure(ctx, ctx->d_P, ctx->P_cap, (size_t)kMaxW * mg * ctx->Np * ctx->Np)))
         return rc;
     if ((rc = ensure(ctx, ctx->d_M, ctx->M_cap,
@@ -2006,11 +2094,12 @@ static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
     if (ctx->prof) hipEventRecord(ctx->ev[2], ctx->stream);
     if (ctx->last_sym) {
         const int64_t NN = ctx->N * ctx->N;
-        hipLaunchKernelGGL(k_reduce_sym, dim3(grid_for(NN, kRedElems)), dim3(kRedElems * kRedParts),
-                           0, ctx->stream, ctx->d_P, ctx->last_count, ctx->Np, (int)ctx->N,
-                           ctx->d_S);
-        hipLaunchKernelGGL(k_expand_sym, dim3(grid_for(NN)), dim3(256), 0, ctx->stream, ctx->d_S,
-                           (int)ctx->N, ctx->G, ctx->d_Nk, ctx->d_M + (size_t)slot * NN);
+        hipLaunchKernelGGL(k_reduce_sym, dim3(grid_for(nb * NN, kRedElems)),
+                           dim3(kRedElems * kRedParts), 0, ctx->stream, ctx->d_P, ctx->last_S,
+                           ctx->last_count, ctx->Np, (int)ctx->N, nb, ctx->d_S);
+        hipLaunchKernelGGL(k_expand_sym, dim3(grid_for(nb * NN)), dim3(256), 0, ctx->stream,
+                           ctx->d_S, (int)ctx->N, nb, ctx->G, ctx->d_Nk,
+                           ctx->d_M + (size_t)slot * NN);
     } else {
         hipLaunchKernelGGL(k_reduce, dim3(grid_for((int64_t)nb * ctx->N * ctx->N, kRedElems)),
                            dim3(kRedElems * kRedParts), 0,
@@ -2078,7 +2167,12 @@ static int check_ready(klsp_ctx* ctx, int precision, const char* who)
         precision != KLSP_PREC_BF16X3)
         return fail(ctx, KLSP_ERR_INVALID_ARG, std::string(who) + ": unknown precision");
     if (precision == KLSP_PREC_FP32 && !ctx->have_W32) return klsp_prepare(ctx, precision);
-    if (precision == KLSP_PREC_BF16X3 && !ctx->have_Wb) return klsp_prepare(ctx, precision);
+    if (precision == KLSP_PREC_BF16X3 && !ctx->have_Wb && !ctx->have_Rb)
+        return klsp_prepare(ctx, precision);
+    if (precision == KLSP_PREC_BF16X3 && ctx->use_sym && ctx->sym == 1 && !ctx->have_Rb)
+        return klsp_prepare(ctx, precision);
+    if (precision == KLSP_PREC_BF16X3 && !(ctx->use_sym && ctx->sym == 1) && !ctx->have_Wb)
+        return klsp_prepare(ctx, precision);
     return KLSP_OK;
 }
 

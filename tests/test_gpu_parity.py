@@ -835,3 +835,41 @@ def test_symmetric_basis_every_variant(N):
             pre.ctx.set_variant(len(times))
     finally:
         pre.ctx.close()
+
+
+@pytest.mark.parametrize("N", [5, 17, 33, 130, 301, 330, 401])
+def test_symmetric_basis_bf16x3(N):
+    """BF16x3 builds in the real basis (folded column pairs; one, two or
+    several workgroups per pair as N grows): M within 1e-4 normwise of the
+    FP64 build, the three- and four-term Taylor passes within 1e-4 of the FP64
+    terms (T_4 by the single hi x hi product: 1e-2), and the same det as the
+    complex BF16x3 kernel to the precision of either."""
+    rng = np.random.default_rng(3000 + N)
+    NPh, n_l = 157, 2
+    W = mirror_symmetric_W(rng, NPh, N, n_l, 1.0 / np.sqrt(NPh * n_l))
+    DJ, F0l = rng.uniform(0.2, 1.0, NPh), rng.standard_normal((NPh, n_l))
+    Ores, Nk = rng.uniform(-2, 2, (NPh, n_l)), rng.uniform(0.5, 1.5, N)
+    pre = from_arrays(2, 0.8, W, DJ, F0l, Ores, Nk)
+    try:
+        assert pre.ctx.symmetric_basis()
+        om = 0.4 + 0.25j
+        M64 = pre.ctx.build_M(om, _ffi.PREC_FP64)
+        Mb = pre.ctx.build_M(om, _ffi.PREC_BF16X3)
+        assert relnorm(Mb, M64) <= 1e-4, relnorm(Mb, M64)
+        T64 = pre.ctx.build_taylor(om, 3, _ffi.PREC_FP64)
+        for nt in (2, 3):
+            Tb = pre.ctx.build_taylor(om, nt, _ffi.PREC_BF16X3)
+            for j in range(nt):
+                assert relnorm(Tb[j], T64[j]) <= 1e-4, (nt, j)
+        oms = [om, 0.9 + 0.1j, 0.2 + 0.3j]
+        db = pre.ctx.det(oms, _ffi.PREC_BF16X3)
+        d64 = pre.ctx.det(oms, _ffi.PREC_FP64)
+        pre.ctx.set_symmetric_basis(False)
+        dc = pre.ctx.det(oms, _ffi.PREC_BF16X3)
+        pre.ctx.set_symmetric_basis(True)
+        for k in range(3):
+            sc = max(abs(d64[k]), 1.0)
+            assert abs(db[k] - d64[k]) <= 1e-3 * sc
+            assert abs(dc[k] - d64[k]) <= 1e-3 * sc
+    finally:
+        pre.ctx.close()
