@@ -4,12 +4,19 @@
 Workload (the drop-in FP64 GPU configuration): Toomre-Zang n=4, q1=7, m=2 on
 the default grid NR=501, Ne=51, N_alpha=301, alpha_max=30, l in [-100, 100]
 (K = 201*501*51 = 5 135 751), nw/nwa = 100001/2001.  A step is one
-det(I - M(omega)) evaluation: resonance weights, the K-long complex
-contraction (FP64 MFMA), the G*N epilogue and the FP64 complex LU, with W
-resident in HBM.  Also reported: Newton roots/s (FP64, from
-omega0 = 0.8794 + 0.1231i), the complex64/FP32-MFMA det rate, the W
-quadrature time, the mixed-precision Newton search (BF16x3 Taylor-model passes,
-FP64 finish; also with FP32 passes) with its distance to the FP64 root, and the CPU baseline
+det(I - M(omega)) evaluation with W resident in HBM: resonance weights, the
+K-long contraction (FP64 MFMA, in the real basis of the symmetric alpha
+grid: two real symmetric products), the G*N epilogue and the FP64 complex
+LU.  The contraction variant is chosen by the library's autotuner first.
+Also measured, with the other configurations of the workload set: Newton
+roots/s (FP64 reference iteration from omega0 = 0.8794 + 0.1231i), the
+mixed-precision search (BF16x3 Taylor-model passes, FP64 finish; also with
+FP32 passes) and its distance to the FP64 root, the FP32 and BF16x3 det
+rates, the whole 64 x 64 omega-grid scan in BF16x3 checked against FP64 dets
+at 64 SplitMix64 points, the 48-draw uneven batch of Newton searches in
+mixed precision and with the FP64 reference iteration (roots counted on
+both sides only where two FP64 Newton steps with tol = 0 move them by at
+most 1e-10), the W quadrature and orbit kernels, and the CPU baseline
 (numpy/OpenBLAS BLAS-form det: one det's worth of l-slab ZGEMMs over 8
 synthetic slabs plus one LU).
 
@@ -33,11 +40,18 @@ METRIC = ("det(I−M(ω)) evals/s + Newton roots/s, mixed vs FP64 GPU at "
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per launch of the FP64 contraction at the default grid, from two
-# rocprofv3 --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, calibrated on a
-# copy kernel of known size); profiles/det_default_grid.md holds the run.
-PROFILED_KERNEL = "klsp::k_contract<true, 4, 4, 1>"
-PROFILED_TRAFFIC_BYTES = 96.916e9 + 0.062e9
+# HBM bytes per launch of the real-basis FP64 contraction at the default grid,
+# from two rocprofv3 --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0,
+# calibrated on a copy kernel of known size); profiles/det_default_grid.md
+# holds the run.
+# kernel name -> FETCH + WRITE bytes per launch (None where not profiled)
+PROFILED_TRAFFIC_BYTES = {}
+
+
+def sym_kernel_name(variant_name: str) -> str:
+    """Kernel symbol of an FP64 real-basis variant ("sym64 4x4 d8 [fulldiag]")."""
+    depth = variant_name.split()[2][1:]
+    return f"klsp::k_contract_sym64<{depth}, {'true' if 'fulldiag' in variant_name else 'false'}>"
 
 
 def log(*a):
@@ -64,17 +78,19 @@ def main() -> int:
     ap.add_argument("--cpu-slabs", type=int, default=-1,
                     help="l-slab ZGEMMs timed for the CPU baseline (-1 = all n_l, 0 = skip)")
     ap.add_argument("--no-newton", action="store_true")
-    ap.add_argument("--uneven", type=int, default=0,
-                    help="also run this many draws of the 48-draw uneven batch of Newton "
-                         "searches (mixed precision; minutes at full size)")
-    ap.add_argument("--uneven-fp64", action="store_true",
-                    help="run the uneven draws in FP64 too and report the root differences")
+    ap.add_argument("--uneven", type=int, default=48,
+                    help="draws of the 48-draw uneven batch of Newton searches run in mixed "
+                         "precision and with the FP64 reference iteration (0 = skip)")
+    ap.add_argument("--no-uneven-fp64", action="store_true",
+                    help="skip the FP64 reference iteration over the uneven draws")
     ap.add_argument("--uneven-orbits", choices=("host", "device"), default="device",
                     help="where the uneven draws' orbits are integrated")
     ap.add_argument("--no-fp32", action="store_true")
-    ap.add_argument("--scan-full", action="store_true",
-                    help="also run the whole 64 x 64 omega-grid scan in BF16x3 and check it against "
-                         "FP64 dets at 64 SplitMix64-drawn grid points (about a minute)")
+    ap.add_argument("--no-scan", action="store_true",
+                    help="skip the whole 64 x 64 omega-grid scan in BF16x3 (checked against FP64 "
+                         "dets at 64 SplitMix64-drawn grid points)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default variant of the FP64 contraction")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -131,6 +147,14 @@ def main() -> int:
     pre_d.ctx.close()
     del pre_d
     ctx.set_profiling(True)
+    sym = ctx.symmetric_basis()
+    tune = {}
+    chosen = _ffi.lib().klsp_variant_name(_ffi.PREC_FP64, 2).decode() if sym else "complex"
+    if sym and not args.no_autotune:
+        t0 = time.perf_counter()
+        tune, chosen = ctx.autotune(_ffi.PREC_FP64, reps=2)
+        log(f"[bench] autotune ({time.perf_counter() - t0:.2f} s): " +
+            ", ".join(f"{k} {v:.2f} ms" for k, v in tune.items()) + f" -> {chosen}")
 
     om0 = complex(2 * 0.4397, 0.1231)
     oms = [om0 + 0.001 * i * (1 + 1j) for i in range(args.warmup + max(args.steps, 3))]
@@ -153,13 +177,28 @@ def main() -> int:
     ms_per_step = 1e3 * elapsed / args.steps
     det_per_s = args.steps / elapsed
     avg_contract = contract_ms / max(launches, 1)
-    flop_build = 8.0 * Kdim * N * N
+    # complex product of the reference form (8 K N^2) and the real-basis work
+    # on a symmetric grid: two real symmetric rank-K products, upper
+    # triangles, 2 K N (N + 1) flop (the MFMAs also cover the 16-block padding
+    # of the triangle: nbk (nbk + 1) / 2 blocks of 16 x 16 for P and Q)
+    flop_complex = 8.0 * Kdim * N * N
+    nbk = -(-N // 16)
+    flop_sym = 2.0 * Kdim * N * (N + 1)
+    flop_sym_exec = 2.0 * 2.0 * Kdim * 256 * nbk * (nbk + 1) / 2
+    flop_build = flop_sym if sym else flop_complex
     achieved_tflops = flop_build / (avg_contract * 1e-3) / 1e12
     log(f"[bench] FP64 det: {ms_per_step:.2f} ms/det ({det_per_s:.3f} det/s); contraction "
-        f"{avg_contract:.2f} ms = {achieved_tflops:.1f} TFLOP/s; LU {lu_ms / args.steps:.2f} ms; "
+        f"{avg_contract:.2f} ms = {achieved_tflops:.1f} TFLOP/s algorithmic "
+        f"({'real basis' if sym else 'complex'}; {flop_complex / (avg_contract * 1e-3) / 1e12:.0f} "
+        f"TFLOP/s of the complex form); LU {lu_ms / args.steps:.2f} ms; "
         f"weights {weights_ms / args.steps:.3f} ms; reduce {reduce_ms / args.steps:.3f} ms")
 
     extra = {"det_per_s_fp64": det_per_s, "contract_ms_fp64": avg_contract,
+             "fp64_symmetric_basis": sym, "fp64_contract_variant": chosen,
+             "fp64_autotune_ms": tune,
+             "fp64_contract_tflops_complex_form": flop_complex / (avg_contract * 1e-3) / 1e12,
+             "fp64_contract_tflops_executed_mfma": (flop_sym_exec if sym else flop_complex)
+                                                   / (avg_contract * 1e-3) / 1e12,
              "lu_ms": lu_ms / args.steps, "weights_ms": weights_ms / args.steps,
              "reduce_ms": reduce_ms / args.steps, "W_quadrature_ms": W_ms,
              "host_orbits_s": t_orb, "precompute_s": t_pre,
@@ -186,7 +225,7 @@ def main() -> int:
         e32 = time.perf_counter() - t0
         rel = max(abs(d32[i] - dets[i]) / abs(dets[i]) for i in range(min(n32, len(dets))))
         extra.update({"det_per_s_fp32": n32 / e32, "contract_ms_fp32": c32 / n32,
-                      "fp32_contract_tflops": flop_build / (c32 / n32 * 1e-3) / 1e12,
+                      "fp32_contract_tflops": flop_complex / (c32 / n32 * 1e-3) / 1e12,
                       "fp32_det_rel_err_vs_fp64": rel})
         log(f"[bench] FP32 det: {1e3 * e32 / n32:.2f} ms/det, contraction {c32 / n32:.2f} ms, "
             f"det rel err vs FP64 {rel:.2e}")
@@ -203,34 +242,21 @@ def main() -> int:
             cb += ctx.last_timing()["contract_ms"]
         eb = time.perf_counter() - t0
         relb = max(abs(db[i] - dets[i]) / abs(dets[i]) for i in range(min(nb, len(dets))))
-        T = ctx.build_taylor(om0, 3, _ffi.PREC_BF16X3)
+        ctx.build_taylor(om0, 3, _ffi.PREC_BF16X3)
         tb3 = ctx.last_timing()["contract_ms"]
+        ctx.build_taylor(om0, 4, _ffi.PREC_BF16X3)
+        tb4 = ctx.last_timing()["contract_ms"]
+        # bf16 MFMA work of the three-term split products (3 per real product)
+        flop_bf = 3.0 * (flop_sym_exec if sym else flop_complex)
         extra.update({"det_per_s_bf16x3": nb / eb, "contract_ms_bf16x3": cb / nb,
-                      "bf16x3_contract_tflops_equiv": flop_build / (cb / nb * 1e-3) / 1e12,
+                      "bf16x3_contract_tflops_mfma": flop_bf / (cb / nb * 1e-3) / 1e12,
                       "bf16x3_det_rel_err_vs_fp64": relb,
-                      "taylor3_contract_ms_bf16x3": tb3})
-        log(f"[bench] BF16x3 det: {1e3 * eb / nb:.2f} ms/det, contraction {cb / nb:.2f} ms, "
-            f"det rel err vs FP64 {relb:.2e}; 3-term Taylor contraction {tb3:.2f} ms")
+                      "taylor3_contract_ms_bf16x3": tb3, "taylor4_contract_ms_bf16x3": tb4})
+        log(f"[bench] BF16x3 det: {1e3 * eb / nb:.2f} ms/det, contraction {cb / nb:.2f} ms "
+            f"({flop_bf / (cb / nb * 1e-3) / 1e15:.2f} PFLOP/s of bf16 MFMA), det rel err vs FP64 "
+            f"{relb:.2e}; 3-term Taylor contraction {tb3:.2f} ms, 4-term {tb4:.2f} ms")
 
-    if not args.no_fp32:
-        # omega-grid scan in BF16x3 (three omegas per contraction, all LUs of a
-        # batch of builds in one launch): one 64-point column of the 64 x 64
-        # window Re w/m in [0.30, 0.60], Im w in [0.02, 0.25], at Re w/m = 0.44
-        t0 = time.perf_counter()
-        col, _, _ = ctx.grid_scan(1, 64, 0.44, 0.44, 0.02, 0.25, precision=_ffi.PREC_BF16X3)
-        ts = time.perf_counter() - t0
-        gam = np.linspace(0.02, 0.25, 64)
-        sample = list(range(0, 64, 8))
-        ref = ctx.det([complex(2 * 0.44, gam[i]) for i in sample], _ffi.PREC_FP64)
-        med = float(np.median(np.abs(ref)))
-        err = max(abs(col[i, 0] - ref[k]) / max(abs(ref[k]), med) for k, i in enumerate(sample))
-        extra.update({"scan_dets_per_s_bf16x3": 64 / ts,
-                      "scan_64x64_s_bf16x3_est": 4096 * ts / 64,
-                      "scan_bf16x3_rel_err_vs_fp64": err})
-        log(f"[bench] BF16x3 scan column: {64 / ts:.1f} det/s (64 x 64 window ~{4096 * ts / 64:.1f} s), "
-            f"rel err vs FP64 at 8 points {err:.2e}")
-
-    if args.scan_full:
+    if not args.no_scan:
         # the whole window: Re w/m in linspace(0.30, 0.60, 64), Im w in linspace(0.02, 0.25, 64)
         from kalnajs_log_spirals_amd.prng import SplitMix64
         t0 = time.perf_counter()
@@ -294,46 +320,63 @@ def main() -> int:
             f"LUs {tmx['lu_ms']:.1f} ms")
 
     if args.uneven > 0:
+        # the uneven batch: mixed-precision searches, then the FP64 reference
+        # iteration from the same starts; on both sides a root counts only if
+        # two FP64 Newton steps with tol = 0 move it by at most 1e-10 (it is
+        # an FP64 root to that tolerance)
         from kalnajs_log_spirals_amd.UnevenBatch import draw_batch, run_batch
         ctx.close()
         draws = draw_batch(48, 0)[:args.uneven]
+
+        def good(rs):
+            return [r for r in rs if r.converged and r.polish_move <= 1e-10]
+
         t0 = time.perf_counter()
-        pol = 2 if args.uneven_fp64 else 0
-        rm = run_batch(draws, precision="mixed", log=log, polish=pol, orbits=args.uneven_orbits)
+        rm = run_batch(draws, precision="mixed", log=log, polish=2, orbits=args.uneven_orbits)
         wall = time.perf_counter() - t0
-        conv = [r for r in rm if r.converged]
+        gm = good(rm)
         tn = sum(r.newton_s for r in rm)
-        extra.update({"uneven_draws": len(draws), "uneven_converged_mixed": len(conv),
+        extra.update({"uneven_draws": len(draws),
+                      "uneven_converged_mixed": sum(r.converged for r in rm),
+                      "uneven_roots_mixed": len(gm),
                       "uneven_wall_s_mixed": wall, "uneven_newton_s_mixed": tn,
-                      "uneven_roots_per_s_mixed": len(conv) / tn if tn > 0 else 0.0,
-                      "uneven_W_s": sum(r.W_s for r in rm),
-                      "uneven_host_s": sum(r.host_s for r in rm),
+                      "uneven_roots_per_s_mixed": len(gm) / tn if tn > 0 else 0.0,
+                      "uneven_fp64_builds_mixed": sum(r.n_fp64_builds for r in rm),
+                      "uneven_lp_builds_mixed": sum(r.n_lp_builds for r in rm),
+                      "uneven_max_fp64_polish_move_mixed":
+                          max((r.polish_move for r in rm if r.converged), default=None),
+                      "uneven_precompute_s": sum(r.W_s for r in rm),
                       "uneven_orbits": args.uneven_orbits})
-        log(f"[bench] uneven batch (mixed): {len(conv)}/{len(draws)} converged, Newton {tn:.1f} s, "
-            f"wall {wall:.1f} s")
-        if args.uneven_fp64:
+        log(f"[bench] uneven batch (mixed): {len(gm)}/{len(draws)} FP64-grade roots, Newton "
+            f"{tn:.1f} s ({len(gm) / tn:.2f} roots/s), wall {wall:.1f} s")
+        if not args.no_uneven_fp64:
             t0 = time.perf_counter()
-            r64 = run_batch(draws, precision="fp64", log=log, polish=pol,
+            r64 = run_batch(draws, precision="fp64", log=log, polish=2,
                             orbits=args.uneven_orbits)
             wall64 = time.perf_counter() - t0
+            g64 = good(r64)
             tn64 = sum(r.newton_s for r in r64)
             both = [(a, b) for a, b in zip(rm, r64) if a.converged and b.converged]
-            # the same root: the polished FP64 root within 1e-6 of the mixed one
-            same = [(a, b) for a, b in both
-                    if abs(complex(a.draw.m * a.Omega_p, a.gamma) - b.polished) < 1e-6]
+            # the same root: the polished roots within 1e-6 of each other
+            same = [(a, b) for a, b in both if abs(a.polished - b.polished) < 1e-6]
             dws = [abs(complex(a.draw.m * a.Omega_p, a.gamma) - b.polished) for a, b in same]
-            moves = [a.polish_move for a in rm if a.converged]
+            rps64 = len(g64) / tn64 if tn64 > 0 else 0.0
+            rpsm = extra["uneven_roots_per_s_mixed"]
             extra.update({"uneven_converged_fp64": sum(r.converged for r in r64),
+                          "uneven_roots_fp64": len(g64),
                           "uneven_newton_s_fp64": tn64, "uneven_wall_s_fp64": wall64,
-                          "uneven_roots_per_s_fp64":
-                              sum(r.converged for r in r64) / tn64 if tn64 > 0 else 0.0,
+                          "uneven_roots_per_s_fp64": rps64,
+                          "uneven_roots_per_s_ratio": rpsm / rps64 if rps64 > 0 else None,
+                          "uneven_fp64_builds_fp64": sum(r.n_fp64_builds for r in r64),
                           "uneven_converged_both": len(both), "uneven_same_root": len(same),
                           "uneven_max_abs_domega_same_root": max(dws) if dws else None,
-                          "uneven_max_fp64_polish_move_mixed": max(moves) if moves else None})
-            log(f"[bench] uneven batch (FP64): {sum(r.converged for r in r64)} converged, Newton "
-                f"{tn64:.1f} s; {len(same)} of {len(both)} on the same root, max |d omega| there "
-                f"{max(dws) if dws else float('nan'):.2e}; largest FP64 polish move of a mixed "
-                f"root {max(moves) if moves else float('nan'):.2e}")
+                          "uneven_max_fp64_polish_move_fp64":
+                              max((r.polish_move for r in r64 if r.converged), default=None)})
+            log(f"[bench] uneven batch (FP64 reference iteration): {sum(r.converged for r in r64)} "
+                f"converged, {len(g64)} FP64-grade roots, Newton {tn64:.1f} s ({rps64:.3f} roots/s); "
+                f"mixed/FP64 roots/s {rpsm / rps64 if rps64 > 0 else float('nan'):.2f}x; "
+                f"{len(same)} of {len(both)} on the same root, max |d omega| there "
+                f"{max(dws) if dws else float('nan'):.2e}")
 
     cpu = None
     if args.cpu_slabs < 0:
@@ -371,6 +414,7 @@ def main() -> int:
             cpu["newton_roots_note"] = (f"derived: det/s over the {evals} builds the reference "
                                         "iteration takes from the same start")
 
+    kernel = sym_kernel_name(chosen) if sym else "klsp::k_contract<true, 4, 4, 1>"
     out = {
         "metric": METRIC, "value": det_per_s, "unit": "det/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
@@ -382,10 +426,13 @@ def main() -> int:
                    "q1": cfg.model.q1},
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS,
-                     "traffic": PROFILED_TRAFFIC_BYTES if args.config == "default" else None,
+                     "traffic": PROFILED_TRAFFIC_BYTES.get(kernel) if args.config == "default"
+                                else None,
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE)",
-                     "kernel": PROFILED_KERNEL, "flop_per_launch": flop_build,
-                     "algorithmic_bytes_per_launch": 16.0 * Kdim * N},
+                     "kernel": kernel, "flop_per_launch": flop_build,
+                     "flop_note": ("2 K N (N + 1): the two real symmetric rank-K products of the "
+                                   "real basis" if sym else "8 K N^2: the complex product"),
+                     "algorithmic_bytes_per_launch": (8.0 if sym else 16.0) * Kdim * N},
         "cpu_baseline": cpu,
         "extra": extra,
     }
