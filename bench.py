@@ -44,8 +44,16 @@ PEAK_HBM_GBS = 8000.0
 # from two rocprofv3 --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0,
 # calibrated on a copy kernel of known size); profiles/det_default_grid.md
 # holds the run.
-# kernel name -> FETCH + WRITE bytes per launch (None where not profiled)
-PROFILED_TRAFFIC_BYTES = {}
+# kernel name -> FETCH + WRITE bytes per launch (the autotuner may keep any
+# variant; each was profiled in the same run)
+PROFILED_TRAFFIC_BYTES = {
+    "klsp::k_contract_sym64<4, false>": 37.395e9 + 0.056e9,
+    "klsp::k_contract_sym64<4, true>": 40.111e9 + 0.062e9,
+    "klsp::k_contract_sym64<6, false>": 35.894e9 + 0.056e9,
+    "klsp::k_contract_sym64<6, true>": 35.778e9 + 0.062e9,
+    "klsp::k_contract_sym64<8, false>": 35.940e9 + 0.056e9,
+    "klsp::k_contract_sym64<8, true>": 35.946e9 + 0.062e9,
+}
 
 
 def sym_kernel_name(variant_name: str) -> str:
