@@ -251,6 +251,20 @@ int klsp_build_taylor(klsp_ctx* ctx, double omega_re, double omega_im, int n_ter
 int klsp_det(klsp_ctx* ctx, const double* omegas, int64_t n, int precision,
              double* dets_out);
 
+/* det(I - M(omega_i)) as klsp_det with KLSP_PREC_FP64, allocating nothing.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Its workspaces -- the weights, the K-piece partials and
+ * the N x N matrices, about 16 (Kp + 64) + 16 Np^2 P + 48 N^2 bytes with P
+ * the work table's K pieces (~211 MB at the default grid) -- are carved from
+ * one 256 MiB array per GPU that the library holds until the process ends;
+ * larger problems fail with KLSP_ERR_INVALID_ARG.  Needs
+ * klsp_prepare(ctx, KLSP_PREC_FP64) after the problem and W are on the device
+ * (which builds the real basis and the work table; KLSP_ERR_NOT_READY
+ * otherwise).  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (from several contexts or threads) race. */
+int klsp_det_noalloc(klsp_ctx* ctx, const double* omegas, int64_t n, double* dets_out);
+
 /* newton_search, NewtonSolver.jl:36 (NewtonResult :19). */
 typedef struct klsp_newton_params {
     int max_iter;          /* NewtonConfig.max_iter                         */

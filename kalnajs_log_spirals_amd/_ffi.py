@@ -45,7 +45,7 @@ EXPORTED = [
     "klsp_compute_jacobian_df", "klsp_orbit_phase_grid", "klsp_precompute_arrays",
     "klsp_upload_problem", "klsp_set_problem_arrays", "klsp_compute_orbits_W",
     "klsp_compute_W", "klsp_upload_W", "klsp_download_W", "klsp_prepare", "klsp_build_M",
-    "klsp_build_taylor", "klsp_det", "klsp_newton", "klsp_grid_scan",
+    "klsp_build_taylor", "klsp_det", "klsp_det_noalloc", "klsp_newton", "klsp_grid_scan",
     "klsp_set_symmetric_basis", "klsp_symmetric_basis", "klsp_autotune", "klsp_variant_name",
     "klsp_set_variant",
     "klsp_set_profiling",
@@ -149,6 +149,7 @@ def lib() -> ct.CDLL:
     L.klsp_build_M.argtypes = [P, ct.c_double, ct.c_double, ct.c_int, _D]
     L.klsp_build_taylor.argtypes = [P, ct.c_double, ct.c_double, ct.c_int, ct.c_int, _D]
     L.klsp_det.argtypes = [P, _D, i64, ct.c_int, _D]
+    L.klsp_det_noalloc.argtypes = [P, _D, i64, _D]
     L.klsp_newton.argtypes = [P, ct.POINTER(klsp_newton_params),
                               ct.POINTER(klsp_newton_result), _D]
     L.klsp_grid_scan.argtypes = [P, ct.POINTER(klsp_scan_params), _D, _D, _D]

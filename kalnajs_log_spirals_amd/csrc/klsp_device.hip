@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1068,6 +1069,7 @@ struct klsp_ctx {
     uint4* d_Rb = nullptr;    size_t Rb_cap = 0;  // bf16 hi/lo planes of R (BF16x3 builds)
     bool have_Rb = false;
     int* d_flag = nullptr;
+    void* d_scratch = nullptr;   // this device's copy of `scratch` (klsp_det_noalloc)
     // profiling
     bool prof = false;
     hipEvent_t ev[6] = {};
@@ -1075,6 +1077,17 @@ struct klsp_ctx {
     double last_W_ms = 0.0;
     double last_orbit_ms = 0.0;
 };
+
+// Workspace of klsp_det_noalloc: a file-scope device array, so that the
+// entry point allocates nothing.  Each GPU the library runs on holds its own
+// copy until the process ends.  Host code never passes the array's name to a
+// kernel (in host code it is only a stand-in for the symbol): the device
+// address of each GPU's copy comes from hipGetSymbolAddress, taken once per
+// device at context creation and kept in g_scratch_ptr.
+constexpr size_t kScratchBytes = (size_t)256 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static std::map<int, void*> g_scratch_ptr;   // device -> address of its `scratch`
 
 static const char* g_noctx_err = "klsp: null context";
 
@@ -1526,6 +1539,22 @@ int klsp_create(int device, klsp_ctx** ctx_out)
         return KLSP_ERR_OOM;
     }
     ctx->det_cap = kMaxBatch;
+    {
+        // this device's copy of `scratch`, looked up once per device
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        const auto it = g_scratch_ptr.find(device);
+        if (it != g_scratch_ptr.end()) {
+            ctx->d_scratch = it->second;
+        } else {
+            void* ptr = nullptr;
+            if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+                klsp_destroy(ctx);
+                return KLSP_ERR_HIP;
+            }
+            g_scratch_ptr[device] = ptr;
+            ctx->d_scratch = ptr;
+        }
+    }
     *ctx_out = ctx;
     return KLSP_OK;
 }
@@ -1915,7 +1944,15 @@ try {
     if (!ctx->have_W) return fail(ctx, KLSP_ERR_NOT_READY, "klsp_prepare: no W on device");
     int rc;
     KLSP_HIP(ctx, hipSetDevice(ctx->device));
-    if (precision == KLSP_PREC_FP64 && (rc = prepare_sym(ctx))) return rc;
+    if (precision == KLSP_PREC_FP64) {
+        // the real basis where W is symmetric, and the work table of one-omega
+        // builds (klsp_det_noalloc needs both made here)
+        if ((rc = prepare_sym(ctx)) || (rc = check_sym(ctx))) return rc;
+        const klsp_ctx::Balanced* b = nullptr;
+        if (ctx->use_sym && ctx->sym == 1) rc = get_balanced_sym(ctx, &b);
+        else rc = get_balanced(ctx, KLSP_PREC_FP64, 1, &b);
+        if (rc) return rc;
+    }
     if (precision == KLSP_PREC_FP32 && !ctx->have_W32) {
         KLSP_HIP(ctx, hipSetDevice(ctx->device));
         const int64_t n = (ctx->Kp + kRowPad) * ctx->Np;
@@ -2561,6 +2598,102 @@ try {
     return klsp::exception_status(ctx ? &ctx->err : nullptr);
 }
 
+// klsp_det in FP64 with every workspace carved from this device's `scratch`
+// (documented in klsp.h).  Layout: weights c [Kp + kRowPad] | K-piece
+// partials [n_slots][Np][Np] | S [N][N] | M, LU work [slots][N][N] | dets,
+// omegas [slots]; nothing needs clearing (every region is written before it
+// is read).
+int klsp_det_noalloc(klsp_ctx* ctx, const double* omegas, int64_t n, double* dets_out)
+try {
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    if (n < 0 || (n > 0 && (!omegas || !dets_out)))
+        return fail(ctx, KLSP_ERR_INVALID_ARG, "klsp_det_noalloc: bad arguments");
+    reset_timing(ctx);
+    if (n == 0) return KLSP_OK;
+    if (!ctx->have_problem || !ctx->have_W || !ctx->have_arrays)
+        return fail(ctx, KLSP_ERR_NOT_READY,
+                    "klsp_det_noalloc: problem arrays and W must be on device");
+    if (!ctx->d_scratch)
+        return fail(ctx, KLSP_ERR_NOT_READY, "klsp_det_noalloc: no scratch on this device");
+    const bool symb = ctx->use_sym && ctx->sym == 1;
+    const klsp_ctx::Balanced* bal = symb ? &ctx->bal_sym : &ctx->bal[0][1];
+    if (ctx->sym < 0 || !bal->built || (symb && (!ctx->have_R || ctx->bal_sym_variant != ctx->sym_variant)))
+        return fail(ctx, KLSP_ERR_NOT_READY,
+                    "klsp_det_noalloc: call klsp_prepare(ctx, KLSP_PREC_FP64) first");
+    const int64_t Kpc = ctx->Kp + kRowPad, NN = ctx->N * ctx->N;
+    const size_t Npp = (size_t)ctx->Np * ctx->Np;
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t b_c = al(sizeof(double2) * (size_t)Kpc);
+    const size_t b_P = al(sizeof(double2) * Npp * (size_t)bal->n_slots);
+    const size_t b_S = al(sizeof(double2) * (size_t)NN);
+    const size_t b_M = al(sizeof(double2) * (size_t)NN);
+    const size_t fixed = b_c + b_P + b_S;
+    const size_t per_slot = 2 * b_M + 2 * 256;
+    int64_t slots = 0;
+    if (fixed + per_slot <= kScratchBytes)
+        slots = std::min<int64_t>({(int64_t)16, n, (int64_t)((kScratchBytes - fixed) / per_slot)});
+    if (slots < 1)
+        return fail(ctx, KLSP_ERR_INVALID_ARG,
+                    "klsp_det_noalloc: the problem needs " + std::to_string(fixed + per_slot) +
+                        " bytes of workspace, more than the " + std::to_string(kScratchBytes) +
+                        "-byte scratch");
+    KLSP_HIP(ctx, hipSetDevice(ctx->device));
+    char* base = (char*)ctx->d_scratch;
+    double2* c_s = (double2*)base;
+    double2* P_s = (double2*)(base + b_c);
+    double2* S_s = (double2*)(base + b_c + b_P);
+    double2* M_s = (double2*)(base + fixed);
+    double2* A_s = (double2*)(base + fixed + slots * b_M);
+    double2* det_s = (double2*)(base + fixed + 2 * slots * b_M);
+    double* om_s = (double*)(base + fixed + 2 * slots * b_M + al(sizeof(double2) * slots));
+    for (int64_t i0 = 0; i0 < n; i0 += slots) {
+        const int nd = (int)std::min<int64_t>(slots, n - i0);
+        KLSP_HIP(ctx, hipMemcpyAsync(om_s, omegas + 2 * i0, sizeof(double) * 2 * nd,
+                                     hipMemcpyHostToDevice, ctx->stream));
+        for (int j = 0; j < nd; ++j) {
+            hipLaunchKernelGGL(k_weights, dim3(grid_for(Kpc)), dim3(256), 0, ctx->stream,
+                               ctx->d_DJ, ctx->d_F0l, ctx->d_Ores, ctx->NPh, ctx->K, Kpc,
+                               om_s + 2 * j, 1, c_s, (float2*)nullptr);
+            ContractArgs a{};
+            a.units = bal->d_units;
+            a.n_units = bal->n_units;
+            a.W = symb ? (const void*)ctx->d_R : (const void*)ctx->d_W64;
+            a.cw = c_s;
+            a.P = P_s;
+            a.Kp = ctx->Kp;
+            a.Np = ctx->Np;
+            a.n_gangs = bal->n_slots;
+            a.wrows = bal->wrows;
+            a.wcols = bal->wcols;
+            const hipError_t e = symb ? contract_sym64_launch(ctx->sym_variant, a, ctx->stream)
+                                      : contract_f64_launch(1, a, ctx->stream);
+            if (e != hipSuccess)
+                return fail(ctx, KLSP_ERR_HIP, std::string("contraction launch: ") + hipGetErrorString(e));
+            if (symb) {
+                hipLaunchKernelGGL(k_reduce_sym, dim3(grid_for(NN, kRedElems)),
+                                   dim3(kRedElems * kRedParts), 0, ctx->stream, P_s, bal->n_slots,
+                                   bal->d_count, ctx->Np, (int)ctx->N, 1, S_s);
+                hipLaunchKernelGGL(k_expand_sym, dim3(grid_for(NN)), dim3(256), 0, ctx->stream,
+                                   S_s, (int)ctx->N, 1, ctx->G, ctx->d_Nk, M_s + (size_t)j * NN);
+            } else {
+                hipLaunchKernelGGL(k_reduce, dim3(grid_for(NN, kRedElems)),
+                                   dim3(kRedElems * kRedParts), 0, ctx->stream, P_s, bal->n_slots,
+                                   bal->d_count, ctx->Np, (int)ctx->N, 1, ctx->G, ctx->d_Nk,
+                                   M_s + (size_t)j * NN);
+            }
+        }
+        hipLaunchKernelGGL(k_lu_det, dim3(nd), dim3(kLuThreads), 0, ctx->stream, M_s,
+                           (const double2*)nullptr, 0, A_s, (int)ctx->N, det_s);
+        KLSP_HIP(ctx, hipGetLastError());
+        KLSP_HIP(ctx, hipMemcpyAsync(dets_out + 2 * i0, det_s, sizeof(double2) * nd,
+                                     hipMemcpyDeviceToHost, ctx->stream));
+        KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return KLSP_OK;
+} catch (...) {
+    return klsp::exception_status(ctx ? &ctx->err : nullptr);
+}
+
 // newton_search, NewtonSolver.jl:36-172, with the forward-difference pair
 // (omega + m*delta, omega + i*delta) evaluated as one two-omega batch.
 int klsp_newton(klsp_ctx* ctx, const klsp_newton_params* np, klsp_newton_result* res,
@@ -2755,6 +2888,10 @@ try {
     }
     ctx->sym_variant = best;
     ctx->sym_tuned[ctx->Np] = best;
+    {
+        const klsp_ctx::Balanced* b = nullptr;
+        if ((rc = get_balanced_sym(ctx, &b))) return rc;
+    }
     *n_variants = nv;
     if (chosen) *chosen = best;
     return KLSP_OK;
