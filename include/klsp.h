@@ -245,6 +245,18 @@ int klsp_build_M(klsp_ctx* ctx, double omega_re, double omega_im, int precision,
 int klsp_build_taylor(klsp_ctx* ctx, double omega_re, double omega_im, int n_terms,
                       int precision, double* T_out);
 
+/* The blocks of the linear generalized eigenproblem omega D C = (E + B) C,
+ * Matlab/Matrix_LogExp.m:373-403, for every radial harmonic l of the
+ * uploaded problem:
+ *   D_l = W_l^T diag(DJ) conj W_l,   E_l = W_l^T diag(DJ Omega_res_l) conj W_l,
+ *   F_l = W_l^T diag(DJ F0l_l) conj W_l
+ * (the contraction of klsp_build_M restricted to one l, with omega-independent
+ * weights; FP64 on the device).  Each output is [N, N, n_l] complex,
+ * column-major (block l at offset N*N*l); any may be NULL.  The global
+ * matrices and B_{l,l'} = G F_l diag(N_kernel) D_{l'} (Matrix_LogExp.m:
+ * 414-449) are small dense algebra on these blocks. */
+int klsp_build_pencil(klsp_ctx* ctx, double* D_out, double* E_out, double* F_out);
+
 /* compute_determinant / compute_determinant_batched, MatrixBuilder.jl:197,
  * :215: det(I - M(omega_i)) for n values of omega (interleaved complex),
  * FP64 complex LU with partial pivoting on the device. */

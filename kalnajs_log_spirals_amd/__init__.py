@@ -20,6 +20,7 @@ from .MatrixBuilder import (compute_determinant, compute_determinant_batched,  #
                             build_M_matrix)
 from .NewtonSolver import NewtonResult, newton_search  # noqa: F401
 from .GridScan import grid_scan, find_minimum  # noqa: F401
-from . import UnevenBatch  # noqa: F401
+from . import UnevenBatch, LinearForm  # noqa: F401
+from .LinearForm import pencil_blocks, assemble_pencil, solve_pencil, linear_form_modes  # noqa: F401
 
 VERSION = "0.1.0"

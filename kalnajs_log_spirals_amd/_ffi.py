@@ -45,7 +45,7 @@ EXPORTED = [
     "klsp_compute_jacobian_df", "klsp_orbit_phase_grid", "klsp_precompute_arrays",
     "klsp_upload_problem", "klsp_set_problem_arrays", "klsp_compute_orbits_W",
     "klsp_compute_W", "klsp_upload_W", "klsp_download_W", "klsp_prepare", "klsp_build_M",
-    "klsp_build_taylor", "klsp_det", "klsp_det_noalloc", "klsp_newton", "klsp_grid_scan",
+    "klsp_build_taylor", "klsp_build_pencil", "klsp_det", "klsp_det_noalloc", "klsp_newton", "klsp_grid_scan",
     "klsp_set_symmetric_basis", "klsp_symmetric_basis", "klsp_autotune", "klsp_variant_name",
     "klsp_set_variant",
     "klsp_set_profiling",
@@ -149,6 +149,7 @@ def lib() -> ct.CDLL:
     L.klsp_build_M.argtypes = [P, ct.c_double, ct.c_double, ct.c_int, _D]
     L.klsp_build_taylor.argtypes = [P, ct.c_double, ct.c_double, ct.c_int, ct.c_int, _D]
     L.klsp_det.argtypes = [P, _D, i64, ct.c_int, _D]
+    L.klsp_build_pencil.argtypes = [P, _D, _D, _D]
     L.klsp_det_noalloc.argtypes = [P, _D, i64, _D]
     L.klsp_newton.argtypes = [P, ct.POINTER(klsp_newton_params),
                               ct.POINTER(klsp_newton_result), _D]
@@ -344,6 +345,13 @@ class Context:
                                           dptr(out)))
         return np.stack([out[2 * j * N * N:2 * (j + 1) * N * N].view(np.complex128)
                          .reshape((N, N), order="F") for j in range(n_terms)])
+
+    def build_pencil(self):
+        """(D_l, E_l, F_l), each [N, N, n_l] complex (klsp_build_pencil)."""
+        NPh, N, n_l = self.dims
+        outs = [np.empty(2 * N * N * n_l) for _ in range(3)]
+        self._chk(lib().klsp_build_pencil(self.handle, *[dptr(o) for o in outs]))
+        return tuple(o.view(np.complex128).reshape((N, N, n_l), order="F") for o in outs)
 
     def det(self, omegas, precision: int = PREC_FP64) -> np.ndarray:
         om = np.ascontiguousarray(np.atleast_1d(np.asarray(omegas, dtype=np.complex128)))

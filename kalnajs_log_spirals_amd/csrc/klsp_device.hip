@@ -647,6 +647,36 @@ __global__ void k_weights_taylor(const double* __restrict__ DJ, const double* __
     }
 }
 
+// omega-independent weights of the linear form's blocks for radial harmonic
+// slab li (Matrix_LogExp.m:385-403): c[0][k] = DJ, c[1][k] = DJ Omega_res,
+// c[2][k] = DJ F0l for the slab's orbits k < NPh, zero on the padding rows
+// (rows of kstride each).
+__global__ void k_weights_pencil(const double* __restrict__ DJ, const double* __restrict__ F0l,
+                                 const double* __restrict__ Ores, int64_t NPh, int li,
+                                 int64_t kstride, double2* __restrict__ c)
+{
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < kstride;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        double d = 0.0, e = 0.0, f = 0.0;
+        if (k < NPh) {
+            const int64_t q = (int64_t)li * NPh + k;
+            d = DJ[k];
+            e = DJ[k] * Ores[q];
+            f = DJ[k] * F0l[q];
+        }
+        c[k] = make_double2(d, 0.0);
+        c[kstride + k] = make_double2(e, 0.0);
+        c[2 * kstride + k] = make_double2(f, 0.0);
+    }
+}
+
+__global__ void k_fill(double* __restrict__ x, int64_t n, double v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = v;
+}
+
 // =====================================================================
 // Stage 3: the contraction M_w = sum_K W^T diag(c_w) conj(W): see
 // klsp_contract.hpp (kernels in klsp_contract_f64.hip / klsp_contract_f32.hip).
@@ -1069,6 +1099,7 @@ struct klsp_ctx {
     uint4* d_Rb = nullptr;    size_t Rb_cap = 0;  // bf16 hi/lo planes of R (BF16x3 builds)
     bool have_Rb = false;
     int* d_flag = nullptr;
+    double* d_ones = nullptr; size_t ones_cap = 0;   // unit column scale (linear-form blocks)
     void* d_scratch = nullptr;   // this device's copy of `scratch` (klsp_det_noalloc)
     // profiling
     bool prof = false;
@@ -1568,7 +1599,7 @@ int klsp_destroy(klsp_ctx* ctx)
     release_W(ctx);
     hipFree(ctx->d_om); hipFree(ctx->d_c); hipFree(ctx->d_P); hipFree(ctx->d_M);
     hipFree(ctx->d_A); hipFree(ctx->d_det); hipFree(ctx->d_coef); hipFree(ctx->d_cf);
-    hipFree(ctx->d_S); hipFree(ctx->d_flag);
+    hipFree(ctx->d_S); hipFree(ctx->d_flag); hipFree(ctx->d_ones);
     for (auto& e : ctx->ev) hipEventDestroy(e);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -2581,6 +2612,64 @@ try {
     KLSP_HIP(ctx, hipMemcpyAsync(T_out, ctx->d_M, sizeof(double2) * n_terms * ctx->N * ctx->N,
                                  hipMemcpyDeviceToHost, ctx->stream));
     KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return KLSP_OK;
+} catch (...) {
+    return klsp::exception_status(ctx ? &ctx->err : nullptr);
+}
+
+int klsp_build_pencil(klsp_ctx* ctx, double* D_out, double* E_out, double* F_out)
+try {
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    if (!D_out && !E_out && !F_out) return KLSP_OK;
+    int rc = check_ready(ctx, KLSP_PREC_FP64, "klsp_build_pencil");
+    if (rc) return rc;
+    KLSP_HIP(ctx, hipSetDevice(ctx->device));
+    reset_timing(ctx);
+    const int64_t NPh = ctx->NPh, N = ctx->N, NN = N * N;
+    const int64_t Kps = round_up(NPh, 8), kstride = Kps + kRowPad;   // one slab of K
+    // the three weight sets share each W pass: the FP64 three-omega kernel
+    // over one slab's rows, in gangs of at least 256 rows filling the device
+    const ContractShape sh = contract_f64_shape(3);
+    const int nbk = ctx->Np / 16;
+    const int wrows = (nbk + sh.MB - 1) / sh.MB, wcols = (nbk + sh.NB - 1) / sh.NB;
+    const int gw = wrows * wcols;
+    const int slots = std::max(1, ctx->n_cu / 8) * std::max(1, sh.waves_per_cu);
+    int gpx = (int)std::max<int64_t>(1, std::min<int64_t>(Kps / (8 * 256), std::max(1, slots / gw)));
+    const int n_gangs = 8 * gpx;
+    const int64_t Kg = round_up((Kps + n_gangs - 1) / n_gangs, 8);
+    if ((rc = ensure(ctx, ctx->d_c, ctx->c_cap, (size_t)3 * kstride))) return rc;
+    if ((rc = ensure(ctx, ctx->d_P, ctx->P_cap, (size_t)3 * n_gangs * ctx->Np * ctx->Np))) return rc;
+    if ((rc = ensure(ctx, ctx->d_M, ctx->M_cap, (size_t)std::max<int64_t>(kMaxW, 3) * NN))) return rc;
+    if ((rc = ensure(ctx, ctx->d_ones, ctx->ones_cap, (size_t)N))) return rc;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(N)), dim3(256), 0, ctx->stream, ctx->d_ones, N, 1.0);
+    double* outs[3] = {D_out, E_out, F_out};
+    for (int64_t li = 0; li < ctx->n_l; ++li) {
+        hipLaunchKernelGGL(k_weights_pencil, dim3(grid_for(kstride)), dim3(256), 0, ctx->stream,
+                           ctx->d_DJ, ctx->d_F0l, ctx->d_Ores, NPh, (int)li, kstride, ctx->d_c);
+        ContractArgs a{};
+        a.W = ctx->d_W64 + (size_t)li * NPh * ctx->Np;   // the slab's first row
+        a.cw = ctx->d_c;
+        a.P = ctx->d_P;
+        a.Kp = Kps;   // rows past the slab meet zero weights
+        a.Np = ctx->Np;
+        a.Kg = Kg;
+        a.n_gangs = n_gangs;
+        a.wrows = wrows;
+        a.wcols = wcols;
+        a.units = nullptr;
+        // the kernel reads weights at stride Kp + kRowPad = kstride
+        KLSP_HIP(ctx, contract_f64_launch(3, a, ctx->stream));
+        hipLaunchKernelGGL(k_reduce, dim3(grid_for(3 * NN, kRedElems)), dim3(kRedElems * kRedParts),
+                           0, ctx->stream, ctx->d_P, n_gangs, (const int*)nullptr, ctx->Np, (int)N,
+                           3, 1.0, ctx->d_ones, ctx->d_M);
+        KLSP_HIP(ctx, hipGetLastError());
+        for (int w = 0; w < 3; ++w)
+            if (outs[w])
+                KLSP_HIP(ctx, hipMemcpyAsync(outs[w] + 2 * NN * li, ctx->d_M + (size_t)w * NN,
+                                             sizeof(double2) * NN, hipMemcpyDeviceToHost,
+                                             ctx->stream));
+        KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     return KLSP_OK;
 } catch (...) {
     return klsp::exception_status(ctx ? &ctx->err : nullptr);

@@ -362,3 +362,48 @@ def test_splitmix64_reference_values():
     assert [g.next_u64() for _ in range(5)] == [
         6457827717110365317, 3203168211198807973, 9817491932198370423,
         4593380528125082431, 16408922859458223821]
+
+
+# --------------------------------------------------------------------------
+# Linear generalized eigenproblem (Matlab/Matrix_LogExp.m:373-530)
+# --------------------------------------------------------------------------
+def _square_problem(seed, N, n_l):
+    """A problem with square, invertible W_l (NPh = N orbits per slab)."""
+    rng = np.random.default_rng(seed)
+    W = (rng.standard_normal((N, N, n_l)) + 1j * rng.standard_normal((N, N, n_l))) / np.sqrt(N)
+    return dict(m=2, G=0.8, W=W, DJ=rng.uniform(0.3, 1.2, N), F0l=rng.standard_normal((N, n_l)),
+                Omega_res=rng.uniform(-2, 2, (N, n_l)), N_kernel=rng.uniform(0.5, 1.5, N))
+
+
+def test_linear_form_is_the_det_formulation():
+    """With square invertible W_l the pencil's characteristic polynomial
+    factors through det(I - M): by Sylvester's identity
+      det(omega D - E - B) = prod_l det(omega D_l - E_l) * det(I - M(omega)),
+    since D_l (omega D_l - E_l)^-1 F_l = W_l^T diag(DJ F0l / (omega - Omega_res))
+    conj W_l.  M comes from the C oracle's triple-loop build
+    (MatrixBuilder.jl:27-69), so a wrong block, weight or B ordering fails."""
+    from oracle import linear_form as lf
+    pre = _square_problem(5, 6, 3)
+    D, E, F = lf.blocks(pre["W"], pre["DJ"], pre["F0l"], pre["Omega_res"])
+    A, Dg = lf.assemble(D, E, F, pre["N_kernel"], pre["G"])
+    for om in (0.4 + 0.3j, -1.1 + 0.05j, 2.5 - 0.7j):
+        lhs = np.linalg.det(om * Dg - A)
+        pa = np.prod([np.linalg.det(om * D[:, :, i] - E[:, :, i]) for i in range(3)])
+        M = oracle.build_M(pre, om)
+        rhs = pa * oracle.det_matrix(np.eye(6) - M)
+        assert abs(lhs - rhs) <= 1e-10 * abs(rhs), (om, lhs, rhs)
+
+
+def test_linear_form_eigenvalues_are_det_zeros():
+    """Every eigenvalue of the pencil (LAPACK QZ) is a zero of
+    det(I - M(omega)) from the C oracle: |det| there is below 1e-8 of its size
+    one 1e-3 away."""
+    from oracle import linear_form as lf
+    pre = _square_problem(6, 5, 2)
+    D, E, F = lf.blocks(pre["W"], pre["DJ"], pre["F0l"], pre["Omega_res"])
+    w, n_inf = lf.solve(*lf.assemble(D, E, F, pre["N_kernel"], pre["G"]))
+    assert n_inf == 0 and w.size == 10
+    for om in w:
+        d0 = abs(oracle.compute_det(pre, om))
+        d1 = abs(oracle.compute_det(pre, om + 1e-3))
+        assert d0 <= 1e-8 * d1, (om, d0, d1)
