@@ -7,12 +7,14 @@ N_alpha, l = -l_max..l_max) with a starting omega; the draws come from
 SplitMix64 so that the batch is the same everywhere.  Parameters a draw does
 not set take the values of configs/default.toml.
 
-The batch runs one draw at a time on the device, each with its own context
-(freed before the next one is built, so the largest draw alone bounds device
-memory).  With orbits = "host" a worker thread computes the next draw's host
-half (grids and orbits) while the device works on the current one; with
-orbits = "device" the orbits are integrated on the device too and only the
-O(NR * Ne) grid and distribution-function work stays on the host.
+The batch runs one draw at a time on the device, all of them in one shared
+context: its W buffers grow to the largest draw's and are reused by every
+later draw that fits (no per-draw allocation and release of tens of GB), so
+the largest draw's W bounds device memory.  With orbits = "host" a worker
+thread computes the next draw's host half (grids and orbits) while the device
+works on the current one; with orbits = "device" the orbits are integrated on
+the device too and only the O(NR * Ne) grid and distribution-function work
+stays on the host.
 """
 from __future__ import annotations
 
