@@ -49,6 +49,7 @@ constexpr int kThreads = kWaves * 64;
 constexpr int kCols = 32;          // B-side columns staged per workgroup (two blocks)
 constexpr int kMaxMB = 5;          // block slots per wave
 constexpr int kMaxNW = kMaxW;
+constexpr int kKq = 64;          // K rows per pair of stages (a gang's range is a multiple)
 
 __device__ __forceinline__ unsigned pk_bf16(float a, float b)
 {
@@ -95,29 +96,33 @@ struct BIn {
     v2f c[NW][4];
 };
 
-template <int NW>
-__device__ __forceinline__ void load_B_in(const unsigned* __restrict__ Rh32,
-                                          const unsigned* __restrict__ Rl32,
-                                          const float2* __restrict__ cwf, int64_t Kpc,
-                                          int64_t K0, int Np, int col_g, bool cvalid, int rg,
-                                          BIn<NW>& o)
+// Stage loads address this gang's planes through a uniform (scalar) base
+// that moves by a fixed stride per stage, plus per-lane byte offsets fixed
+// for the whole K range: no address arithmetic per stage in vector registers.
+__device__ __forceinline__ v4u ld16(const char* base, unsigned off)
 {
-    const int64_t q = (K0 >> 3) + (rg >> 1);
-    if (cvalid) {
-        const int64_t x = (q * Np + col_g) * 4 + 2 * (rg & 1);   // dwords
-        o.h = (v2u){Rh32[x], Rh32[x + 1]};
-        o.l = (v2u){Rl32[x], Rl32[x + 1]};
-    } else {
-        o.h = (v2u){0, 0};
-        o.l = (v2u){0, 0};
+    return *(const v4u*)(base + off);
+}
+
+// B inputs of a stage from the plane record row at `hb` (hi; lo at hb +
+// plane) and the weight rows at `wb` (set w at wb + w * wstride bytes).
+template <int NW>
+__device__ __forceinline__ void load_B_in(const char* hb, int64_t plane, const char* wb,
+                                          int64_t wstride, unsigned offB, unsigned offW,
+                                          bool cvalid, BIn<NW>& o)
+{
+    const v2u h = *(const v2u*)(hb + offB), l = *(const v2u*)(hb + plane + offB);
+    o.h = cvalid ? h : (v2u){0, 0};
+    o.l = cvalid ? l : (v2u){0, 0};
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        const v4f c01 = *(const v4f*)(wb + w * wstride + offW);
+        const v4f c23 = *(const v4f*)(wb + w * wstride + offW + 16);
+        o.c[w][0] = (v2f){c01.x, c01.y};
+        o.c[w][1] = (v2f){c01.z, c01.w};
+        o.c[w][2] = (v2f){c23.x, c23.y};
+        o.c[w][3] = (v2f){c23.z, c23.w};
     }
-#pragma unroll
-    for (int w = 0; w < NW; ++w)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const float2 c = cwf[w * Kpc + K0 + 4 * rg + t];
-            o.c[w][t] = (v2f){c.x, c.y};
-        }
 }
 
 template <int NW, int NF>
@@ -163,21 +168,16 @@ __device__ __forceinline__ void lds_barrier()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+// A operands of a stage: slot s's record at hb + offA[s] (hi) and the same
+// offset in the lo plane.  Idle slots read block 0 (their MFMAs are skipped).
 template <int MB>
-__device__ __forceinline__ void load_A(const v4u* __restrict__ Rh, const v4u* __restrict__ Rl,
-                                       int64_t q0, int Np, const int (&row0)[MB], int g, int i,
+__device__ __forceinline__ void load_A(const char* hb, int64_t plane, const unsigned (&offA)[MB],
                                        v4u (&Ah)[MB], v4u (&Al)[MB])
 {
 #pragma unroll
     for (int s = 0; s < MB; ++s) {
-        if (row0[s] < 0) {
-            Ah[s] = (v4u){0, 0, 0, 0};
-            Al[s] = Ah[s];
-            continue;
-        }
-        const int64_t x = (q0 + g) * Np + row0[s] + i;
-        Ah[s] = Rh[x];
-        Al[s] = Rl[x];
+        Ah[s] = ld16(hb, offA[s]);
+        Al[s] = ld16(hb + plane, offA[s]);
     }
 }
 
@@ -229,17 +229,25 @@ __global__ __launch_bounds__(kThreads) void k_contract_symbf16(ContractArgs a)
         else if (sl < pr.nslots) { row0[s] = 16 * (sl - pr.na); csel[s] = 1; }
         else { row0[s] = -1; csel[s] = -1; }
     }
-    const int64_t Kp32 = (a.Kp + 31) / 32 * 32;
-    const int64_t Kb = min((int64_t)gang * a.Kg, Kp32);
-    const int64_t Ke = min(Kp32, Kb + a.Kg);
-    const int64_t nq = Kp32 / 8;              // records per plane column
-    const v4u* __restrict__ Rh = (const v4u*)a.W;
-    const v4u* __restrict__ Rl = Rh + nq * Np;
-    const int64_t Kpc = a.Kp + kRowPad;
+    // a gang's K range is a whole number of stage pairs (kKq rows)
+    const int64_t Kq = (a.Kp + kKq - 1) / kKq * kKq;
+    const int64_t Kb = min((int64_t)gang * a.Kg, Kq);
+    const int64_t Ke = min(Kq, Kb + a.Kg);
+    const int64_t nq = Kq / 8;                          // records per plane column
+    const int64_t plane = nq * Np * 16;                 // hi plane -> lo plane, bytes
+    const int64_t stage = (int64_t)Np * 64;             // 4 records of Np columns
+    const int64_t wstride = (a.Kp + kRowPad) * (int64_t)sizeof(float2);
+    const char* hb = (const char*)a.W + (Kb >> 3) * (int64_t)Np * 16;
+    const char* wb = (const char*)(a.cwf + Kb);
     // B-producing role: staged column col (two blocks: ca, cb), rows 4 rg..
     const int col = tid % kCols, rg = tid / kCols;
     const int col_g = (col < 16 ? 16 * pr.ca : 16 * pr.cb) + (col & 15);
     const bool cvalid = col < 16 || pr.cb != pr.ca;
+    unsigned offA[MB];
+#pragma unroll
+    for (int s = 0; s < MB; ++s) offA[s] = (unsigned)((g * Np + max(row0[s], 0) + i) * 16);
+    const unsigned offB = (unsigned)(((rg >> 1) * Np + col_g) * 16 + 8 * (rg & 1));
+    const unsigned offW = (unsigned)(4 * rg * (int)sizeof(float2));
     v4f P[NW][MB], Q[NW][MB];
 #pragma unroll
     for (int w = 0; w < NW; ++w)
@@ -248,29 +256,32 @@ __global__ __launch_bounds__(kThreads) void k_contract_symbf16(ContractArgs a)
             P[w][s] = (v4f){0, 0, 0, 0};
             Q[w][s] = (v4f){0, 0, 0, 0};
         }
-    v4u Ah[MB], Al[MB], Ahn[MB], Aln[MB];
-    BIn<NW> bin, binn;
+    // two stages a trip on two register sets, each stage's loads issued one
+    // stage ahead; the trip has no exit between its stages (the range holds
+    // an even number of stages)
+    v4u A0h[MB], A0l[MB], A1h[MB], A1l[MB];
+    BIn<NW> b0, b1;
     if (Kb < Ke) {
-        load_A<MB>(Rh, Rl, Kb >> 3, Np, row0, g, i, Ah, Al);
-        load_B_in<NW>((const unsigned*)Rh, (const unsigned*)Rl, a.cwf, Kpc, Kb, Np, col_g,
-                      cvalid, rg, bin);
+        load_A<MB>(hb, plane, offA, A0h, A0l);
+        load_B_in<NW>(hb, plane, wb, wstride, offB, offW, cvalid, b0);
     }
-    int buf = 0;
-    for (int64_t K0 = Kb; K0 < Ke; K0 += 32) {
-        const int64_t Kn = K0 + 32 < Ke ? K0 + 32 : K0;   // the next stage (the last repeats)
-        produce_B<NW, NF>(sb[buf], bin, rg, col);
-        load_B_in<NW>((const unsigned*)Rh, (const unsigned*)Rl, a.cwf, Kpc, Kn, Np, col_g,
-                      cvalid, rg, binn);
-        load_A<MB>(Rh, Rl, Kn >> 3, Np, row0, g, i, Ahn, Aln);
+    for (int64_t K0 = Kb; K0 < Ke; K0 += kKq) {
+        const char* h0 = hb + ((K0 - Kb) >> 5) * stage;
+        const char* w0 = wb + (K0 - Kb) * (int64_t)sizeof(float2);
+        produce_B<NW, NF>(sb[0], b0, rg, col);
+        load_B_in<NW>(h0 + stage, plane, w0 + 32 * sizeof(float2), wstride, offB, offW, cvalid,
+                      b1);
+        load_A<MB>(h0 + stage, plane, offA, A1h, A1l);
         lds_barrier();
-        mfma_stage<MB, NW, NF>(sb[buf], Ah, Al, csel, P, Q, g, i);
-        bin = binn;
-#pragma unroll
-        for (int s = 0; s < MB; ++s) {
-            Ah[s] = Ahn[s];
-            Al[s] = Aln[s];
+        mfma_stage<MB, NW, NF>(sb[0], A0h, A0l, csel, P, Q, g, i);
+        produce_B<NW, NF>(sb[1], b1, rg, col);
+        if (K0 + kKq < Ke) {
+            load_B_in<NW>(h0 + 2 * stage, plane, w0 + 64 * sizeof(float2), wstride, offB, offW,
+                          cvalid, b0);
+            load_A<MB>(h0 + 2 * stage, plane, offA, A0h, A0l);
         }
-        buf ^= 1;
+        lds_barrier();
+        mfma_stage<MB, NW, NF>(sb[1], A1h, A1l, csel, P, Q, g, i);
     }
     // FP32 accumulator lanes hold rows 4g + v of a block (column i); partial
     // slices are column-major: column = B-side real column
@@ -382,7 +393,7 @@ hipError_t contract_symbf16_launch(int nw, int nf, int mb, const ContractArgs& a
     return hipLaunchKernel(kernel_for(nw, nf, mb), dim3(grid), dim3(kThreads), params, 0, stream);
 }
 
-int64_t symbf16_records(int64_t Kp) { return (Kp + 31) / 32 * 32 / 8; }
+int64_t symbf16_records(int64_t Kp) { return (Kp + kKq - 1) / kKq * kKq / 8; }
 
 hipError_t split_R_bf16(const double2* W64, int64_t rows, int N, int Np, int64_t nq, void* Rb,
                         hipStream_t stream)

@@ -1434,11 +1434,13 @@ static GangPlan plan_gangs_symbf16(const klsp_ctx* ctx, int nw, int nf, int* mb)
         if (tail < best) { best = tail; xmin = x; }
     }
     if (gpx == 0) gpx = std::max(1, xmin);
-    const int64_t Kp32 = round_up(ctx->Kp, 32);
-    while (gpx > 1 && (int64_t)gpx * 8 * 256 > Kp32) --gpx;
+    // K ranges of whole stage pairs (64 rows: the kernel's loop has no exit
+    // between the two stages of a trip)
+    const int64_t Kq = round_up(ctx->Kp, 64);
+    while (gpx > 1 && (int64_t)gpx * 8 * 256 > Kq) --gpx;
     gp.gangs_per_xcd = gpx;
     gp.n_gangs = 8 * gpx;
-    gp.Kg = round_up((Kp32 + gp.n_gangs - 1) / gp.n_gangs, 32);
+    gp.Kg = round_up((Kq + gp.n_gangs - 1) / gp.n_gangs, 64);
     return gp;
 }
 
