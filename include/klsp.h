@@ -358,6 +358,21 @@ typedef struct klsp_scan_params {
 int klsp_grid_scan(klsp_ctx* ctx, const klsp_scan_params* sp, double* det_grid_out,
                    double* Omega_p_init_out, double* gamma_init_out);
 
+/* The scan's device work as one HIP graph (no counterpart in the reference,
+ * whose scan launches one det at a time, GridScan.jl:104).  By default
+ * klsp_grid_scan uploads its omegas once and launches every build (weights,
+ * contraction, reduction) and every LU of the scan as a single graph that
+ * writes the dets to device memory; the graph is instantiated on the first
+ * scan and launched again by later scans of the same size, precision and
+ * problem shape on the context (a changed buffer, size or contraction
+ * variant instantiates a new one).  Results are the same as launching the
+ * kernels one by one, which klsp_set_scan_graph(ctx, 0) selects.  With
+ * profiling on, klsp_timing.contract_ms of a graph scan is the whole graph's
+ * device time.  klsp_scan_graph_stats reports how many graphs the context
+ * has instantiated and launched. */
+int klsp_set_scan_graph(klsp_ctx* ctx, int on);
+int klsp_scan_graph_stats(klsp_ctx* ctx, int64_t* builds_out, int64_t* launches_out);
+
 /* Real basis of a symmetric alpha grid (no counterpart in the reference;
  * the arithmetic of MatrixBuilder.jl:119-141 rearranged).  Where W[:, N-1-a]
  * = conj W[:, a] for every column a and the centre column of an odd N is

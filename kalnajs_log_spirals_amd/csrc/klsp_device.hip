@@ -1101,6 +1101,15 @@ struct klsp_ctx {
     int* d_flag = nullptr;
     double* d_ones = nullptr; size_t ones_cap = 0;   // unit column scale (linear-form blocks)
     void* d_scratch = nullptr;   // this device's copy of `scratch` (klsp_det_noalloc)
+    // the grid scan as one HIP graph: its omegas and dets live on the device,
+    // and the instantiated graph is kept for the next scan with the same key
+    // (every pointer, size and launch parameter the capture recorded)
+    double* d_scan_om = nullptr;   size_t scan_om_cap = 0;
+    double2* d_scan_det = nullptr; size_t scan_det_cap = 0;
+    hipGraphExec_t scan_exec = nullptr;
+    std::vector<int64_t> scan_key;
+    int scan_graph = 1;            // klsp_set_scan_graph
+    int64_t scan_graph_builds = 0, scan_graph_launches = 0;
     // profiling
     bool prof = false;
     hipEvent_t ev[6] = {};
@@ -1602,6 +1611,8 @@ int klsp_destroy(klsp_ctx* ctx)
     hipFree(ctx->d_om); hipFree(ctx->d_c); hipFree(ctx->d_P); hipFree(ctx->d_M);
     hipFree(ctx->d_A); hipFree(ctx->d_det); hipFree(ctx->d_coef); hipFree(ctx->d_cf);
     hipFree(ctx->d_S); hipFree(ctx->d_flag); hipFree(ctx->d_ones);
+    if (ctx->scan_exec) hipGraphExecDestroy(ctx->scan_exec);
+    hipFree(ctx->d_scan_om); hipFree(ctx->d_scan_det);
     for (auto& e : ctx->ev) hipEventDestroy(e);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -2117,8 +2128,11 @@ static int launch_contract(klsp_ctx* ctx, int precision, int nb, int nf)
 static void add_build_timing(klsp_ctx* ctx, int nb);
 
 // ... into matrix slots slot..slot+nb-1 of ctx->d_M.
+// With om_dev set, the omegas are read from that device array (no host copy:
+// the form a captured graph needs) and om is not used.
 static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
-                       bool taylor = false, int slot = 0, int nf = 0)
+                       bool taylor = false, int slot = 0, int nf = 0,
+                       const double* om_dev = nullptr)
 {
     if (nf <= 0 || nf > nb || precision != KLSP_PREC_BF16X3 || nb != kMaxW) nf = nb;
     const int64_t Kpc = ctx->Kp + kRowPad;
@@ -2147,7 +2161,7 @@ static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
     if ((rc = ensure(ctx, ctx->d_M, ctx->M_cap,
                      (size_t)std::max(kMaxW, slot + nb) * ctx->N * ctx->N)))
         return rc;
-    if (!taylor)
+    if (!taylor && !om_dev)
         KLSP_HIP(ctx, hipMemcpyAsync(ctx->d_om, om, sizeof(double) * 2 * nb,
                                      hipMemcpyHostToDevice, ctx->stream));
     if (ctx->prof) hipEventRecord(ctx->ev[0], ctx->stream);
@@ -2158,7 +2172,7 @@ static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
     else
         hipLaunchKernelGGL(k_weights, dim3(grid_for((int64_t)nb * Kpc)), dim3(256), 0,
                            ctx->stream, ctx->d_DJ, ctx->d_F0l, ctx->d_Ores, ctx->NPh, ctx->K,
-                           Kpc, ctx->d_om, nb, ctx->d_c, cf);
+                           Kpc, om_dev ? om_dev : ctx->d_om, nb, ctx->d_c, cf);
     if (ctx->prof) hipEventRecord(ctx->ev[1], ctx->stream);
     if ((rc = launch_contract(ctx, precision, nb, nf))) return rc;
     if (ctx->prof) hipEventRecord(ctx->ev[2], ctx->stream);
@@ -2201,11 +2215,12 @@ static void add_build_timing(klsp_ctx* ctx, int nb)
 
 // LU launch over matrices [0, nd) of ctx->d_M (coef as k_lu_det), dets to
 // ctx->d_det, with its device time added to the profile.
-static int launch_lu(klsp_ctx* ctx, int nd, const double2* coef, int nt)
+static int launch_lu(klsp_ctx* ctx, int nd, const double2* coef, int nt,
+                     double2* det_out = nullptr)
 {
     if (ctx->prof) hipEventRecord(ctx->ev[4], ctx->stream);
     hipLaunchKernelGGL(k_lu_det, dim3(nd), dim3(kLuThreads), 0, ctx->stream, ctx->d_M, coef, nt,
-                       ctx->d_A, (int)ctx->N, ctx->d_det);
+                       ctx->d_A, (int)ctx->N, det_out ? det_out : ctx->d_det);
     if (ctx->prof) hipEventRecord(ctx->ev[5], ctx->stream);
     KLSP_HIP(ctx, hipGetLastError());
     return KLSP_OK;
@@ -2226,6 +2241,15 @@ static int det_slots(const klsp_ctx* ctx)
 {
     const int64_t fit = ((int64_t)256 << 20) / std::max<int64_t>(1, 16 * ctx->N * ctx->N);
     return (int)std::max<int64_t>(kMaxBatch, std::min<int64_t>(64, fit));
+}
+
+// Matrices per LU launch for n omegas built maxb at a time: whole batches
+// only (a launch never ends on a narrower contraction pass unless the omegas
+// run out), so a scan groups its omegas the same way in every form.
+static int batch_slots(const klsp_ctx* ctx, int64_t n, int maxb)
+{
+    const int fit = std::max(maxb, det_slots(ctx) / maxb * maxb);
+    return (int)std::min<int64_t>(fit, round_up(n, maxb));
 }
 
 static int check_ready(klsp_ctx* ctx, int precision, const char* who)
@@ -2255,7 +2279,7 @@ static int det_many(klsp_ctx* ctx, const double* om, int64_t n, int precision, d
     // FP64 is MFMA-bound at one omega per pass (larger batches only lower the
     // occupancy); the low-precision passes share W loads among up to three.
     const int maxb = precision == KLSP_PREC_FP64 ? 1 : kMaxBatch;
-    const int slots = (int)std::min<int64_t>(det_slots(ctx), round_up(n, maxb));
+    const int slots = batch_slots(ctx, n, maxb);
     const size_t NN = (size_t)ctx->N * ctx->N;
     if ((rc = ensure(ctx, ctx->d_M, ctx->M_cap, (size_t)std::max(slots, kMaxW) * NN))) return rc;
     if ((rc = ensure(ctx, ctx->d_A, ctx->A_cap, (size_t)std::max(slots, kMaxBatch) * NN))) return rc;
@@ -2272,6 +2296,128 @@ static int det_many(klsp_ctx* ctx, const double* om, int64_t n, int precision, d
         KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
         add_lu_timing(ctx);
     }
+    return KLSP_OK;
+}
+
+// Everything a captured scan recorded that can change between calls: buffer
+// addresses, sizes, launch parameters, the contraction path and its variant.
+static std::vector<int64_t> scan_graph_key(const klsp_ctx* ctx, int64_t n, int precision,
+                                           int slots, int maxb)
+{
+    int64_t g = 0;
+    std::memcpy(&g, &ctx->G, sizeof g);
+    auto P = [](const void* q) { return (int64_t)(uintptr_t)q; };
+    std::vector<int64_t> k = {
+        n, precision, slots, maxb, ctx->N, ctx->Np, ctx->Kp, ctx->K, ctx->NPh, ctx->n_cu, g,
+        ctx->use_sym, ctx->sym, ctx->have_R, ctx->have_Rb, ctx->have_Wb, ctx->have_W32,
+        ctx->sym_variant, P(ctx->d_W64), P(ctx->d_W32), P(ctx->d_Wb), P(ctx->d_R), P(ctx->d_Rb),
+        P(ctx->d_c), P(ctx->d_cf), P(ctx->d_P), P(ctx->d_S), P(ctx->d_M), P(ctx->d_A),
+        P(ctx->d_DJ), P(ctx->d_F0l), P(ctx->d_Ores), P(ctx->d_Nk), P(ctx->d_scan_om),
+        P(ctx->d_scan_det), P(ctx->bal_sym.d_units), P(ctx->bal_sym.d_count),
+        ctx->bal_sym.n_units, ctx->bal_sym.n_slots};
+    for (const auto& row : ctx->bal)
+        for (const auto& b : row) {
+            k.push_back(P(b.d_units));
+            k.push_back(P(b.d_count));
+            k.push_back(b.n_units);
+            k.push_back(b.n_slots);
+        }
+    return k;
+}
+
+// Profiling off for the scope (events cannot be recorded into a captured
+// graph), restored on every return.
+struct ProfilingOff {
+    klsp_ctx* c;
+    bool was;
+    explicit ProfilingOff(klsp_ctx* ctx) : c(ctx), was(ctx->prof) { c->prof = false; }
+    ~ProfilingOff() { c->prof = was; }
+};
+
+// det(I - M(omega_i)) for the n omegas of a grid scan as one HIP graph: the
+// omegas are uploaded once, each batch's weights read them on the device,
+// and the LU launches write their dets into a device array, so the graph
+// holds kernel nodes only (no host copies, no memory sets).  A graph with
+// the same key as the last one is launched again (its omegas are read from
+// the device array at launch).  Batches are built eagerly once before a
+// capture, so that every workspace and work table exists (nothing is
+// allocated while capturing).
+static int scan_by_graph(klsp_ctx* ctx, const double* om, int64_t n, int precision, double* out)
+{
+    int rc = check_ready(ctx, precision, "klsp_grid_scan");
+    if (rc) return rc;
+    KLSP_HIP(ctx, hipSetDevice(ctx->device));
+    const int maxb = precision == KLSP_PREC_FP64 ? 1 : kMaxBatch;
+    const int slots = batch_slots(ctx, n, maxb);
+    const size_t NN = (size_t)ctx->N * ctx->N;
+    if ((rc = ensure(ctx, ctx->d_M, ctx->M_cap, (size_t)std::max(slots, kMaxW) * NN))) return rc;
+    if ((rc = ensure(ctx, ctx->d_A, ctx->A_cap, (size_t)std::max(slots, kMaxBatch) * NN))) return rc;
+    if ((rc = ensure(ctx, ctx->d_det, ctx->det_cap, (size_t)std::max(slots, kMaxBatch)))) return rc;
+    if ((rc = ensure(ctx, ctx->d_scan_om, ctx->scan_om_cap, (size_t)2 * n))) return rc;
+    if ((rc = ensure(ctx, ctx->d_scan_det, ctx->scan_det_cap, (size_t)n))) return rc;
+    KLSP_HIP(ctx, hipMemcpyAsync(ctx->d_scan_om, om, sizeof(double) * 2 * n,
+                                 hipMemcpyHostToDevice, ctx->stream));
+    const bool prof = ctx->prof;
+    ProfilingOff off(ctx);
+    const int64_t builds = (n + maxb - 1) / maxb + (n / slots);   // upper bound for the timing
+    std::vector<int64_t> key = scan_graph_key(ctx, n, precision, slots, maxb);
+    if (!ctx->scan_exec || key != ctx->scan_key) {
+        // every batch width the graph uses (full batches and the last one's
+        // remainder; slots hold whole batches), built once outside the
+        // capture from the first omegas
+        const int widths[2] = {(int)std::min<int64_t>(maxb, n), (int)(n % maxb)};
+        for (int nb : widths)
+            if (nb > 0 &&
+                (rc = build_batch(ctx, nullptr, nb, precision, false, 0, 0, ctx->d_scan_om)))
+                return rc;
+        key = scan_graph_key(ctx, n, precision, slots, maxb);
+        if (ctx->scan_exec) {
+            hipGraphExecDestroy(ctx->scan_exec);
+            ctx->scan_exec = nullptr;
+        }
+        KLSP_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int crc = KLSP_OK;
+        for (int64_t i0 = 0; i0 < n && crc == KLSP_OK; i0 += slots) {
+            const int nd = (int)std::min<int64_t>(slots, n - i0);
+            for (int j0 = 0; j0 < nd && crc == KLSP_OK; j0 += maxb) {
+                const int nb = std::min(maxb, nd - j0);
+                crc = build_batch(ctx, nullptr, nb, precision, false, j0, 0,
+                                  ctx->d_scan_om + 2 * (i0 + j0));
+            }
+            if (crc == KLSP_OK) crc = launch_lu(ctx, nd, nullptr, 0, ctx->d_scan_det + i0);
+        }
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+        if (crc != KLSP_OK || ec != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            if (crc != KLSP_OK) return crc;
+            return fail(ctx, KLSP_ERR_HIP,
+                        std::string("klsp_grid_scan: graph capture: ") + hipGetErrorString(ec));
+        }
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ei != hipSuccess)
+            return fail(ctx, KLSP_ERR_HIP,
+                        std::string("klsp_grid_scan: graph instantiation: ") + hipGetErrorString(ei));
+        ctx->scan_exec = ex;
+        ctx->scan_key = key;
+        ++ctx->scan_graph_builds;
+    }
+    if (prof) hipEventRecord(ctx->ev[4], ctx->stream);
+    KLSP_HIP(ctx, hipGraphLaunch(ctx->scan_exec, ctx->stream));
+    if (prof) hipEventRecord(ctx->ev[5], ctx->stream);
+    ++ctx->scan_graph_launches;
+    KLSP_HIP(ctx, hipMemcpyAsync(out, ctx->d_scan_det, sizeof(double2) * n,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (prof) {
+        float d = 0;
+        hipEventElapsedTime(&d, ctx->ev[4], ctx->ev[5]);
+        ctx->last.contract_ms += d;   // the whole graph (no per-phase split)
+    }
+    ctx->last.n_omega += n;
+    ctx->last.contract_launches += builds;
     return KLSP_OK;
 }
 
@@ -2904,7 +3050,9 @@ try {
             om[2 * x] = ctx->m * Opg[j];
             om[2 * x + 1] = gg[i];
         }
-    if ((rc = det_many(ctx, om.data(), n, sp->precision, det_grid))) return rc;
+    if ((rc = ctx->scan_graph ? scan_by_graph(ctx, om.data(), n, sp->precision, det_grid)
+                              : det_many(ctx, om.data(), n, sp->precision, det_grid)))
+        return rc;
     // find_minimum (GridScan.jl:234-246): argmin |det| in column-major
     // order, first occurrence; Julia's argmin takes NaN as the minimum
     int64_t best = 0;
@@ -2919,6 +3067,21 @@ try {
     return KLSP_OK;
 } catch (...) {
     return klsp::exception_status(ctx ? &ctx->err : nullptr);
+}
+
+int klsp_set_scan_graph(klsp_ctx* ctx, int on)
+{
+    if (!ctx) return KLSP_ERR_INVALID_ARG;
+    ctx->scan_graph = on ? 1 : 0;
+    return KLSP_OK;
+}
+
+int klsp_scan_graph_stats(klsp_ctx* ctx, int64_t* builds_out, int64_t* launches_out)
+{
+    if (!ctx || !builds_out || !launches_out) return KLSP_ERR_INVALID_ARG;
+    *builds_out = ctx->scan_graph_builds;
+    *launches_out = ctx->scan_graph_launches;
+    return KLSP_OK;
 }
 
 int klsp_autotune(klsp_ctx* ctx, int precision, int reps, int* n_variants, double* ms_out,
