@@ -93,6 +93,8 @@ def main() -> int:
                     help="skip the FP64 reference iteration over the uneven draws")
     ap.add_argument("--uneven-orbits", choices=("host", "device"), default="device",
                     help="where the uneven draws' orbits are integrated")
+    ap.add_argument("--uneven-workers", type=int, default=1,
+                    help="contexts the uneven draws are packed onto (1: one at a time)")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-scan", action="store_true",
                     help="skip the whole 64 x 64 omega-grid scan in BF16x3 (checked against FP64 "
@@ -340,7 +342,8 @@ def main() -> int:
             return [r for r in rs if r.converged and r.polish_move <= 1e-10]
 
         t0 = time.perf_counter()
-        rm = run_batch(draws, precision="mixed", log=log, polish=2, orbits=args.uneven_orbits)
+        rm = run_batch(draws, precision="mixed", log=log, polish=2, orbits=args.uneven_orbits,
+                       workers=args.uneven_workers)
         wall = time.perf_counter() - t0
         gm = good(rm)
         tn = sum(r.newton_s for r in rm)
@@ -354,13 +357,14 @@ def main() -> int:
                       "uneven_max_fp64_polish_move_mixed":
                           max((r.polish_move for r in rm if r.converged), default=None),
                       "uneven_precompute_s": sum(r.W_s for r in rm),
-                      "uneven_orbits": args.uneven_orbits})
+                      "uneven_orbits": args.uneven_orbits,
+                      "uneven_workers": args.uneven_workers})
         log(f"[bench] uneven batch (mixed): {len(gm)}/{len(draws)} FP64-grade roots, Newton "
             f"{tn:.1f} s ({len(gm) / tn:.2f} roots/s), wall {wall:.1f} s")
         if not args.no_uneven_fp64:
             t0 = time.perf_counter()
             r64 = run_batch(draws, precision="fp64", log=log, polish=2,
-                            orbits=args.uneven_orbits)
+                            orbits=args.uneven_orbits, workers=args.uneven_workers)
             wall64 = time.perf_counter() - t0
             g64 = good(r64)
             tn64 = sum(r.newton_s for r in r64)

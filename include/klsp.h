@@ -371,6 +371,14 @@ int klsp_grid_scan(klsp_ctx* ctx, const klsp_scan_params* sp, double* det_grid_o
  * device time.  klsp_scan_graph_stats reports how many graphs the context
  * has instantiated and launched. */
 int klsp_set_scan_graph(klsp_ctx* ctx, int on);
+
+/* Device memory the context holds between calls: W and its copies, the
+ * problem arrays and the grow-only workspaces, in bytes (no counterpart in
+ * the reference).  Orbit samples and quadrature temporaries of
+ * klsp_compute_W / klsp_compute_orbits_W are freed before those return and
+ * are not included.  A caller packing several contexts onto one device (the
+ * uneven batch) budgets HBM with it. */
+int klsp_device_bytes(klsp_ctx* ctx, int64_t* bytes_out);
 int klsp_scan_graph_stats(klsp_ctx* ctx, int64_t* builds_out, int64_t* launches_out);
 
 /* Real basis of a symmetric alpha grid (no counterpart in the reference;

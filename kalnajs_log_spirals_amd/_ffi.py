@@ -47,7 +47,7 @@ EXPORTED = [
     "klsp_compute_W", "klsp_upload_W", "klsp_download_W", "klsp_prepare", "klsp_build_M",
     "klsp_build_taylor", "klsp_build_pencil", "klsp_det", "klsp_det_noalloc", "klsp_newton", "klsp_grid_scan",
     "klsp_set_symmetric_basis", "klsp_symmetric_basis", "klsp_autotune", "klsp_variant_name",
-    "klsp_set_variant", "klsp_set_scan_graph", "klsp_scan_graph_stats",
+    "klsp_set_variant", "klsp_set_scan_graph", "klsp_scan_graph_stats", "klsp_device_bytes",
     "klsp_set_profiling",
     "klsp_last_timing", "klsp_last_W_ms", "klsp_last_orbit_ms",
 ]
@@ -161,6 +161,7 @@ def lib() -> ct.CDLL:
     L.klsp_variant_name.argtypes = [ct.c_int, ct.c_int]
     L.klsp_set_symmetric_basis.argtypes = [P, ct.c_int]
     L.klsp_set_scan_graph.argtypes = [P, ct.c_int]
+    L.klsp_device_bytes.argtypes = [P, ct.POINTER(ct.c_int64)]
     L.klsp_scan_graph_stats.argtypes = [P, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)]
     L.klsp_symmetric_basis.argtypes = [P, ct.POINTER(ct.c_int)]
     L.klsp_set_profiling.argtypes = [P, ct.c_int]
@@ -382,6 +383,12 @@ class Context:
         self._chk(lib().klsp_grid_scan(self.handle, ct.byref(sp), dptr(dg), ct.byref(o1),
                                        ct.byref(o2)))
         return dg.view(np.complex128).reshape((n_gamma, n_Op), order="F"), o1.value, o2.value
+
+    def device_bytes(self) -> int:
+        """Device memory this context holds between calls (klsp_device_bytes)."""
+        v = ct.c_int64(0)
+        self._chk(lib().klsp_device_bytes(self.handle, ct.byref(v)))
+        return v.value
 
     def set_scan_graph(self, on: bool):
         """grid_scan as one HIP graph launch (default on) or kernel by kernel."""

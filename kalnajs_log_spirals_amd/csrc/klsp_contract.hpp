@@ -43,6 +43,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 
 #include "klsp_internal.hpp"

@@ -297,7 +297,7 @@ const void* kernel_for(int nw, int nf)
 
 Bf16Geometry contract_bf16_geometry(int Np, int nw, int nf)
 {
-    static int occ[kMaxNW + 1][2] = {};
+    static std::atomic<int> occ[kMaxNW + 1][2] = {};
     const int i = nw < 1 ? 1 : (nw > kMaxNW ? kMaxNW : nw);
     const int h = (i >= 4 && nf < 4) ? 1 : 0;
     if (occ[i][h] == 0) {

@@ -26,7 +26,7 @@ static const void* kernel_for(int nw)
 
 ContractShape contract_f32_shape(int nw)
 {
-    static int occ[4] = {0, 0, 0, 0};
+    static std::atomic<int> occ[4] = {0, 0, 0, 0};
     const int i = nw < 1 ? 1 : (nw > 3 ? 3 : nw);
     if (occ[i] == 0) {
         int per_cu = 0;

@@ -26,7 +26,7 @@ static const void* kernel_for(int nw, int* MB, int* NB)
 
 ContractShape contract_f64_shape(int nw)
 {
-    static int occ[4] = {0, 0, 0, 0};
+    static std::atomic<int> occ[4] = {0, 0, 0, 0};
     const int i = nw < 1 ? 1 : (nw > 3 ? 3 : nw);
     int MB = 0, NB = 0;
     const void* k = kernel_for(i, &MB, &NB);

@@ -223,7 +223,7 @@ static int clamp_variant(int v) { return v < 0 || v >= kNumVariants ? 0 : v; }
 
 ContractShape contract_sym64_shape(int variant)
 {
-    static int occ[kNumVariants] = {};
+    static std::atomic<int> occ[kNumVariants] = {};
     const int v = clamp_variant(variant);
     if (occ[v] == 0) {
         int per_cu = 0;

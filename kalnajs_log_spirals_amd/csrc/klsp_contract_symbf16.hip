@@ -369,7 +369,7 @@ Bf16Geometry contract_symbf16_geometry(int Np, int nw, int nf)
     g.wrows = per_pair;                       // workgroups per pair
     g.wcols = (nb + 1) / 2;                   // pairs (the middle column of an odd nb alone)
     g.mb = mb;
-    static int occ[kMaxNW + 1][2][kMaxMB + 1] = {};
+    static std::atomic<int> occ[kMaxNW + 1][2][kMaxMB + 1] = {};
     const int i = nw < 1 ? 1 : (nw > kMaxNW ? kMaxNW : nw);
     const int hh = (i >= 4 && nf < 4) ? 1 : 0;
     if (occ[i][hh][mb] == 0) {

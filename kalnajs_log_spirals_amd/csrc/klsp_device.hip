@@ -3069,6 +3069,27 @@ try {
     return klsp::exception_status(ctx ? &ctx->err : nullptr);
 }
 
+int klsp_device_bytes(klsp_ctx* ctx, int64_t* bytes_out)
+{
+    if (!ctx || !bytes_out) return KLSP_ERR_INVALID_ARG;
+    int64_t b = 0;
+    b += (int64_t)ctx->W64_cap * sizeof(double2) + (int64_t)ctx->W32_cap * sizeof(float2);
+    b += (int64_t)ctx->Wb_cap * sizeof(uint4) + (int64_t)ctx->R_cap * sizeof(double);
+    b += (int64_t)ctx->Rb_cap * sizeof(uint4) + (int64_t)ctx->c_cap * sizeof(double2);
+    b += (int64_t)ctx->cf_cap * sizeof(float2) + (int64_t)ctx->P_cap * sizeof(double2);
+    b += (int64_t)ctx->M_cap * sizeof(double2) + (int64_t)ctx->A_cap * sizeof(double2);
+    b += (int64_t)ctx->det_cap * sizeof(double2) + (int64_t)ctx->S_cap * sizeof(double2);
+    b += (int64_t)ctx->ones_cap * sizeof(double) + (int64_t)ctx->scan_om_cap * sizeof(double);
+    b += (int64_t)ctx->scan_det_cap * sizeof(double2);
+    if (ctx->have_problem)   // DJ, F0l, Omega_res, N_kernel
+        b += (ctx->NPh + 2 * ctx->K + ctx->N) * (int64_t)sizeof(double);
+    for (const auto& row : ctx->bal)
+        for (const auto& t : row) b += (int64_t)t.n_units * sizeof(WorkUnit) + 4096;
+    b += (int64_t)ctx->bal_sym.n_units * sizeof(WorkUnit) + 4096;
+    *bytes_out = b;
+    return KLSP_OK;
+}
+
 int klsp_set_scan_graph(klsp_ctx* ctx, int on)
 {
     if (!ctx) return KLSP_ERR_INVALID_ARG;
