@@ -2687,9 +2687,11 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
             if (!(std::abs(Dp) > 0.0) || !std::isfinite(std::abs(Dp))) break;
             const cd wn = bounded(wc + s - D / Dp);
             const double step = std::abs(wn - (wc + s));
-            // converged: a step below 1e-13 |w|, or no longer shrinking (the
-            // rounding floor of the FP64 dets)
-            if (step <= 1e-13 * std::max(1.0, std::abs(wn)) || (it >= 2 && step > 0.5 * prev_step)) {
+            // converged: a step below 1e-12 |w| (a hundredth of the 1e-10 the
+            // roots are held to; the point kept is the one the step starts
+            // from), or no longer shrinking (the rounding floor of the FP64
+            // dets)
+            if (step <= 1e-12 * std::max(1.0, std::abs(wn)) || (it >= 2 && step > 0.5 * prev_step)) {
                 stepped = true;
                 break;
             }
@@ -2701,6 +2703,10 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
             // it strays so far that the model itself is doubtful)
             if (std::abs(s - sa) > anchor_tol) far = true;
             if (std::abs(s - sa) > 0.25 * wc.imag()) break;
+            // the next anchor is built here: once Newton's next step (~ step^2
+            // / Im w near a simple zero) is below a thousandth of anchor_tol,
+            // further steps on this anchor's model change nothing it needs
+            if (far && step * step < 1e-3 * anchor_tol * wc.imag()) break;
         }
         w = wc + s;
         if (!far) {
