@@ -399,6 +399,15 @@ bool contract_sym64_full_diag(int variant);
 const char* contract_sym64_name(int variant);
 hipError_t contract_sym64_launch(int variant, const ContractArgs& a, hipStream_t stream);
 
+// Real-basis FP32 contraction (klsp_contract_sym32.hip): 4 x 2 tiles on or
+// above the diagonal, one weight set per launch (a.cwf, a.P point at it).
+ContractShape contract_sym32_shape();
+int contract_sym32_quantum();
+hipError_t contract_sym32_launch(const ContractArgs& a, hipStream_t stream);
+// FP32 real basis of `rows` rows of W into R32 ([rows][Np] floats).
+hipError_t split_R_f32(const double2* W64, int64_t rows, int N, int Np, float* R32,
+                       hipStream_t stream);
+
 // BF16 three-term contraction (klsp_contract_bf16.hip): workgroups of
 // kBf16Waves waves stacked along beta, kBf16MB blocks per wave, kBf16NB alpha
 // blocks per workgroup; K ranges per gang are multiples of 16 rows.

@@ -1098,6 +1098,9 @@ struct klsp_ctx {
     double2* d_S = nullptr;   size_t S_cap = 0;   // S = P + iQ of that launch
     uint4* d_Rb = nullptr;    size_t Rb_cap = 0;  // bf16 hi/lo planes of R (BF16x3 builds)
     bool have_Rb = false;
+    float* d_R32 = nullptr;   size_t R32_cap = 0; // R in FP32 (FP32 builds)
+    bool have_R32 = false;
+    Balanced bal_sym32;          // work table of the real-basis FP32 contraction
     int* d_flag = nullptr;
     double* d_ones = nullptr; size_t ones_cap = 0;   // unit column scale (linear-form blocks)
     void* d_scratch = nullptr;   // this device's copy of `scratch` (klsp_det_noalloc)
@@ -1321,6 +1324,37 @@ static bool plan_balanced_sym(const klsp_ctx* ctx, int variant, klsp_ctx::Balanc
     return true;
 }
 
+// The same for the real-basis FP32 contraction: 4 x 2 tiles (r, c) with any
+// block on or above the diagonal (4r <= 2c + 1), each computing all its
+// blocks; count covers blocks bi <= bj.
+static bool plan_balanced_sym32(const klsp_ctx* ctx, klsp_ctx::Balanced& out,
+                                std::vector<WorkUnit>& units, std::vector<int>& count)
+{
+    const ContractShape sh = contract_sym32_shape();
+    const int nb = ctx->Np / 16;
+    const int wr = (nb + sh.MB - 1) / sh.MB, wc = (nb + sh.NB - 1) / sh.NB;
+    std::vector<TileDesc> tiles;
+    std::vector<int> index(wr * wc, -1);
+    for (int r = 0; r < wr; ++r)
+        for (int c = 0; c < wc; ++c) {
+            if (sh.MB * r > sh.NB * c + sh.NB - 1) continue;
+            const int mv = std::min(sh.MB, nb - r * sh.MB), nv = std::min(sh.NB, nb - c * sh.NB);
+            index[r * wc + c] = (int)tiles.size();
+            tiles.push_back({r * wc + c, mv * nv});
+        }
+    std::vector<int> pieces;
+    if (!plan_units(ctx, tiles, sh.MB * sh.NB, sh.waves_per_cu, false, contract_sym32_quantum(),
+                    out, units, pieces))
+        return false;
+    count.assign((size_t)nb * nb, 0);
+    for (int bi = 0; bi < nb; ++bi)
+        for (int bj = bi; bj < nb; ++bj)
+            count[bi + (size_t)bj * nb] = pieces[index[(bi / sh.MB) * wc + bj / sh.NB]];
+    out.wrows = wr;
+    out.wcols = wc;
+    return true;
+}
+
 static ContractShape shape_for(int precision, int nw);
 
 // The balanced table for (precision, nw), built and uploaded on first use.
@@ -1386,6 +1420,23 @@ static int get_balanced_sym(klsp_ctx* ctx, const klsp_ctx::Balanced** out)
         int rc = store_balanced(ctx, b, nbal, units, count);
         if (rc) return rc;
         ctx->bal_sym_variant = ctx->sym_variant;
+    }
+    *out = &b;
+    return KLSP_OK;
+}
+
+// The balanced table of the real-basis FP32 contraction.
+static int get_balanced_sym32(klsp_ctx* ctx, const klsp_ctx::Balanced** out)
+{
+    klsp_ctx::Balanced& b = ctx->bal_sym32;
+    if (!b.built) {
+        std::vector<WorkUnit> units;
+        std::vector<int> count;
+        klsp_ctx::Balanced nbal;
+        if (!plan_balanced_sym32(ctx, nbal, units, count))
+            return fail(ctx, KLSP_ERR_INVALID_ARG, "contraction: no work plan fits the device");
+        int rc = store_balanced(ctx, b, nbal, units, count);
+        if (rc) return rc;
     }
     *out = &b;
     return KLSP_OK;
@@ -1477,6 +1528,9 @@ static int max_gangs(klsp_ctx* ctx, int* rc)
     }
     g = std::max(g, plan_gangs_bf16(ctx, kMaxW, kMaxW - 1).n_gangs);
     g = std::max(g, plan_gangs_symbf16(ctx, kMaxW, kMaxW - 1, &mb).n_gangs);
+    const klsp_ctx::Balanced* b32 = nullptr;
+    if ((*rc = get_balanced_sym32(ctx, &b32))) return 0;
+    g = std::max(g, b32->n_slots);
     return g;
 }
 
@@ -1491,6 +1545,9 @@ static void free_balanced(klsp_ctx* ctx)
     hipFree(ctx->bal_sym.d_units);
     hipFree(ctx->bal_sym.d_count);
     ctx->bal_sym = klsp_ctx::Balanced();
+    hipFree(ctx->bal_sym32.d_units);
+    hipFree(ctx->bal_sym32.d_count);
+    ctx->bal_sym32 = klsp_ctx::Balanced();
 }
 
 static int free_problem(klsp_ctx* ctx)
@@ -1508,14 +1565,17 @@ static int free_problem(klsp_ctx* ctx)
 static void release_W(klsp_ctx* ctx)
 {
     hipFree(ctx->d_W64); hipFree(ctx->d_W32); hipFree(ctx->d_Wb); hipFree(ctx->d_R);
-    hipFree(ctx->d_Rb);
+    hipFree(ctx->d_Rb); hipFree(ctx->d_R32);
+    ctx->d_R32 = nullptr;
+    ctx->R32_cap = 0;
+    ctx->have_R32 = false;
     ctx->d_W64 = nullptr;
     ctx->d_W32 = nullptr;
     ctx->d_Wb = nullptr;
     ctx->d_R = nullptr;
     ctx->d_Rb = nullptr;
     ctx->W64_cap = ctx->W32_cap = ctx->Wb_cap = ctx->R_cap = ctx->Rb_cap = 0;
-    ctx->have_W = ctx->have_W32 = ctx->have_Wb = ctx->have_R = ctx->have_Rb = false;
+    ctx->have_W = ctx->have_W32 = ctx->have_Wb = ctx->have_R = ctx->have_Rb = ctx->have_R32 = false;
     ctx->sym = -1;
 }
 
@@ -1685,7 +1745,7 @@ try {
 static int alloc_W64(klsp_ctx* ctx)
 {
     const size_t n = (size_t)(ctx->Kp + kRowPad) * ctx->Np;
-    ctx->have_W = ctx->have_W32 = ctx->have_Wb = ctx->have_R = ctx->have_Rb = false;
+    ctx->have_W = ctx->have_W32 = ctx->have_Wb = ctx->have_R = ctx->have_Rb = ctx->have_R32 = false;
     ctx->sym = -1;
     if (n > ctx->W64_cap) release_W(ctx);
     int rc = ensure(ctx, ctx->d_W64, ctx->W64_cap, n);
@@ -1997,7 +2057,17 @@ try {
         else rc = get_balanced(ctx, KLSP_PREC_FP64, 1, &b);
         if (rc) return rc;
     }
-    if (precision == KLSP_PREC_FP32 && !ctx->have_W32) {
+    if (precision == KLSP_PREC_FP32 && (rc = check_sym(ctx))) return rc;
+    if (precision == KLSP_PREC_FP32 && ctx->use_sym && ctx->sym == 1 && !ctx->have_R32) {
+        // the real basis in FP32
+        const int64_t rows = ctx->Kp + kRowPad;
+        if ((rc = ensure(ctx, ctx->d_R32, ctx->R32_cap, (size_t)(rows * ctx->Np)))) return rc;
+        KLSP_HIP(ctx, split_R_f32(ctx->d_W64, rows, (int)ctx->N, ctx->Np, ctx->d_R32,
+                                  ctx->stream));
+        KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->have_R32 = true;
+    }
+    if (precision == KLSP_PREC_FP32 && !(ctx->use_sym && ctx->sym == 1) && !ctx->have_W32) {
         KLSP_HIP(ctx, hipSetDevice(ctx->device));
         const int64_t n = (ctx->Kp + kRowPad) * ctx->Np;
         if ((rc = ensure(ctx, ctx->d_W32, ctx->W32_cap, (size_t)n))) return rc;
@@ -2062,6 +2132,34 @@ static int launch_contract(klsp_ctx* ctx, int precision, int nb, int nf)
             return fail(ctx, KLSP_ERR_HIP, std::string("contraction launch: ") + hipGetErrorString(e));
         ctx->last_S = bal->n_slots;
         ctx->last_count = bal->d_count;
+        ctx->last_sym = true;
+        return KLSP_OK;
+    }
+    if (precision == KLSP_PREC_FP32 && ctx->use_sym && ctx->sym == 1 && ctx->have_R32) {
+        // one launch per weight set: set w reads its weights at d_cf + w Kpc
+        // and writes its slices at d_P + w nsl Np^2 (the reduce's layout)
+        const klsp_ctx::Balanced* b = nullptr;
+        if ((rc = get_balanced_sym32(ctx, &b))) return rc;
+        const int64_t Kpc = ctx->Kp + kRowPad;
+        for (int w = 0; w < nb; ++w) {
+            ContractArgs a{};
+            a.units = b->d_units;
+            a.n_units = b->n_units;
+            a.W = ctx->d_R32;
+            a.cwf = ctx->d_cf + w * Kpc;
+            a.P = ctx->d_P + (int64_t)w * b->n_slots * ctx->Np * ctx->Np;
+            a.Kp = ctx->Kp;
+            a.Np = ctx->Np;
+            a.n_gangs = b->n_slots;
+            a.wrows = b->wrows;
+            a.wcols = b->wcols;
+            const hipError_t e = contract_sym32_launch(a, ctx->stream);
+            if (e != hipSuccess)
+                return fail(ctx, KLSP_ERR_HIP,
+                            std::string("contraction launch: ") + hipGetErrorString(e));
+        }
+        ctx->last_S = b->n_slots;
+        ctx->last_count = b->d_count;
         ctx->last_sym = true;
         return KLSP_OK;
     }
@@ -2138,8 +2236,8 @@ static int build_batch(klsp_ctx* ctx, const double* om, int nb, int precision,
     const int64_t Kpc = ctx->Kp + kRowPad;
     int rc;
     if ((rc = ensure(ctx, ctx->d_c, ctx->c_cap, (size_t)kMaxW * Kpc))) return rc;
-    float2* cf = nullptr;
-    if (precision == KLSP_PREC_BF16X3) {
+    float2* cf = nullptr;   // the complex64 copy the BF16x3 and FP32 builds read
+    if (precision == KLSP_PREC_BF16X3 || precision == KLSP_PREC_FP32) {
         if ((rc = ensure(ctx, ctx->d_cf, ctx->cf_cap, (size_t)kMaxW * Kpc))) return rc;
         cf = ctx->d_cf;
     }
@@ -2260,7 +2358,12 @@ static int check_ready(klsp_ctx* ctx, int precision, const char* who)
     if (precision != KLSP_PREC_FP64 && precision != KLSP_PREC_FP32 &&
         precision != KLSP_PREC_BF16X3)
         return fail(ctx, KLSP_ERR_INVALID_ARG, std::string(who) + ": unknown precision");
-    if (precision == KLSP_PREC_FP32 && !ctx->have_W32) return klsp_prepare(ctx, precision);
+    if (precision == KLSP_PREC_FP32 && !ctx->have_W32 && !ctx->have_R32)
+        return klsp_prepare(ctx, precision);
+    if (precision == KLSP_PREC_FP32 && ctx->use_sym && ctx->sym == 1 && !ctx->have_R32)
+        return klsp_prepare(ctx, precision);
+    if (precision == KLSP_PREC_FP32 && !(ctx->use_sym && ctx->sym == 1) && !ctx->have_W32)
+        return klsp_prepare(ctx, precision);
     if (precision == KLSP_PREC_BF16X3 && !ctx->have_Wb && !ctx->have_Rb)
         return klsp_prepare(ctx, precision);
     if (precision == KLSP_PREC_BF16X3 && ctx->use_sym && ctx->sym == 1 && !ctx->have_Rb)
@@ -2314,7 +2417,9 @@ static std::vector<int64_t> scan_graph_key(const klsp_ctx* ctx, int64_t n, int p
         P(ctx->d_c), P(ctx->d_cf), P(ctx->d_P), P(ctx->d_S), P(ctx->d_M), P(ctx->d_A),
         P(ctx->d_DJ), P(ctx->d_F0l), P(ctx->d_Ores), P(ctx->d_Nk), P(ctx->d_scan_om),
         P(ctx->d_scan_det), P(ctx->bal_sym.d_units), P(ctx->bal_sym.d_count),
-        ctx->bal_sym.n_units, ctx->bal_sym.n_slots};
+        ctx->bal_sym.n_units, ctx->bal_sym.n_slots, P(ctx->d_R32), ctx->have_R32,
+        P(ctx->bal_sym32.d_units), P(ctx->bal_sym32.d_count), ctx->bal_sym32.n_units,
+        ctx->bal_sym32.n_slots};
     for (const auto& row : ctx->bal)
         for (const auto& b : row) {
             k.push_back(P(b.d_units));
@@ -3086,12 +3191,13 @@ int klsp_device_bytes(klsp_ctx* ctx, int64_t* bytes_out)
     b += (int64_t)ctx->M_cap * sizeof(double2) + (int64_t)ctx->A_cap * sizeof(double2);
     b += (int64_t)ctx->det_cap * sizeof(double2) + (int64_t)ctx->S_cap * sizeof(double2);
     b += (int64_t)ctx->ones_cap * sizeof(double) + (int64_t)ctx->scan_om_cap * sizeof(double);
-    b += (int64_t)ctx->scan_det_cap * sizeof(double2);
+    b += (int64_t)ctx->scan_det_cap * sizeof(double2) + (int64_t)ctx->R32_cap * sizeof(float);
     if (ctx->have_problem)   // DJ, F0l, Omega_res, N_kernel
         b += (ctx->NPh + 2 * ctx->K + ctx->N) * (int64_t)sizeof(double);
     for (const auto& row : ctx->bal)
         for (const auto& t : row) b += (int64_t)t.n_units * sizeof(WorkUnit) + 4096;
     b += (int64_t)ctx->bal_sym.n_units * sizeof(WorkUnit) + 4096;
+    b += (int64_t)ctx->bal_sym32.n_units * sizeof(WorkUnit) + 4096;
     *bytes_out = b;
     return KLSP_OK;
 }

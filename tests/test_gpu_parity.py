@@ -873,3 +873,37 @@ def test_symmetric_basis_bf16x3(N):
             assert abs(dc[k] - d64[k]) <= 1e-3 * sc
     finally:
         pre.ctx.close()
+
+
+@pytest.mark.parametrize("N", [1, 2, 5, 16, 17, 31, 33, 64, 65, 130, 301])
+def test_symmetric_basis_fp32(N):
+    """FP32 builds in the real basis (4 x 2 tiles over the upper triangle,
+    K-tile sums in FP64): M within 2e-6 normwise of the FP64 build, as the
+    complex FP32 build is; the three Taylor terms of an FP32 pass alike; dets
+    within 1e-5 of FP64; one launch per weight set in three-omega batches."""
+    rng = np.random.default_rng(5000 + N)
+    NPh, n_l = 131, 3
+    W = mirror_symmetric_W(rng, NPh, N, n_l, 1.0 / np.sqrt(NPh * n_l))
+    DJ, F0l = rng.uniform(0.2, 1.0, NPh), rng.standard_normal((NPh, n_l))
+    Ores, Nk = rng.uniform(-2, 2, (NPh, n_l)), rng.uniform(0.5, 1.5, N)
+    pre = from_arrays(2, 0.8, W, DJ, F0l, Ores, Nk)
+    try:
+        om = 0.4 + 0.25j
+        M64 = pre.ctx.build_M(om, _ffi.PREC_FP64)
+        Mf = pre.ctx.build_M(om, _ffi.PREC_FP32)
+        assert relnorm(Mf, M64) <= 2e-6, relnorm(Mf, M64)
+        T64 = pre.ctx.build_taylor(om, 3, _ffi.PREC_FP64)
+        Tf = pre.ctx.build_taylor(om, 3, _ffi.PREC_FP32)
+        for j in range(3):
+            assert relnorm(Tf[j], T64[j]) <= 2e-6, (j, relnorm(Tf[j], T64[j]))
+        oms = [om, 0.9 + 0.1j, 0.2 + 0.3j, 0.6 + 0.2j]
+        df = pre.ctx.det(oms, _ffi.PREC_FP32)
+        d64 = pre.ctx.det(oms, _ffi.PREC_FP64)
+        for a, b in zip(df, d64):
+            assert abs(a - b) <= 1e-5 * max(abs(b), 1e-300), (a, b)
+        pre.ctx.set_symmetric_basis(False)
+        Mc = pre.ctx.build_M(om, _ffi.PREC_FP32)
+        pre.ctx.set_symmetric_basis(True)
+        assert relnorm(Mc, M64) <= 2e-6
+    finally:
+        pre.ctx.close()
