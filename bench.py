@@ -47,12 +47,12 @@ PEAK_HBM_GBS = 8000.0
 # kernel name -> FETCH + WRITE bytes per launch (the autotuner may keep any
 # variant; each was profiled in the same run)
 PROFILED_TRAFFIC_BYTES = {
-    "klsp::k_contract_sym64<4, false>": 37.395e9 + 0.056e9,
-    "klsp::k_contract_sym64<4, true>": 40.111e9 + 0.062e9,
-    "klsp::k_contract_sym64<6, false>": 35.894e9 + 0.056e9,
-    "klsp::k_contract_sym64<6, true>": 35.778e9 + 0.062e9,
-    "klsp::k_contract_sym64<8, false>": 35.940e9 + 0.056e9,
-    "klsp::k_contract_sym64<8, true>": 35.946e9 + 0.062e9,
+    "klsp::k_contract_sym64<4, false>": 37.51e9 + 0.056e9,
+    "klsp::k_contract_sym64<4, true>": 39.69e9 + 0.062e9,
+    "klsp::k_contract_sym64<6, false>": 35.95e9 + 0.056e9,
+    "klsp::k_contract_sym64<6, true>": 35.82e9 + 0.062e9,
+    "klsp::k_contract_sym64<8, false>": 35.97e9 + 0.056e9,
+    "klsp::k_contract_sym64<8, true>": 36.04e9 + 0.062e9,
 }
 
 
@@ -223,22 +223,28 @@ def main() -> int:
                                              * NPh / (W_ms * 1e-3) / 1e12}
 
     if not args.no_fp32:
-        ctx.prepare(_ffi.PREC_FP32)
-        ctx.det([om0], _ffi.PREC_FP32)
-        n32 = min(max(3, args.steps // 2), len(oms) - args.warmup)
-        c32 = 0.0
-        t0 = time.perf_counter()
-        d32 = []
-        for i in range(n32):
-            d32.append(ctx.det([oms[args.warmup + i]], _ffi.PREC_FP32)[0])
-            c32 += ctx.last_timing()["contract_ms"]
-        e32 = time.perf_counter() - t0
-        rel = max(abs(d32[i] - dets[i]) / abs(dets[i]) for i in range(min(n32, len(dets))))
-        extra.update({"det_per_s_fp32": n32 / e32, "contract_ms_fp32": c32 / n32,
-                      "fp32_contract_tflops": flop_complex / (c32 / n32 * 1e-3) / 1e12,
-                      "fp32_det_rel_err_vs_fp64": rel})
-        log(f"[bench] FP32 det: {1e3 * e32 / n32:.2f} ms/det, contraction {c32 / n32:.2f} ms, "
-            f"det rel err vs FP64 {rel:.2e}")
+        # FP32 builds: the real basis (what FP32 dets use on this symmetric
+        # grid), then the complex form (asymmetric grids)
+        for form, real_basis in (("", True), ("_complex_form", False)):
+            ctx.set_symmetric_basis(real_basis)
+            ctx.prepare(_ffi.PREC_FP32)
+            ctx.det([om0], _ffi.PREC_FP32)
+            n32 = min(max(3, args.steps // 2), len(oms) - args.warmup)
+            c32 = 0.0
+            t0 = time.perf_counter()
+            d32 = []
+            for i in range(n32):
+                d32.append(ctx.det([oms[args.warmup + i]], _ffi.PREC_FP32)[0])
+                c32 += ctx.last_timing()["contract_ms"]
+            e32 = time.perf_counter() - t0
+            rel = max(abs(d32[i] - dets[i]) / abs(dets[i]) for i in range(min(n32, len(dets))))
+            flop32 = flop_sym if real_basis and sym else flop_complex
+            extra.update({f"det_per_s_fp32{form}": n32 / e32, f"contract_ms_fp32{form}": c32 / n32,
+                          f"fp32_contract_tflops{form}": flop32 / (c32 / n32 * 1e-3) / 1e12,
+                          f"fp32_det_rel_err_vs_fp64{form}": rel})
+            log(f"[bench] FP32 det{form.replace('_', ' ')}: {1e3 * e32 / n32:.2f} ms/det, "
+                f"contraction {c32 / n32:.2f} ms, det rel err vs FP64 {rel:.2e}")
+        ctx.set_symmetric_basis(True)
 
     if not args.no_fp32:
         ctx.prepare(_ffi.PREC_BF16X3)
