@@ -334,10 +334,13 @@ typedef struct klsp_newton_result {
  *      FP64-accurate; its zero near s_a (the same 16-point circle, radius
  *      anchor_tol below) is confirmed by Newton steps on it (FP64 LUs) until
  *      the step stops shrinking; converged if then |det| < tol.  A new anchor is built if
- *      the root moves so far from the last one that the low-precision
- *      terms' error (their model's slope error + ~1e-4 BF16x3 / 1e-5 FP32)
- *      times that move could exceed 5e-11, at the anchored model's own zero
- *      (converged on first).
+ *      the root moves so far from the last one that the model's slope error
+ *      times that move could exceed 5e-11: the slope error is estimated as
+ *      twice the truncation of the pass's series (its first omitted term,
+ *      bounded by |T_nt| / Im w_c) plus four times the rounding error the
+ *      low-precision terms showed at the first anchor (the move of the
+ *      pass's zero times |T_2| / |T_1|); the new anchor is built at the
+ *      anchored model's own zero (converged on first).
  * iterations = low-precision passes + anchors + 1; history rows record each
  * centre with its low-precision |det|, each anchor with its FP64 |det|, and
  * the final point.  Bounds (gamma >= 1e-4, Omega_p in [0, 1]) apply to

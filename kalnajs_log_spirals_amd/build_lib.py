@@ -10,6 +10,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -53,14 +54,18 @@ def _run(cmd):
 
 
 def build(verbose: bool = True) -> str:
+    """Compile every source (in parallel, MAX_JOBS or up to 8 at once) and link."""
     os.makedirs(LIBDIR, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src, is_hip, extra in SOURCES:
         obj = os.path.join(LIBDIR, src.rsplit(".", 1)[0] + ".o")
         cmd = [HIPCC] + COMMON + (DEVICE + ["-x", "hip"] if is_hip else ["-x", "c++"]) + extra
         cmd += ["-c", os.path.join(CSRC, src), "-o", obj]
-        _run(cmd)
+        cmds.append(cmd)
         objs.append(obj)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "8"))))
+    with ThreadPoolExecutor(jobs) as pool:
+        list(pool.map(_run, cmds))
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, LIB)

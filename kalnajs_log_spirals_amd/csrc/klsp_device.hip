@@ -817,6 +817,35 @@ __global__ void k_expand_sym(const double2* __restrict__ Sall, int N, int nb, do
     }
 }
 
+// Sum of squares of |A_b - B_b| (B null: of |A_b|) over the NN entries of
+// matrix b = blockIdx.x, in FP64 -> out[b] (the Taylor-term norms of the
+// mixed search's error bound).
+__global__ __launch_bounds__(256) void k_sumsq(const double2* __restrict__ A,
+                                               const double2* __restrict__ B, int64_t NN,
+                                               double* __restrict__ out)
+{
+    __shared__ double red[256];
+    const double2* a = A + (int64_t)blockIdx.x * NN;
+    const double2* b = B ? B + (int64_t)blockIdx.x * NN : nullptr;
+    double acc = 0.0;
+    for (int64_t x = threadIdx.x; x < NN; x += blockDim.x) {
+        double2 v = a[x];
+        if (b) {
+            const double2 u = b[x];
+            v.x -= u.x;
+            v.y -= u.y;
+        }
+        acc = fma(v.x, v.x, fma(v.y, v.y, acc));
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
 // =====================================================================
 // Stage 4: det(I - M) by complex LU with partial pivoting, one workgroup
 // per matrix (MatrixBuilder.jl:182 -> LinearAlgebra det -> LAPACK zgetrf).
@@ -1044,6 +1073,7 @@ __global__ __launch_bounds__(kLuThreads) void k_lu_det(const double2* __restrict
 constexpr int kSymDefaultVariant = 2;   // depth 8, upper-triangle diagonal tiles
 constexpr int kCircle = 16;               // circle points of a model-det zero search
 constexpr int kModelPts = kCircle + 1;    // with the centre: matrices per model LU launch
+constexpr int kNorms = kMaxW;             // Frobenius norms of a pass's Taylor terms
 struct klsp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -1067,6 +1097,7 @@ struct klsp_ctx {
     // workspaces
     double* d_om = nullptr;
     double2* d_coef = nullptr;  // [kModelPts][kMaxW] Taylor-model coefficients
+    double* d_norm = nullptr;   // [kNorms] sums of squares (k_sumsq) for the mixed search
     double2* d_c = nullptr;   size_t c_cap = 0;
     float2* d_cf = nullptr;   size_t cf_cap = 0;   // FP32 copy of the weights
     double2* d_P = nullptr;   size_t P_cap = 0;
@@ -1634,6 +1665,7 @@ int klsp_create(int device, klsp_ctx** ctx_out)
     for (auto& e : ctx->ev) hipEventCreate(&e);
     if (hipMalloc((void**)&ctx->d_om, sizeof(double) * 2 * kMaxBatch) != hipSuccess ||
         hipMalloc((void**)&ctx->d_coef, sizeof(double2) * kModelPts * kMaxW) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_norm, sizeof(double) * kNorms) != hipSuccess ||
         hipMalloc((void**)&ctx->d_det, sizeof(double2) * kMaxBatch) != hipSuccess ||
         hipMalloc((void**)&ctx->d_flag, sizeof(int)) != hipSuccess) {
         ctx->det_cap = 0;
@@ -1669,7 +1701,7 @@ int klsp_destroy(klsp_ctx* ctx)
     free_problem(ctx);
     release_W(ctx);
     hipFree(ctx->d_om); hipFree(ctx->d_c); hipFree(ctx->d_P); hipFree(ctx->d_M);
-    hipFree(ctx->d_A); hipFree(ctx->d_det); hipFree(ctx->d_coef); hipFree(ctx->d_cf);
+    hipFree(ctx->d_A); hipFree(ctx->d_det); hipFree(ctx->d_coef); hipFree(ctx->d_norm); hipFree(ctx->d_cf);
     hipFree(ctx->d_S); hipFree(ctx->d_flag); hipFree(ctx->d_ones);
     if (ctx->scan_exec) hipGraphExecDestroy(ctx->scan_exec);
     hipFree(ctx->d_scan_om); hipFree(ctx->d_scan_det);
@@ -2661,6 +2693,8 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
     // error).  A pass ends once a model Newton step below accept_step is
     // applied: from there the step error is ~|step|^2 / Im w.
     const int nt_max = lp_prec == KLSP_PREC_BF16X3 ? kMaxTerms : kMaxBatch;
+    const int64_t NN = ctx->N * ctx->N;
+    double tnorm[kMaxW] = {};           // Frobenius norms of the last pass's terms
     int nt = nt_max;                    // Taylor terms of the next pass
     int nt_last = nt;                   // ... of the last pass built
     double slope_last = 1.0;            // predicted relative slope error of its model
@@ -2671,7 +2705,14 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
         // model times s^3 and the anchored finish times 3 s_a^2 (s - s_a))
         if ((rc = build_batch(ctx, om, nt, lp_prec, true, 0, nt == kMaxW ? kMaxW - 1 : nt)))
             return rc;
+        hipLaunchKernelGGL(k_sumsq, dim3(nt), dim3(256), 0, ctx->stream, ctx->d_M,
+                           (const double2*)nullptr, NN, ctx->d_norm);
+        KLSP_HIP(ctx, hipGetLastError());
+        double tn2[kMaxW];
+        KLSP_HIP(ctx, hipMemcpyAsync(tn2, ctx->d_norm, sizeof(double) * nt, hipMemcpyDeviceToHost,
+                                     ctx->stream));
         KLSP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (int j = 0; j < nt; ++j) tnorm[j] = std::sqrt(tn2[j]);
         acc_timing(acc, ctx);
         ++n_lp;
         nt_last = nt;
@@ -2731,16 +2772,24 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
     // default grid, it admits |d omega| ~ 1e-8.)  Newton on the model needs
     // FP64 LUs only; the anchor is rebuilt where the root moves too far.
     const double h64 = 1e-5;            // central-difference step on the model
-    // The anchored model's error moves the root by ~|s - s_a| (slope error of
-    // the last pass's model + the low-precision terms' own relative error,
-    // up to ~1e-4 for BF16x3 and 1e-5 for FP32 at the det level: measured
-    // <= 4e-5 at the default grid, ~1e-4 on a coarse test grid); a move
-    // beyond anchor_tol could leave more than 5e-11 (half the 1e-10 the
-    // roots are held to), so a new anchor is built there.  (Budgeting 2e-4
-    // for BF16x3 was measured: the default-grid root, whose low-precision
-    // root lies 1.7e-7 off, then needs a second FP64 build.)
-    const double eps_lp = lp_prec == KLSP_PREC_BF16X3 ? 1e-4 : 1e-5;
-    const double anchor_tol = 5e-11 / (std::min(slope_last, 1.0) + eps_lp);
+    // The anchored model's error moves its zero by ~sigma |s - s_a|, sigma the
+    // relative error of the model's slope, which has two parts:
+    //  - truncation of the pass's series: the first term left out enters the
+    //    slope as nt s_a^(nt-1) T_{nt+1}, and |T_{nt+1}| <= |T_nt| / Im w_c
+    //    (every pole of the weights lies on the real axis);
+    //  - rounding of the low-precision terms, which moved the pass's zero by
+    //    delta_b = |s - s_a| at the first anchor: the slope T_2 carries
+    //    ~delta_b |T_2| / |T_1| of relative error.
+    // A zero is accepted where
+    //   |s - s_a| (2 trunc + 4 delta_b |T_2| / |T_1|) <= 5e-11,
+    // half the 1e-10 the roots are held to; further out a new anchor is built
+    // at the model's zero.  (Over the default grid and the 48 uneven draws
+    // the first anchor's zero lies at most 1.9x this estimate from the FP64
+    // root; default grid: truncation 3.9e-5, the BF16x3 zero 5.0e-7 off,
+    // anchor_tol 6.0e-7, one FP64 build.)  Without finite term
+    // norms the bound falls back to nt r^(nt-1) + 1e-4 (BF16x3) or 1e-5 (FP32).
+    const double eps_fallback = lp_prec == KLSP_PREC_BF16X3 ? 1e-4 : 1e-5;
+    double delta_b = NAN;               // |s - s_a| at the first anchor
     cd D(NAN, NAN);
     int conv = 0;
     for (int anchor = 0; anchor < std::max(1, np->max_iter); ++anchor) {
@@ -2756,6 +2805,18 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
             ++n_64;
         }
         const cd sa = w - wc;
+        // anchor_tol: the largest |s - s_a| the bound above accepts (at the
+        // first anchor delta_b is that same distance: a quadratic)
+        double anchor_tol = NAN;
+        if (nt_last >= 2 && tnorm[0] > 0.0 && tnorm[1] > 0.0 && std::isfinite(tnorm[nt_last - 1])) {
+            const double a = 2.0 * nt_last * std::pow(std::abs(sa), nt_last - 1) *
+                             (tnorm[nt_last - 1] / std::max(wc.imag(), 1e-300)) / tnorm[1];
+            const double b = 4.0 * tnorm[1] / tnorm[0];
+            anchor_tol = anchor == 0 ? 2.0 * 5e-11 / (a + std::sqrt(a * a + 4.0 * b * 5e-11))
+                                     : 5e-11 / (a + b * delta_b);
+        }
+        if (!(anchor_tol > 0.0) || !std::isfinite(anchor_tol))
+            anchor_tol = 5e-11 / (std::min(slope_last, 1.0) + eps_fallback);
         cd s = sa;
         double prev_step = INFINITY;
         bool far = false;
@@ -2813,6 +2874,7 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
             // further steps on this anchor's model change nothing it needs
             if (far && step * step < 1e-3 * anchor_tol * wc.imag()) break;
         }
+        if (anchor == 0) delta_b = std::abs(s - sa);
         w = wc + s;
         if (!far) {
             conv = stepped && std::abs(D) < np->tol;

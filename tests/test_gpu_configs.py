@@ -208,7 +208,8 @@ def test_config2_default_grid_fp64_newton(default_grid, default_fp64_root):
 
 def test_config3_default_grid_mixed_root(default_grid, default_fp64_root):
     """The mixed-precision search (BF16x3 Taylor passes, FP64 anchored finish)
-    lands within 1e-10 of the FP64 root with at most two FP64 builds."""
+    lands within 1e-10 of the FP64 root with one BF16x3 pass and one FP64
+    build (the pass's zero lies 5e-7 off, inside the first anchor's bound)."""
     cfg, pre, dev = default_grid
     rm = K.newton_search(cfg, pre, dev, K.GPUType.AMD, np.float64, *OMEGA0, precision="mixed")
     dw = abs(rm.omega - default_fp64_root.omega)
@@ -216,8 +217,7 @@ def test_config3_default_grid_mixed_root(default_grid, default_fp64_root):
           f"builds, |d omega| {dw:.2e}", flush=True)
     assert rm.converged
     assert dw <= 1e-10
-    assert rm.n_fp64_builds <= 2
-    assert rm.n_fp64_builds < default_fp64_root.n_det_evals
+    assert rm.n_fp64_builds == 1 and rm.n_lp_builds == 1
 
 
 def test_config4_default_grid_64x64_scan(default_grid):
