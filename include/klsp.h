@@ -375,6 +375,15 @@ int klsp_grid_scan(klsp_ctx* ctx, const klsp_scan_params* sp, double* det_grid_o
  * has instantiated and launched. */
 int klsp_set_scan_graph(klsp_ctx* ctx, int on);
 
+/* Which LU computes det(I - M) (no counterpart in the reference; for tests
+ * and measurement).  0 (default): batches of at most 24 matrices of order
+ * N <= 512 take the step-wise LU, one launch per 16-column panel with a
+ * workgroup per column block, so one det spreads over many CUs; larger
+ * batches and larger N take one workgroup per matrix.  1 forces the
+ * one-workgroup LU, 2 the step-wise LU (for N <= 512; larger N keep the
+ * one-workgroup LU).  Both follow zgetrf's pivoting and give the same bits. */
+int klsp_set_lu_form(klsp_ctx* ctx, int form);
+
 /* Device memory the context holds between calls: W and its copies, the
  * problem arrays and the grow-only workspaces, in bytes (no counterpart in
  * the reference).  Orbit samples and quadrature temporaries of

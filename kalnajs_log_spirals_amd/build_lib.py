@@ -45,6 +45,7 @@ SOURCES = [
     ("klsp_contract_bf16.hip", True, ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
     ("klsp_contract_symbf16.hip", True, ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
     ("klsp_contract_sym32.hip", True, ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
+    ("klsp_lu.hip", True, []),
 ]
 
 

@@ -28,7 +28,9 @@
 
 #include "klsp.h"
 #include "klsp_contract.hpp"
+#include "klsp_cplx.hpp"
 #include "klsp_internal.hpp"
+#include "klsp_lu.hpp"
 
 using namespace klsp;
 
@@ -39,28 +41,6 @@ using namespace klsp;
 // Device helpers
 // =====================================================================
 
-// 1/(c + i d) by Smith's algorithm with the same range scaling as Julia's
-// inv(::ComplexF64); the weights use it so c(omega) rounds like the reference.
-__device__ __forceinline__ double2 d_inv(double c, double d)
-{
-    double s = 1.0;
-    const double cd = fmax(fabs(c), fabs(d));
-    const double eps = 2.220446049250313e-16;
-    if (cd >= 0.5 * 1.7976931348623157e308) { c *= 0.5; d *= 0.5; s *= 0.5; }
-    if (cd <= 2.2250738585072014e-308 * 2.0 / eps) {
-        const double bs = 2.0 / (eps * eps);
-        c *= bs; d *= bs; s *= bs;
-    }
-    double p, q;
-    if (fabs(d) <= fabs(c)) {
-        const double r = d / c, t = 1.0 / (c + d * r);
-        p = t; q = -r * t;
-    } else {
-        const double r = c / d, t = 1.0 / (d + c * r);
-        p = r * t; q = -t;
-    }
-    return make_double2(p * s, q * s);
-}
 
 // sin x and cos x for |x| < 2^20, without branches: Cody-Waite reduction by
 // pi/2 with pi/2 split into three doubles (x - q P1 - q P2 - q P3, each step
@@ -103,10 +83,6 @@ __device__ __forceinline__ void sincos_small(double x, double* sn, double* cs)
 // Largest |argument| given to sincos_small by the W quadrature.
 constexpr double kSmallTrigArg = 1048576.0;   // 2^20
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b)
-{
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
 
 // =====================================================================
 // Stage 1: W_l(alpha) orbit quadrature (BasisFunctions.jl:199-266)
@@ -870,21 +846,6 @@ __device__ __forceinline__ void argmax_merge(double& v, int& idx, double ov, int
     if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
 }
 
-// x - a b for complex values with four fused multiply-adds (the LU's
-// multiply-subtract; zgetrf's BLAS kernels fuse the same way)
-__device__ __forceinline__ double2 cfms(double2 x, double2 a, double2 b)
-{
-    return make_double2(fma(-a.x, b.x, fma(a.y, b.y, x.x)), fma(-a.x, b.y, fma(-a.y, b.x, x.y)));
-}
-
-// lane src's value in every lane (src wave-uniform)
-__device__ __forceinline__ double bcast(double v, int src)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, src);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
 
 
 // With coef == nullptr matrix b is I - M[b]; otherwise it is
@@ -1143,6 +1104,11 @@ struct klsp_ctx {
     hipGraphExec_t scan_exec = nullptr;
     std::vector<int64_t> scan_key;
     int scan_graph = 1;            // klsp_set_scan_graph
+    // LU form (klsp_set_lu_form): 0 = step-wise for batches of at most
+    // kLuStepMaxMatrices of order <= kLuStepMaxN, one workgroup per matrix
+    // otherwise; 1 / 2 force one (2 only up to order kLuStepMaxN)
+    void* d_luaux = nullptr;  size_t luaux_cap = 0;   // step-wise LU bookkeeping (bytes)
+    int lu_form = 0;
     int64_t scan_graph_builds = 0, scan_graph_launches = 0;
     // profiling
     bool prof = false;
@@ -1702,6 +1668,7 @@ int klsp_destroy(klsp_ctx* ctx)
     release_W(ctx);
     hipFree(ctx->d_om); hipFree(ctx->d_c); hipFree(ctx->d_P); hipFree(ctx->d_M);
     hipFree(ctx->d_A); hipFree(ctx->d_det); hipFree(ctx->d_coef); hipFree(ctx->d_norm); hipFree(ctx->d_cf);
+    hipFree(ctx->d_luaux);
     hipFree(ctx->d_S); hipFree(ctx->d_flag); hipFree(ctx->d_ones);
     if (ctx->scan_exec) hipGraphExecDestroy(ctx->scan_exec);
     hipFree(ctx->d_scan_om); hipFree(ctx->d_scan_det);
@@ -2343,14 +2310,40 @@ static void add_build_timing(klsp_ctx* ctx, int nb)
     ctx->last.contract_launches += 1;
 }
 
+// Whether an LU launch of nd matrices takes the step-wise LU (klsp_lu.hip).
+static bool lu_steps(const klsp_ctx* ctx, int nd)
+{
+    return ctx->N <= kLuStepMaxN &&
+           (ctx->lu_form == 2 || (ctx->lu_form == 0 && nd <= kLuStepMaxMatrices));
+}
+
+// The step-wise LU's bookkeeping for nd matrices (grow-only; callers that
+// capture a graph make sure of it before the capture starts).
+static int ensure_lu_aux(klsp_ctx* ctx, int nd)
+{
+    unsigned char* aux = static_cast<unsigned char*>(ctx->d_luaux);
+    const int rc = ensure(ctx, aux, ctx->luaux_cap, (size_t)nd * lu_step_aux_bytes());
+    ctx->d_luaux = aux;
+    return rc;
+}
+
 // LU launch over matrices [0, nd) of ctx->d_M (coef as k_lu_det), dets to
 // ctx->d_det, with its device time added to the profile.
 static int launch_lu(klsp_ctx* ctx, int nd, const double2* coef, int nt,
                      double2* det_out = nullptr)
 {
+    const bool steps = lu_steps(ctx, nd);
+    if (steps) {
+        int rc;
+        if ((rc = ensure_lu_aux(ctx, nd))) return rc;
+    }
     if (ctx->prof) hipEventRecord(ctx->ev[4], ctx->stream);
-    hipLaunchKernelGGL(k_lu_det, dim3(nd), dim3(kLuThreads), 0, ctx->stream, ctx->d_M, coef, nt,
-                       ctx->d_A, (int)ctx->N, det_out ? det_out : ctx->d_det);
+    if (steps)
+        KLSP_HIP(ctx, launch_lu_steps(ctx->stream, ctx->d_M, coef, nt, ctx->d_A, (int)ctx->N, nd,
+                                      ctx->d_luaux, det_out ? det_out : ctx->d_det));
+    else
+        hipLaunchKernelGGL(k_lu_det, dim3(nd), dim3(kLuThreads), 0, ctx->stream, ctx->d_M, coef,
+                           nt, ctx->d_A, (int)ctx->N, det_out ? det_out : ctx->d_det);
     if (ctx->prof) hipEventRecord(ctx->ev[5], ctx->stream);
     KLSP_HIP(ctx, hipGetLastError());
     return KLSP_OK;
@@ -2451,7 +2444,7 @@ static std::vector<int64_t> scan_graph_key(const klsp_ctx* ctx, int64_t n, int p
         P(ctx->d_scan_det), P(ctx->bal_sym.d_units), P(ctx->bal_sym.d_count),
         ctx->bal_sym.n_units, ctx->bal_sym.n_slots, P(ctx->d_R32), ctx->have_R32,
         P(ctx->bal_sym32.d_units), P(ctx->bal_sym32.d_count), ctx->bal_sym32.n_units,
-        ctx->bal_sym32.n_slots};
+        ctx->bal_sym32.n_slots, P(ctx->d_luaux), ctx->lu_form};
     for (const auto& row : ctx->bal)
         for (const auto& b : row) {
             k.push_back(P(b.d_units));
@@ -2492,6 +2485,7 @@ static int scan_by_graph(klsp_ctx* ctx, const double* om, int64_t n, int precisi
     if ((rc = ensure(ctx, ctx->d_det, ctx->det_cap, (size_t)std::max(slots, kMaxBatch)))) return rc;
     if ((rc = ensure(ctx, ctx->d_scan_om, ctx->scan_om_cap, (size_t)2 * n))) return rc;
     if ((rc = ensure(ctx, ctx->d_scan_det, ctx->scan_det_cap, (size_t)n))) return rc;
+    if ((rc = ensure_lu_aux(ctx, std::max(slots, kMaxBatch)))) return rc;
     KLSP_HIP(ctx, hipMemcpyAsync(ctx->d_scan_om, om, sizeof(double) * 2 * n,
                                  hipMemcpyHostToDevice, ctx->stream));
     const bool prof = ctx->prof;
@@ -3268,6 +3262,13 @@ int klsp_set_scan_graph(klsp_ctx* ctx, int on)
 {
     if (!ctx) return KLSP_ERR_INVALID_ARG;
     ctx->scan_graph = on ? 1 : 0;
+    return KLSP_OK;
+}
+
+int klsp_set_lu_form(klsp_ctx* ctx, int form)
+{
+    if (!ctx || form < 0 || form > 2) return KLSP_ERR_INVALID_ARG;
+    ctx->lu_form = form;
     return KLSP_OK;
 }
 
