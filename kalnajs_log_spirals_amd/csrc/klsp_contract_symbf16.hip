@@ -181,28 +181,38 @@ __device__ __forceinline__ void load_A(const char* hb, int64_t plane, const unsi
     }
 }
 
+// The MFMAs of a stage.  With `next` set, each slot's A operands are
+// reloaded from the stage two ahead (`hn`) as soon as the slot's last MFMA
+// has issued, so those loads have about two stages to arrive.
 template <int MB, int NW, int NF>
-__device__ __forceinline__ void mfma_stage(const StageB<NW>& sb, const v4u (&Ah)[MB],
-                                           const v4u (&Al)[MB], const int (&csel)[MB],
-                                           v4f (&P)[NW][MB], v4f (&Q)[NW][MB], int g, int i)
+__device__ __forceinline__ void mfma_stage(const StageB<NW>& sb, v4u (&Ah)[MB], v4u (&Al)[MB],
+                                           const int (&csel)[MB], v4f (&P)[NW][MB],
+                                           v4f (&Q)[NW][MB], int g, int i, bool next,
+                                           const char* hn, int64_t plane,
+                                           const unsigned (&offA)[MB])
 {
 #pragma unroll
     for (int s = 0; s < MB; ++s) {
-        if (csel[s] < 0) continue;
-        const int col = 16 * csel[s] + i;
+        if (csel[s] >= 0) {
+            const int col = 16 * csel[s] + i;
 #pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            const v4u ph = sb.rec[w][0][g][col], qh = sb.rec[w][2][g][col];
-            P[w][s] = mfma(Ah[s], ph, P[w][s]);
-            Q[w][s] = mfma(Ah[s], qh, Q[w][s]);
-            if (w < NF) {
-                const v4u pl = sb.rec[w][1][g][col], ql = sb.rec[w][3][g][col];
-                P[w][s] = mfma(Al[s], ph, P[w][s]);
-                Q[w][s] = mfma(Al[s], qh, Q[w][s]);
-                P[w][s] = mfma(Ah[s], pl, P[w][s]);
-                Q[w][s] = mfma(Ah[s], ql, Q[w][s]);
+            for (int w = 0; w < NW; ++w) {
+                const v4u ph = sb.rec[w][0][g][col], qh = sb.rec[w][2][g][col];
+                P[w][s] = mfma(Ah[s], ph, P[w][s]);
+                Q[w][s] = mfma(Ah[s], qh, Q[w][s]);
+                if (w < NF) {
+                    const v4u pl = sb.rec[w][1][g][col], ql = sb.rec[w][3][g][col];
+                    P[w][s] = mfma(Al[s], ph, P[w][s]);
+                    Q[w][s] = mfma(Al[s], qh, Q[w][s]);
+                    P[w][s] = mfma(Ah[s], pl, P[w][s]);
+                    Q[w][s] = mfma(Ah[s], ql, Q[w][s]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
             }
-            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (next) {
+            Ah[s] = ld16(hn, offA[s]);
+            Al[s] = ld16(hn + plane, offA[s]);
         }
     }
 }
@@ -256,32 +266,43 @@ __global__ __launch_bounds__(kThreads) void k_contract_symbf16(ContractArgs a)
             P[w][s] = (v4f){0, 0, 0, 0};
             Q[w][s] = (v4f){0, 0, 0, 0};
         }
-    // two stages a trip on two register sets, each stage's loads issued one
-    // stage ahead; the trip has no exit between its stages (the range holds
-    // an even number of stages)
+    // two stages a trip on two register sets; a stage's B inputs are loaded
+    // one stage ahead, its A operands one stage ahead or (deep) two stages
+    // ahead, slot by slot as each slot of the stage two back finishes with
+    // them; the trip has no exit between its stages (the range holds an even
+    // number of stages)
     v4u A0h[MB], A0l[MB], A1h[MB], A1l[MB];
     BIn<NW> b0, b1;
+    // One weight set: a stage is 6 MB MFMAs, too few to cover a load issued
+    // one stage ahead, so A is loaded two stages ahead (measured 5.06 -> 4.66
+    // ms at the default grid).  Three or four weight sets give 18-24 MFMAs a
+    // slot; there the deeper prefetch measured 2-5% slower.
+    constexpr bool deep = NW == 1;
     if (Kb < Ke) {
         load_A<MB>(hb, plane, offA, A0h, A0l);
+        if (deep) load_A<MB>(hb + stage, plane, offA, A1h, A1l);
         load_B_in<NW>(hb, plane, wb, wstride, offB, offW, cvalid, b0);
     }
     for (int64_t K0 = Kb; K0 < Ke; K0 += kKq) {
         const char* h0 = hb + ((K0 - Kb) >> 5) * stage;
         const char* w0 = wb + (K0 - Kb) * (int64_t)sizeof(float2);
+        const bool more = K0 + kKq < Ke;
         produce_B<NW, NF>(sb[0], b0, rg, col);
         load_B_in<NW>(h0 + stage, plane, w0 + 32 * sizeof(float2), wstride, offB, offW, cvalid,
                       b1);
-        load_A<MB>(h0 + stage, plane, offA, A1h, A1l);
+        if (!deep) load_A<MB>(h0 + stage, plane, offA, A1h, A1l);
         lds_barrier();
-        mfma_stage<MB, NW, NF>(sb[0], A0h, A0l, csel, P, Q, g, i);
+        mfma_stage<MB, NW, NF>(sb[0], A0h, A0l, csel, P, Q, g, i, deep && more, h0 + 2 * stage,
+                               plane, offA);
         produce_B<NW, NF>(sb[1], b1, rg, col);
-        if (K0 + kKq < Ke) {
+        if (more) {
             load_B_in<NW>(h0 + 2 * stage, plane, w0 + 64 * sizeof(float2), wstride, offB, offW,
                           cvalid, b0);
-            load_A<MB>(h0 + 2 * stage, plane, offA, A0h, A0l);
+            if (!deep) load_A<MB>(h0 + 2 * stage, plane, offA, A0h, A0l);
         }
         lds_barrier();
-        mfma_stage<MB, NW, NF>(sb[1], A1h, A1l, csel, P, Q, g, i);
+        mfma_stage<MB, NW, NF>(sb[1], A1h, A1l, csel, P, Q, g, i, deep && more, h0 + 3 * stage,
+                               plane, offA);
     }
     // FP32 accumulator lanes hold rows 4g + v of a block (column i); partial
     // slices are column-major: column = B-side real column
