@@ -433,6 +433,9 @@ hipError_t split_W_bf16(const double2* W64, int64_t rows, int Np, int64_t nq, vo
 // symbf16_records(Kp) records of eight K rows (2 x records x Np x 16 bytes),
 // made from W by split_R_bf16.
 Bf16Geometry contract_symbf16_geometry(int Np, int nw, int nf);
+// Share of a launch's block slots (nw weight sets) holding a block of the
+// upper triangle: the slots past a pair's blocks idle through the launch.
+double contract_symbf16_slot_use(int Np, int nw);
 hipError_t contract_symbf16_launch(int nw, int nf, int mb, const ContractArgs& a,
                                    hipStream_t stream);
 int64_t symbf16_records(int64_t Kp);

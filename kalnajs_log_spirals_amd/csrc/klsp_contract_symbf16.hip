@@ -364,10 +364,12 @@ const void* kernel_nw(int mb)
     }
 }
 
-// nf < nw only for the four Taylor terms of the mixed search (three full)
+// nf < nw only for the mixed search's Taylor terms: four with three full,
+// or (split into two launches) two with one full
 const void* kernel_for(int nw, int nf, int mb)
 {
     if (nw >= 4 && nf < 4) return kernel_nw<4, 3>(mb);
+    if (nw == 2 && nf < 2) return kernel_nw<2, 1>(mb);
     return nw <= 1 ? kernel_nw<1, 1>(mb) : nw == 2 ? kernel_nw<2, 2>(mb)
          : nw == 3 ? kernel_nw<3, 3>(mb) : kernel_nw<4, 4>(mb);
 }
@@ -392,7 +394,7 @@ Bf16Geometry contract_symbf16_geometry(int Np, int nw, int nf)
     g.mb = mb;
     static std::atomic<int> occ[kMaxNW + 1][2][kMaxMB + 1] = {};
     const int i = nw < 1 ? 1 : (nw > kMaxNW ? kMaxNW : nw);
-    const int hh = (i >= 4 && nf < 4) ? 1 : 0;
+    const int hh = nf < i ? 1 : 0;
     if (occ[i][hh][mb] == 0) {
         int per_cu = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_for(i, nf, mb), kThreads,
@@ -403,6 +405,14 @@ Bf16Geometry contract_symbf16_geometry(int Np, int nw, int nf)
     }
     g.wg_per_cu = occ[i][hh][mb];
     return g;
+}
+
+// Share of the launch's block slots that hold an upper-triangle block.
+double contract_symbf16_slot_use(int Np, int nw)
+{
+    const Bf16Geometry g = contract_symbf16_geometry(Np, nw, nw);
+    const int nb = Np / 16;
+    return (double)nb * (nb + 1) / 2 / ((double)g.wcols * g.wrows * kWaves * g.mb);
 }
 
 hipError_t contract_symbf16_launch(int nw, int nf, int mb, const ContractArgs& a,

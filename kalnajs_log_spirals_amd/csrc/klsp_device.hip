@@ -1473,6 +1473,15 @@ static GangPlan plan_gangs_bf16(const klsp_ctx* ctx, int nw, int nf)
 // Gang geometry of the real-basis BF16 contraction: folded column pairs
 // (wcols) x workgroups per pair (wrows); gangs per XCD as for the complex
 // kernel; K ranges per gang in multiples of 32 rows.
+// Whether BF16x3 real-basis passes of three or four weight sets are better
+// run as passes of two: their block slots fill the triangle clearly better.
+static bool bf16_split(const klsp_ctx* ctx)
+{
+    return contract_symbf16_slot_use((int)ctx->Np, 2) >=
+           std::max(contract_symbf16_slot_use((int)ctx->Np, 3),
+                    contract_symbf16_slot_use((int)ctx->Np, 4)) + 0.05;
+}
+
 static GangPlan plan_gangs_symbf16(const klsp_ctx* ctx, int nw, int nf, int* mb)
 {
     const Bf16Geometry geo = contract_symbf16_geometry(ctx->Np, nw, nf);
@@ -2163,22 +2172,32 @@ static int launch_contract(klsp_ctx* ctx, int precision, int nb, int nf)
         return KLSP_OK;
     }
     if (bf && ctx->use_sym && ctx->sym == 1 && ctx->have_Rb) {
+        // four weight sets as two launches of two where that fills the
+        // block slots better (N = 301: 95% against 79%; 3.35 ms a weight set
+        // against 3.6); the second launch's weights and partial slices are
+        // those of sets 2 and 3, so the reduce sees one launch of four
+        const int nl = nb == 4 && bf16_split(ctx) ? 2 : 1;
+        const int nw = nb / nl;
         int mb = 0;
-        gp = plan_gangs_symbf16(ctx, nb, nf, &mb);
-        ContractArgs a{};
-        a.W = ctx->d_Rb;
-        a.cw = ctx->d_c;
-        a.cwf = ctx->d_cf;
-        a.P = ctx->d_P;
-        a.Kp = ctx->Kp;
-        a.Np = ctx->Np;
-        a.Kg = gp.Kg;
-        a.n_gangs = gp.n_gangs;
-        a.wrows = gp.wrows;
-        a.wcols = gp.wcols;
-        const hipError_t e = contract_symbf16_launch(nb, nf, mb, a, ctx->stream);
-        if (e != hipSuccess)
-            return fail(ctx, KLSP_ERR_HIP, std::string("contraction launch: ") + hipGetErrorString(e));
+        gp = plan_gangs_symbf16(ctx, nw, nl == 2 ? nw : nf, &mb);
+        for (int h = 0; h < nl; ++h) {
+            ContractArgs a{};
+            a.W = ctx->d_Rb;
+            a.cw = ctx->d_c;
+            a.cwf = ctx->d_cf + (size_t)h * nw * (ctx->Kp + kRowPad);
+            a.P = ctx->d_P + (size_t)h * nw * gp.n_gangs * ctx->Np * ctx->Np;
+            a.Kp = ctx->Kp;
+            a.Np = ctx->Np;
+            a.Kg = gp.Kg;
+            a.n_gangs = gp.n_gangs;
+            a.wrows = gp.wrows;
+            a.wcols = gp.wcols;
+            const int nfh = nl == 1 ? nf : std::min(nw, std::max(0, nf - h * nw));
+            const hipError_t e = contract_symbf16_launch(nw, nfh, mb, a, ctx->stream);
+            if (e != hipSuccess)
+                return fail(ctx, KLSP_ERR_HIP,
+                            std::string("contraction launch: ") + hipGetErrorString(e));
+        }
         ctx->last_S = gp.n_gangs;
         ctx->last_count = nullptr;
         ctx->last_sym = true;
@@ -2398,15 +2417,24 @@ static int check_ready(klsp_ctx* ctx, int precision, const char* who)
     return KLSP_OK;
 }
 
+// Omegas per contraction pass: FP64 is MFMA-bound at one omega per pass
+// (larger batches only lower the occupancy); the low-precision passes share
+// W loads among up to three, or two in the real basis in BF16x3 where passes
+// of two fill the block slots better (bf16_split).
+static int omega_batch(const klsp_ctx* ctx, int precision)
+{
+    if (precision == KLSP_PREC_FP64) return 1;
+    if (precision == KLSP_PREC_BF16X3 && ctx->use_sym && ctx->sym == 1 && bf16_split(ctx)) return 2;
+    return kMaxBatch;
+}
+
 // det(I - M(omega_i)) for n omegas; the workhorse of klsp_det/newton/scan.
 static int det_many(klsp_ctx* ctx, const double* om, int64_t n, int precision, double* out)
 {
     int rc = check_ready(ctx, precision, "klsp_det");
     if (rc) return rc;
     KLSP_HIP(ctx, hipSetDevice(ctx->device));
-    // FP64 is MFMA-bound at one omega per pass (larger batches only lower the
-    // occupancy); the low-precision passes share W loads among up to three.
-    const int maxb = precision == KLSP_PREC_FP64 ? 1 : kMaxBatch;
+    const int maxb = omega_batch(ctx, precision);
     const int slots = batch_slots(ctx, n, maxb);
     const size_t NN = (size_t)ctx->N * ctx->N;
     if ((rc = ensure(ctx, ctx->d_M, ctx->M_cap, (size_t)std::max(slots, kMaxW) * NN))) return rc;
@@ -2477,7 +2505,7 @@ static int scan_by_graph(klsp_ctx* ctx, const double* om, int64_t n, int precisi
     int rc = check_ready(ctx, precision, "klsp_grid_scan");
     if (rc) return rc;
     KLSP_HIP(ctx, hipSetDevice(ctx->device));
-    const int maxb = precision == KLSP_PREC_FP64 ? 1 : kMaxBatch;
+    const int maxb = omega_batch(ctx, precision);
     const int slots = batch_slots(ctx, n, maxb);
     const size_t NN = (size_t)ctx->N * ctx->N;
     if ((rc = ensure(ctx, ctx->d_M, ctx->M_cap, (size_t)std::max(slots, kMaxW) * NN))) return rc;
@@ -2770,7 +2798,9 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
     // relative error of the model's slope, which has two parts:
     //  - truncation of the pass's series: the first term left out enters the
     //    slope as nt s_a^(nt-1) T_{nt+1}, and |T_{nt+1}| <= |T_nt| / Im w_c
-    //    (every pole of the weights lies on the real axis);
+    //    (every pole of the weights lies on the real axis), or, from three
+    //    terms on, the geometric continuation |T_nt|^2 / |T_{nt-1}| where
+    //    that is smaller;
     //  - rounding of the low-precision terms, which moved the pass's zero by
     //    delta_b = |s - s_a| at the first anchor: the slope T_2 carries
     //    ~delta_b |T_2| / |T_1| of relative error.
@@ -2778,9 +2808,9 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
     //   |s - s_a| (2 trunc + 4 delta_b |T_2| / |T_1|) <= 5e-11,
     // half the 1e-10 the roots are held to; further out a new anchor is built
     // at the model's zero.  (Over the default grid and the 48 uneven draws
-    // the first anchor's zero lies at most 1.9x this estimate from the FP64
-    // root; default grid: truncation 3.9e-5, the BF16x3 zero 5.0e-7 off,
-    // anchor_tol 6.0e-7, one FP64 build.)  Without finite term
+    // the first anchor's zero lies at most 2.0x this estimate from the FP64
+    // root; default grid: truncation 2.2e-5, the BF16x3 zero 5.0e-7 off,
+    // anchor_tol 9.3e-7, one FP64 build.)  Without finite term
     // norms the bound falls back to nt r^(nt-1) + 1e-4 (BF16x3) or 1e-5 (FP32).
     const double eps_fallback = lp_prec == KLSP_PREC_BF16X3 ? 1e-4 : 1e-5;
     double delta_b = NAN;               // |s - s_a| at the first anchor
@@ -2803,8 +2833,15 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
         // first anchor delta_b is that same distance: a quadratic)
         double anchor_tol = NAN;
         if (nt_last >= 2 && tnorm[0] > 0.0 && tnorm[1] > 0.0 && std::isfinite(tnorm[nt_last - 1])) {
-            const double a = 2.0 * nt_last * std::pow(std::abs(sa), nt_last - 1) *
-                             (tnorm[nt_last - 1] / std::max(wc.imag(), 1e-300)) / tnorm[1];
+            // |T_{nt+1}| <= |T_nt| / Im w_c; from three terms on, also the
+            // geometric continuation of the last two (T_1 holds the
+            // non-resonant bulk, so T_2 / T_1 says nothing about the tail)
+            double t_next = tnorm[nt_last - 1] / std::max(wc.imag(), 1e-300);
+            if (nt_last >= 3 && tnorm[nt_last - 2] > 0.0)
+                t_next = std::min(t_next, tnorm[nt_last - 1] * tnorm[nt_last - 1] /
+                                              tnorm[nt_last - 2]);
+            const double a = 2.0 * nt_last * std::pow(std::abs(sa), nt_last - 1) * t_next /
+                             tnorm[1];
             const double b = 4.0 * tnorm[1] / tnorm[0];
             anchor_tol = anchor == 0 ? 2.0 * 5e-11 / (a + std::sqrt(a * a + 4.0 * b * 5e-11))
                                      : 5e-11 / (a + b * delta_b);
