@@ -384,6 +384,15 @@ int klsp_set_scan_graph(klsp_ctx* ctx, int on);
  * one-workgroup LU).  Both follow zgetrf's pivoting and give the same bits. */
 int klsp_set_lu_form(klsp_ctx* ctx, int form);
 
+/* Which work decomposition the BF16x3 contraction on a symmetric alpha grid
+ * uses (no counterpart in the reference; for tests and measurement).
+ * 0 (default): chosen per problem from how well each fills the upper
+ * triangle's blocks.  1: folded column-block pairs, one block per slot.
+ * 2: folded two-column strips, both blocks of a strip row per slot (half the
+ * operand loads; launches of at most two weight sets).  For the same gang
+ * plan both give the same bits. */
+int klsp_set_bf16_form(klsp_ctx* ctx, int form);
+
 /* Device memory the context holds between calls: W and its copies, the
  * problem arrays and the grow-only workspaces, in bytes (no counterpart in
  * the reference).  Orbit samples and quadrature temporaries of

@@ -47,7 +47,8 @@ EXPORTED = [
     "klsp_compute_W", "klsp_upload_W", "klsp_download_W", "klsp_prepare", "klsp_build_M",
     "klsp_build_taylor", "klsp_build_pencil", "klsp_det", "klsp_det_noalloc", "klsp_newton", "klsp_grid_scan",
     "klsp_set_symmetric_basis", "klsp_symmetric_basis", "klsp_autotune", "klsp_variant_name",
-    "klsp_set_variant", "klsp_set_scan_graph", "klsp_set_lu_form", "klsp_scan_graph_stats", "klsp_device_bytes",
+    "klsp_set_variant", "klsp_set_scan_graph", "klsp_set_lu_form", "klsp_set_bf16_form",
+    "klsp_scan_graph_stats", "klsp_device_bytes",
     "klsp_set_profiling",
     "klsp_last_timing", "klsp_last_W_ms", "klsp_last_orbit_ms",
 ]
@@ -162,6 +163,7 @@ def lib() -> ct.CDLL:
     L.klsp_set_symmetric_basis.argtypes = [P, ct.c_int]
     L.klsp_set_scan_graph.argtypes = [P, ct.c_int]
     L.klsp_set_lu_form.argtypes = [P, ct.c_int]
+    L.klsp_set_bf16_form.argtypes = [P, ct.c_int]
     L.klsp_device_bytes.argtypes = [P, ct.POINTER(ct.c_int64)]
     L.klsp_scan_graph_stats.argtypes = [P, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)]
     L.klsp_symmetric_basis.argtypes = [P, ct.POINTER(ct.c_int)]
@@ -394,6 +396,10 @@ class Context:
     def set_lu_form(self, form: int):
         """klsp_set_lu_form: 0 automatic, 1 one workgroup per matrix, 2 step-wise."""
         self._chk(lib().klsp_set_lu_form(self.handle, int(form)))
+
+    def set_bf16_form(self, form: int):
+        """klsp_set_bf16_form: 0 automatic, 1 folded column pairs, 2 folded strips."""
+        self._chk(lib().klsp_set_bf16_form(self.handle, int(form)))
 
     def set_scan_graph(self, on: bool):
         """grid_scan as one HIP graph launch (default on) or kernel by kernel."""

@@ -426,17 +426,21 @@ hipError_t split_W_bf16(const double2* W64, int64_t rows, int Np, int64_t nq, vo
                         hipStream_t stream);
 
 // BF16 three-term contraction in the real basis of a symmetric alpha grid
-// (klsp_contract_symbf16.hip): folded column-block pairs of the upper
-// triangle (wcols pairs x wrows workgroups each, mb block slots per wave),
-// partial slice P + iQ per gang and weight set; K ranges per gang are
-// multiples of 32 rows.  The real basis is held as bf16 hi and lo planes of
-// symbf16_records(Kp) records of eight K rows (2 x records x Np x 16 bytes),
-// made from W by split_R_bf16.
-Bf16Geometry contract_symbf16_geometry(int Np, int nw, int nf);
+// (klsp_contract_symbf16.hip), in one of two work decompositions of the upper
+// triangle: kBf16Fold (folded column-block pairs, up to four weight sets a
+// launch) or kBf16Strip (folded pairs of two-column strips, up to two); both
+// give the same bits for the same gang plan.  wcols column groups x wrows
+// workgroups each, partial slice P + iQ per gang and weight set; K ranges
+// per gang are multiples of 64 rows.  The real basis is held as bf16 hi and
+// lo planes of symbf16_records(Kp) records of eight K rows
+// (2 x records x Np x 16 bytes), made from W by split_R_bf16.
+constexpr int kBf16Fold = 1, kBf16Strip = 2;
+Bf16Geometry contract_symbf16_geometry(int form, int Np, int nw, int nf);
+int contract_symbf16_max_sets(int form);
 // Share of a launch's block slots (nw weight sets) holding a block of the
-// upper triangle: the slots past a pair's blocks idle through the launch.
-double contract_symbf16_slot_use(int Np, int nw);
-hipError_t contract_symbf16_launch(int nw, int nf, int mb, const ContractArgs& a,
+// upper triangle (the rest idle through the launch).
+double contract_symbf16_slot_use(int form, int Np, int nw);
+hipError_t contract_symbf16_launch(int form, int nw, int nf, int mb, const ContractArgs& a,
                                    hipStream_t stream);
 int64_t symbf16_records(int64_t Kp);
 hipError_t split_R_bf16(const double2* W64, int64_t rows, int N, int Np, int64_t nq, void* Rb,

@@ -21,16 +21,28 @@
 // T_4, as in the complex BF16 kernel).  Accumulation is FP32 over a gang's K
 // range; gang partials are summed in FP64 by the reduce.
 //
-// Work decomposition.  The upper triangle of nb x nb blocks is folded: column
-// blocks c and nb - 1 - c form a pair holding (c + 1) + (nb - c) = nb + 1
-// blocks (the middle column of an odd nb alone holds (nb + 1) / 2), so equal
-// slot counts cover it with little idle work.  A workgroup is 4 waves of MB
-// block slots each over one pair (a pair with more than 4 MB blocks takes
-// several workgroups); per stage of 32 K rows its 256 threads form the B
-// operands of the pair's two column blocks for NW weight sets once and stage
-// them in LDS in operand order; each wave reads its A operands (the rows of
-// its slots) from global memory one stage ahead.  Gangs (the workgroups
-// covering every pair) walk contiguous K ranges on one XCD.
+// Two work decompositions of the upper triangle of nb x nb blocks; both
+// stage the B operands of a workgroup's column blocks in LDS (formed once a
+// stage by all its threads, in operand order), read A (the rows of a wave's
+// slots) from global memory one stage ahead (two for one weight set), and
+// walk a gang's contiguous K range on one XCD.  For one block and stage both
+// issue the same MFMAs in the same order, so with the same gang plan they
+// give the same bits.
+//
+// Fold form.  Column blocks c and nb - 1 - c form a pair holding
+// (c + 1) + (nb - c) = nb + 1 blocks (the middle column of an odd nb alone
+// holds (nb + 1) / 2), so equal slot counts cover the triangle with little
+// idle work.  A workgroup is 4 waves of MB block slots (one block each) over
+// one pair; a pass of up to four weight sets is one launch.
+//
+// Strip form.  Strip j is column blocks 2j and 2j + 1 (one block for the last
+// strip of an odd nb) with its rows 0 .. min(2j + 1, nb - 1); a row slot is
+// one row of a strip, both its blocks, so one A load feeds the MFMAs of two
+// column blocks and a wave reads its strip's B operands from LDS once a
+// stage.  Strips j and ns - 1 - j fold into one workgroup of 8 waves of kMR
+// row slots, each wave in one strip.  Half the A loads and less than half the
+// LDS reads of the fold form, but its slots fill the triangle well only for
+// some nb (79% at nb = 19); a launch takes at most two weight sets.
 #include <algorithm>
 
 #include "klsp_contract.hpp"
@@ -44,12 +56,7 @@ typedef float v2f __attribute__((ext_vector_type(2)));
 typedef unsigned v4u __attribute__((ext_vector_type(4)));
 typedef unsigned v2u __attribute__((ext_vector_type(2)));
 
-constexpr int kWaves = 4;
-constexpr int kThreads = kWaves * 64;
-constexpr int kCols = 32;          // B-side columns staged per workgroup (two blocks)
-constexpr int kMaxMB = 5;          // block slots per wave
-constexpr int kMaxNW = kMaxW;
-constexpr int kKq = 64;          // K rows per pair of stages (a gang's range is a multiple)
+constexpr int kKq = 64;            // K rows per pair of stages (a gang's range is a multiple)
 
 __device__ __forceinline__ unsigned pk_bf16(float a, float b)
 {
@@ -66,30 +73,14 @@ __device__ __forceinline__ v4f mfma(v4u a, v4u b, v4f c)
 }
 
 // B operands of one stage in operand order: rec[w][part][g][col], part =
-// P hi, P lo, Q hi, Q lo; K rows 8g..8g+7 of column col.
-template <int NW>
+// P hi, P lo, Q hi, Q lo; K rows 8g..8g+7 of staged column col (C columns).
+template <int NW, int C>
 struct StageB {
-    v4u rec[NW][4][4][kCols];
+    v4u rec[NW][4][4][C];
 };
 
-// The pair of column blocks of workgroup slice `pair` and its slot list:
-// slot s < na is block (s, ca), then (s - na, cb) for the second column.
-struct Pair {
-    int ca, cb, na, nslots;
-};
-
-__device__ __forceinline__ Pair pair_of(int p, int nb)
-{
-    Pair q;
-    q.ca = p;
-    q.cb = nb - 1 - p;
-    q.na = p + 1;
-    q.nslots = q.ca == q.cb ? q.na : nb + 1;
-    return q;
-}
-
-// B inputs of one stage for this thread: rows 4 rg .. 4 rg + 3 (rg = tid / 32)
-// of staged column tid % 32, hi and lo (4 bf16 each), and their weights.
+// B inputs of one stage for this thread: rows 4 rg .. 4 rg + 3 of its staged
+// column, hi and lo (4 bf16 each), and their weights.
 template <int NW>
 struct BIn {
     v2u h, l;
@@ -109,11 +100,10 @@ __device__ __forceinline__ v4u ld16(const char* base, unsigned off)
 template <int NW>
 __device__ __forceinline__ void load_B_in(const char* hb, int64_t plane, const char* wb,
                                           int64_t wstride, unsigned offB, unsigned offW,
-                                          bool cvalid, BIn<NW>& o)
+                                          BIn<NW>& o)
 {
-    const v2u h = *(const v2u*)(hb + offB), l = *(const v2u*)(hb + plane + offB);
-    o.h = cvalid ? h : (v2u){0, 0};
-    o.l = cvalid ? l : (v2u){0, 0};
+    o.h = *(const v2u*)(hb + offB);
+    o.l = *(const v2u*)(hb + plane + offB);
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
         const v4f c01 = *(const v4f*)(wb + w * wstride + offW);
@@ -125,8 +115,8 @@ __device__ __forceinline__ void load_B_in(const char* hb, int64_t plane, const c
     }
 }
 
-template <int NW, int NF>
-__device__ __forceinline__ void produce_B(StageB<NW>& sb, const BIn<NW>& in, int rg, int col)
+template <int NW, int NF, int C>
+__device__ __forceinline__ void produce_B(StageB<NW, C>& sb, const BIn<NW>& in, int rg, int col)
 {
     float x[4];
     x[0] = lo_f(in.h.x) + lo_f(in.l.x);
@@ -169,7 +159,8 @@ __device__ __forceinline__ void lds_barrier()
 }
 
 // A operands of a stage: slot s's record at hb + offA[s] (hi) and the same
-// offset in the lo plane.  Idle slots read block 0 (their MFMAs are skipped).
+// offset in the lo plane.  Idle slots read a valid block (their MFMAs are
+// skipped).
 template <int MB>
 __device__ __forceinline__ void load_A(const char* hb, int64_t plane, const unsigned (&offA)[MB],
                                        v4u (&Ah)[MB], v4u (&Al)[MB])
@@ -181,13 +172,102 @@ __device__ __forceinline__ void load_A(const char* hb, int64_t plane, const unsi
     }
 }
 
-// The MFMAs of a stage.  With `next` set, each slot's A operands are
-// reloaded from the stage two ahead (`hn`) as soon as the slot's last MFMA
-// has issued, so those loads have about two stages to arrive.
-template <int MB, int NW, int NF>
-__device__ __forceinline__ void mfma_stage(const StageB<NW>& sb, v4u (&Ah)[MB], v4u (&Al)[MB],
-                                           const int (&csel)[MB], v4f (&P)[NW][MB],
-                                           v4f (&Q)[NW][MB], int g, int i, bool next,
+// Per-gang K range and plane addressing shared by both forms.
+struct GangRange {
+    int64_t Kb, Ke, plane, stage, wstride;
+    const char* hb;
+    const char* wb;
+};
+
+__device__ __forceinline__ GangRange gang_range(const ContractArgs& a, int gang)
+{
+    GangRange r;
+    const int Np = a.Np;
+    // a gang's K range is a whole number of stage pairs (kKq rows)
+    const int64_t Kq = (a.Kp + kKq - 1) / kKq * kKq;
+    r.Kb = min((int64_t)gang * a.Kg, Kq);
+    r.Ke = min(Kq, r.Kb + a.Kg);
+    r.plane = Kq / 8 * Np * 16;                 // hi plane -> lo plane, bytes
+    r.stage = (int64_t)Np * 64;                 // 4 records of Np columns
+    r.wstride = (a.Kp + kRowPad) * (int64_t)sizeof(float2);
+    r.hb = (const char*)a.W + (r.Kb >> 3) * (int64_t)Np * 16;
+    r.wb = (const char*)(a.cwf + r.Kb);
+    return r;
+}
+
+// The K loop of both forms: two stages a trip on two register sets; a
+// stage's B inputs are loaded one stage ahead, its A operands one stage ahead
+// or (deep: one weight set, whose stage is too short to cover a load issued
+// one stage ahead) two stages ahead; the trip has no exit between its stages
+// (the range holds an even number of stages).  On the last trip the loads of
+// the next trip still issue, from this trip's rows (never used), so that
+// every trip issues the same loads in the same order and the compiler's
+// waits count only the loads issued after the one waited for.  `stage_fn`
+// runs a stage's MFMAs: (B buffer, A hi, A lo, address of the stage two
+// ahead for the deep reload).
+template <int NW, int NF, int C, int MB, bool deep, typename StageFn>
+__device__ __forceinline__ void k_loop(const GangRange& r, StageB<NW, C> (&sb)[2],
+                                       const unsigned (&offA)[MB], unsigned offB, unsigned offW,
+                                       int rg, int col, StageFn stage_fn)
+{
+    if (r.Kb >= r.Ke) return;
+    v4u A0h[MB], A0l[MB], A1h[MB], A1l[MB];
+    BIn<NW> b0, b1;
+    load_A<MB>(r.hb, r.plane, offA, A0h, A0l);
+    if (deep) load_A<MB>(r.hb + r.stage, r.plane, offA, A1h, A1l);
+    load_B_in<NW>(r.hb, r.plane, r.wb, r.wstride, offB, offW, b0);
+    for (int64_t K0 = r.Kb; K0 < r.Ke; K0 += kKq) {
+        const char* h0 = r.hb + ((K0 - r.Kb) >> 5) * r.stage;
+        const char* w0 = r.wb + (K0 - r.Kb) * (int64_t)sizeof(float2);
+        const bool more = K0 + kKq < r.Ke;
+        const int64_t ahead = more ? 2 * r.stage : 0;
+        const char* w2 = w0 + (more ? 64 * sizeof(float2) : 0);
+        produce_B<NW, NF, C>(sb[0], b0, rg, col);
+        load_B_in<NW>(h0 + r.stage, r.plane, w0 + 32 * sizeof(float2), r.wstride, offB, offW, b1);
+        if (!deep) load_A<MB>(h0 + r.stage, r.plane, offA, A1h, A1l);
+        lds_barrier();
+        stage_fn(sb[0], A0h, A0l, h0 + ahead);
+        produce_B<NW, NF, C>(sb[1], b1, rg, col);
+        load_B_in<NW>(h0 + ahead, r.plane, w2, r.wstride, offB, offW, b0);
+        if (!deep) load_A<MB>(h0 + ahead, r.plane, offA, A0h, A0l);
+        lds_barrier();
+        stage_fn(sb[1], A1h, A1l, h0 + ahead + r.stage);
+    }
+}
+
+// ---------------------------------------------------------------- fold form
+
+namespace fold {
+
+constexpr int kWaves = 4;
+constexpr int kThreads = kWaves * 64;
+constexpr int kCols = 32;          // B-side columns staged per workgroup (two blocks)
+constexpr int kMaxMB = 5;          // block slots per wave
+constexpr int kMaxNW = kMaxW;
+
+// The pair of column blocks of workgroup slice `pair` and its slot list:
+// slot s < na is block (s, ca), then (s - na, cb) for the second column.
+struct Pair {
+    int ca, cb, na, nslots;
+};
+
+__device__ __forceinline__ Pair pair_of(int p, int nb)
+{
+    Pair q;
+    q.ca = p;
+    q.cb = nb - 1 - p;
+    q.na = p + 1;
+    q.nslots = q.ca == q.cb ? q.na : nb + 1;
+    return q;
+}
+
+// The MFMAs of a stage.  With Next, each slot's A operands are reloaded
+// from the stage two ahead (`hn`) as soon as the slot's last MFMA has
+// issued, so those loads have about two stages to arrive.
+template <int MB, int NW, int NF, bool Next>
+__device__ __forceinline__ void mfma_stage(const StageB<NW, kCols>& sb, v4u (&Ah)[MB],
+                                           v4u (&Al)[MB], const int (&csel)[MB],
+                                           v4f (&P)[NW][MB], v4f (&Q)[NW][MB], int g, int i,
                                            const char* hn, int64_t plane,
                                            const unsigned (&offA)[MB])
 {
@@ -210,7 +290,7 @@ __device__ __forceinline__ void mfma_stage(const StageB<NW>& sb, v4u (&Ah)[MB], 
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
-        if (next) {
+        if constexpr (Next) {
             Ah[s] = ld16(hn, offA[s]);
             Al[s] = ld16(hn + plane, offA[s]);
         }
@@ -218,9 +298,9 @@ __device__ __forceinline__ void mfma_stage(const StageB<NW>& sb, v4u (&Ah)[MB], 
 }
 
 template <int MB, int NW, int NF>
-__global__ __launch_bounds__(kThreads) void k_contract_symbf16(ContractArgs a)
+__global__ __launch_bounds__(kThreads) void k_contract_symbf16_fold(ContractArgs a)
 {
-    __shared__ StageB<NW> sb[2];
+    __shared__ StageB<NW, kCols> sb[2];
     const int L = blockIdx.x;
     const int xcd = L & 7, j = L >> 3;
     const int gw = a.wrows * a.wcols;         // workgroups per gang: wcols pairs x wrows
@@ -239,20 +319,12 @@ __global__ __launch_bounds__(kThreads) void k_contract_symbf16(ContractArgs a)
         else if (sl < pr.nslots) { row0[s] = 16 * (sl - pr.na); csel[s] = 1; }
         else { row0[s] = -1; csel[s] = -1; }
     }
-    // a gang's K range is a whole number of stage pairs (kKq rows)
-    const int64_t Kq = (a.Kp + kKq - 1) / kKq * kKq;
-    const int64_t Kb = min((int64_t)gang * a.Kg, Kq);
-    const int64_t Ke = min(Kq, Kb + a.Kg);
-    const int64_t nq = Kq / 8;                          // records per plane column
-    const int64_t plane = nq * Np * 16;                 // hi plane -> lo plane, bytes
-    const int64_t stage = (int64_t)Np * 64;             // 4 records of Np columns
-    const int64_t wstride = (a.Kp + kRowPad) * (int64_t)sizeof(float2);
-    const char* hb = (const char*)a.W + (Kb >> 3) * (int64_t)Np * 16;
-    const char* wb = (const char*)(a.cwf + Kb);
+    const GangRange r = gang_range(a, gang);
     // B-producing role: staged column col (two blocks: ca, cb), rows 4 rg..
+    // (the middle column of an odd nb stages its block twice; no slot reads
+    // the second copy)
     const int col = tid % kCols, rg = tid / kCols;
     const int col_g = (col < 16 ? 16 * pr.ca : 16 * pr.cb) + (col & 15);
-    const bool cvalid = col < 16 || pr.cb != pr.ca;
     unsigned offA[MB];
 #pragma unroll
     for (int s = 0; s < MB; ++s) offA[s] = (unsigned)((g * Np + max(row0[s], 0) + i) * 16);
@@ -266,44 +338,16 @@ __global__ __launch_bounds__(kThreads) void k_contract_symbf16(ContractArgs a)
             P[w][s] = (v4f){0, 0, 0, 0};
             Q[w][s] = (v4f){0, 0, 0, 0};
         }
-    // two stages a trip on two register sets; a stage's B inputs are loaded
-    // one stage ahead, its A operands one stage ahead or (deep) two stages
-    // ahead, slot by slot as each slot of the stage two back finishes with
-    // them; the trip has no exit between its stages (the range holds an even
-    // number of stages)
-    v4u A0h[MB], A0l[MB], A1h[MB], A1l[MB];
-    BIn<NW> b0, b1;
     // One weight set: a stage is 6 MB MFMAs, too few to cover a load issued
     // one stage ahead, so A is loaded two stages ahead (measured 5.06 -> 4.66
     // ms at the default grid).  Three or four weight sets give 18-24 MFMAs a
     // slot; there the deeper prefetch measured 2-5% slower.
     constexpr bool deep = NW == 1;
-    if (Kb < Ke) {
-        load_A<MB>(hb, plane, offA, A0h, A0l);
-        if (deep) load_A<MB>(hb + stage, plane, offA, A1h, A1l);
-        load_B_in<NW>(hb, plane, wb, wstride, offB, offW, cvalid, b0);
-    }
-    for (int64_t K0 = Kb; K0 < Ke; K0 += kKq) {
-        const char* h0 = hb + ((K0 - Kb) >> 5) * stage;
-        const char* w0 = wb + (K0 - Kb) * (int64_t)sizeof(float2);
-        const bool more = K0 + kKq < Ke;
-        produce_B<NW, NF>(sb[0], b0, rg, col);
-        load_B_in<NW>(h0 + stage, plane, w0 + 32 * sizeof(float2), wstride, offB, offW, cvalid,
-                      b1);
-        if (!deep) load_A<MB>(h0 + stage, plane, offA, A1h, A1l);
-        lds_barrier();
-        mfma_stage<MB, NW, NF>(sb[0], A0h, A0l, csel, P, Q, g, i, deep && more, h0 + 2 * stage,
-                               plane, offA);
-        produce_B<NW, NF>(sb[1], b1, rg, col);
-        if (more) {
-            load_B_in<NW>(h0 + 2 * stage, plane, w0 + 64 * sizeof(float2), wstride, offB, offW,
-                          cvalid, b0);
-            if (!deep) load_A<MB>(h0 + 2 * stage, plane, offA, A0h, A0l);
-        }
-        lds_barrier();
-        mfma_stage<MB, NW, NF>(sb[1], A1h, A1l, csel, P, Q, g, i, deep && more, h0 + 3 * stage,
-                               plane, offA);
-    }
+    k_loop<NW, NF, kCols, MB, deep>(
+        r, sb, offA, offB, offW, rg, col,
+        [&](const StageB<NW, kCols>& s, v4u(&Ah)[MB], v4u(&Al)[MB], const char* hn) {
+            mfma_stage<MB, NW, NF, deep>(s, Ah, Al, csel, P, Q, g, i, hn, r.plane, offA);
+        });
     // FP32 accumulator lanes hold rows 4g + v of a block (column i); partial
     // slices are column-major: column = B-side real column
 #pragma unroll
@@ -320,6 +364,223 @@ __global__ __launch_bounds__(kThreads) void k_contract_symbf16(ContractArgs a)
         }
     }
 }
+
+template <int NW, int NF>
+const void* kernel_nw(int mb)
+{
+    switch (mb) {
+    case 1: return (const void*)k_contract_symbf16_fold<1, NW, NF>;
+    case 2: return (const void*)k_contract_symbf16_fold<2, NW, NF>;
+    case 3: return (const void*)k_contract_symbf16_fold<3, NW, NF>;
+    case 4: return (const void*)k_contract_symbf16_fold<4, NW, NF>;
+    default: return (const void*)k_contract_symbf16_fold<5, NW, NF>;
+    }
+}
+
+// nf < nw only for the mixed search's Taylor terms: four with three full,
+// or (split into two launches) two with one full
+const void* kernel_for(int nw, int nf, int mb)
+{
+    if (nw >= 4 && nf < 4) return kernel_nw<4, 3>(mb);
+    if (nw == 2 && nf < 2) return kernel_nw<2, 1>(mb);
+    return nw <= 1 ? kernel_nw<1, 1>(mb) : nw == 2 ? kernel_nw<2, 2>(mb)
+         : nw == 3 ? kernel_nw<3, 3>(mb) : kernel_nw<4, 4>(mb);
+}
+
+// Slots per wave and workgroups per column pair: the fewest workgroups per
+// pair with at most max_mb slots per wave, then the fewest slots that hold
+// the pair's nb + 1 blocks.  max_mb keeps the accumulators and the prefetched
+// A operands within 256 VGPRs (two workgroups per CU): 5 slots for one or two
+// weight sets, 4 for three, 3 for four.
+void layout(int nb, int nw, int* wrows, int* wcols, int* mb)
+{
+    const int need = nb + 1;                  // blocks of a full pair
+    const int max_mb = nw <= 2 ? kMaxMB : nw == 3 ? 4 : 3;
+    const int per_pair = (need + kWaves * max_mb - 1) / (kWaves * max_mb);
+    *mb = (need + kWaves * per_pair - 1) / (kWaves * per_pair);
+    *wrows = per_pair;                        // workgroups per pair
+    *wcols = (nb + 1) / 2;                    // pairs (the middle column of an odd nb alone)
+}
+
+}  // namespace fold
+
+// --------------------------------------------------------------- strip form
+
+namespace strip {
+
+constexpr int kWaves = 8;
+constexpr int kThreads = kWaves * 64;
+constexpr int kCols = 64;          // B-side columns staged per workgroup (two strips)
+constexpr int kMR = 3;             // row slots per wave
+constexpr int kMaxNW = 2;          // weight sets per launch
+
+// Strip j: column blocks c0 = 2j and, when it exists, c0 + 1; rows
+// 0 .. nrows - 1.  Row r holds block (r, c0) when r <= c0 and (r, c0 + 1)
+// when `two`.
+struct Strip {
+    int c0, nrows;
+    bool two;
+};
+
+__host__ __device__ inline Strip strip_of(int j, int nb)
+{
+    Strip s;
+    s.c0 = 2 * j;
+    s.two = 2 * j + 1 < nb;
+    s.nrows = 2 * j + 2 < nb ? 2 * j + 2 : nb;
+    return s;
+}
+
+__host__ __device__ inline int strip_waves(int j, int nb)
+{
+    return (strip_of(j, nb).nrows + kMR - 1) / kMR;
+}
+
+// The MFMAs of a stage: per weight set the strip's B operands (both column
+// blocks) from LDS once, then every slot's blocks (idle blocks skip their
+// MFMAs).  With Next, the A operands are reloaded from the stage two ahead
+// (`hn`) after the stage's MFMAs have issued.
+template <int NW, int NF, bool Next>
+__device__ __forceinline__ void mfma_stage(const StageB<NW, kCols>& sb, v4u (&Ah)[kMR],
+                                           v4u (&Al)[kMR], int colb, const bool (&on0)[kMR],
+                                           const bool (&on1)[kMR], v4f (&P)[NW][kMR][2],
+                                           v4f (&Q)[NW][kMR][2], int g, const char* hn,
+                                           int64_t plane, const unsigned (&offA)[kMR])
+{
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        v4u ph[2], qh[2], pl[2], ql[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            ph[c] = sb.rec[w][0][g][colb + 16 * c];
+            qh[c] = sb.rec[w][2][g][colb + 16 * c];
+            if (w < NF) {
+                pl[c] = sb.rec[w][1][g][colb + 16 * c];
+                ql[c] = sb.rec[w][3][g][colb + 16 * c];
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < kMR; ++s)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                if (c == 0 ? on0[s] : on1[s]) {
+                    P[w][s][c] = mfma(Ah[s], ph[c], P[w][s][c]);
+                    Q[w][s][c] = mfma(Ah[s], qh[c], Q[w][s][c]);
+                    if (w < NF) {
+                        P[w][s][c] = mfma(Al[s], ph[c], P[w][s][c]);
+                        Q[w][s][c] = mfma(Al[s], qh[c], Q[w][s][c]);
+                        P[w][s][c] = mfma(Ah[s], pl[c], P[w][s][c]);
+                        Q[w][s][c] = mfma(Ah[s], ql[c], Q[w][s][c]);
+                    }
+                }
+            }
+    }
+    if constexpr (Next) load_A<kMR>(hn, plane, offA, Ah, Al);
+}
+
+template <int NW, int NF>
+__global__ __launch_bounds__(kThreads) void k_contract_symbf16_strip(ContractArgs a)
+{
+    __shared__ StageB<NW, kCols> sb[2];
+    const int L = blockIdx.x;
+    const int xcd = L & 7, j = L >> 3;
+    const int gw = a.wrows * a.wcols;         // workgroups per gang: wcols strip pairs x wrows
+    const int gang = xcd + 8 * (j / gw), widx = j % gw;
+    if (gang >= a.n_gangs) return;            // whole workgroup
+    const int Np = a.Np, nb = Np / 16, ns = (nb + 1) / 2;
+    const int pr = widx / a.wrows, sub = widx % a.wrows;
+    const int ja = pr, jb = ns - 1 - pr;      // the folded strips (one when ja == jb)
+    const Strip SA = strip_of(ja, nb), SB = strip_of(jb, nb);
+    const int wva = strip_waves(ja, nb), wvb = jb != ja ? strip_waves(jb, nb) : 0;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, i = lane & 15, g = lane >> 4;
+    // this wave's strip (sel: 0 = first, 1 = second, -1 idle) and row slots
+    const int gv = sub * kWaves + wave;
+    const int sel = gv < wva ? 0 : gv < wva + wvb ? 1 : -1;
+    const Strip S = sel == 1 ? SB : SA;
+    const int rbase = (sel == 1 ? gv - wva : gv) * kMR;
+    int row[kMR];
+    bool on0[kMR], on1[kMR];
+#pragma unroll
+    for (int s = 0; s < kMR; ++s) {
+        row[s] = rbase + s;
+        const bool v = sel >= 0 && row[s] < S.nrows;
+        on0[s] = v && row[s] <= S.c0;
+        on1[s] = v && S.two;
+    }
+    const int colb = 32 * max(sel, 0) + i;    // staged column of the strip's first block
+    const GangRange r = gang_range(a, gang);
+    // B-producing role: staged column col (first strip's blocks below 32,
+    // the second's from 32), rows 4 rg .. 4 rg + 3.  A strip of one block
+    // stages it twice and the middle strip of an odd strip count fills both
+    // halves; no MFMA reads those copies.
+    const int col = tid % kCols, rg = tid / kCols;
+    const Strip SC = col < 32 ? SA : SB;
+    const int col_g = 16 * SC.c0 + (SC.two ? (col & 31) : (col & 15));
+    unsigned offA[kMR];
+#pragma unroll
+    for (int s = 0; s < kMR; ++s)
+        offA[s] = (unsigned)((g * Np + 16 * min(row[s], nb - 1) + i) * 16);
+    const unsigned offB = (unsigned)(((rg >> 1) * Np + col_g) * 16 + 8 * (rg & 1));
+    const unsigned offW = (unsigned)(4 * rg * (int)sizeof(float2));
+    v4f P[NW][kMR][2], Q[NW][kMR][2];
+#pragma unroll
+    for (int w = 0; w < NW; ++w)
+#pragma unroll
+        for (int s = 0; s < kMR; ++s)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                P[w][s][c] = (v4f){0, 0, 0, 0};
+                Q[w][s][c] = (v4f){0, 0, 0, 0};
+            }
+    constexpr bool deep = NW == 1;
+    k_loop<NW, NF, kCols, kMR, deep>(
+        r, sb, offA, offB, offW, rg, col,
+        [&](const StageB<NW, kCols>& s, v4u(&Ah)[kMR], v4u(&Al)[kMR], const char* hn) {
+            mfma_stage<NW, NF, deep>(s, Ah, Al, colb, on0, on1, P, Q, g, hn, r.plane, offA);
+        });
+    // FP32 accumulator lanes hold rows 4g + v of a block (column i); partial
+    // slices are column-major: column = B-side real column
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        double2* Pw = a.P + ((int64_t)w * a.n_gangs + gang) * (int64_t)Np * Np;
+#pragma unroll
+        for (int s = 0; s < kMR; ++s)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                if (!(c == 0 ? on0[s] : on1[s])) continue;
+                double2* dst = Pw + (int64_t)(16 * (S.c0 + c) + i) * Np + 16 * row[s] + 4 * g;
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    dst[v] = make_double2((double)P[w][s][c][v], (double)Q[w][s][c][v]);
+            }
+    }
+}
+
+const void* kernel_for(int nw, int nf)
+{
+    if (nw <= 1)
+        return nf < 1 ? (const void*)k_contract_symbf16_strip<1, 0>
+                      : (const void*)k_contract_symbf16_strip<1, 1>;
+    return nf < 1   ? (const void*)k_contract_symbf16_strip<2, 0>
+           : nf < 2 ? (const void*)k_contract_symbf16_strip<2, 1>
+                    : (const void*)k_contract_symbf16_strip<2, 2>;
+}
+
+// Strip pairs and workgroups per pair: the most waves any pair's two strips
+// need, in workgroups of kWaves.
+void layout(int nb, int* wrows, int* wcols)
+{
+    const int ns = (nb + 1) / 2;
+    int most = 1;
+    for (int p = 0; p < (ns + 1) / 2; ++p) {
+        const int jb = ns - 1 - p;
+        most = std::max(most, strip_waves(p, nb) + (jb != p ? strip_waves(jb, nb) : 0));
+    }
+    *wrows = (most + kWaves - 1) / kWaves;
+    *wcols = (ns + 1) / 2;
+}
+
+}  // namespace strip
 
 // W (complex128 rows of K, `rows` rows present) -> the real basis's bf16 hi
 // and lo planes of nq records of eight K rows (columns j < nU from Re W[h+j],
@@ -352,76 +613,69 @@ __global__ void k_W64_to_Rbf16(const double2* __restrict__ W, int64_t rows, int 
     }
 }
 
-template <int NW, int NF>
-const void* kernel_nw(int mb)
+int occupancy(const void* k, int threads)
 {
-    switch (mb) {
-    case 1: return (const void*)k_contract_symbf16<1, NW, NF>;
-    case 2: return (const void*)k_contract_symbf16<2, NW, NF>;
-    case 3: return (const void*)k_contract_symbf16<3, NW, NF>;
-    case 4: return (const void*)k_contract_symbf16<4, NW, NF>;
-    default: return (const void*)k_contract_symbf16<5, NW, NF>;
-    }
-}
-
-// nf < nw only for the mixed search's Taylor terms: four with three full,
-// or (split into two launches) two with one full
-const void* kernel_for(int nw, int nf, int mb)
-{
-    if (nw >= 4 && nf < 4) return kernel_nw<4, 3>(mb);
-    if (nw == 2 && nf < 2) return kernel_nw<2, 1>(mb);
-    return nw <= 1 ? kernel_nw<1, 1>(mb) : nw == 2 ? kernel_nw<2, 2>(mb)
-         : nw == 3 ? kernel_nw<3, 3>(mb) : kernel_nw<4, 4>(mb);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, threads, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    return per_cu;
 }
 
 }  // namespace
 
-// Slots per wave and workgroups per column pair: the fewest workgroups per
-// pair with at most max_mb slots per wave, then the fewest slots that hold
-// the pair's nb + 1 blocks.  max_mb keeps the accumulators and the prefetched
-// A operands within 256 VGPRs (two workgroups per CU): 5 slots for one or two
-// weight sets, 4 for three, 3 for four.
-Bf16Geometry contract_symbf16_geometry(int Np, int nw, int nf)
+int contract_symbf16_max_sets(int form)
+{
+    return form == kBf16Strip ? strip::kMaxNW : fold::kMaxNW;
+}
+
+Bf16Geometry contract_symbf16_geometry(int form, int Np, int nw, int nf)
 {
     const int nb = Np / 16;
-    const int need = nb + 1;                  // blocks of a full pair
-    const int max_mb = nw <= 2 ? kMaxMB : nw == 3 ? 4 : 3;
-    const int per_pair = (need + kWaves * max_mb - 1) / (kWaves * max_mb);
-    const int mb = (need + kWaves * per_pair - 1) / (kWaves * per_pair);
     Bf16Geometry g;
-    g.wrows = per_pair;                       // workgroups per pair
-    g.wcols = (nb + 1) / 2;                   // pairs (the middle column of an odd nb alone)
-    g.mb = mb;
-    static std::atomic<int> occ[kMaxNW + 1][2][kMaxMB + 1] = {};
-    const int i = nw < 1 ? 1 : (nw > kMaxNW ? kMaxNW : nw);
-    const int hh = nf < i ? 1 : 0;
-    if (occ[i][hh][mb] == 0) {
-        int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_for(i, nf, mb), kThreads,
-                                                         0) != hipSuccess ||
-            per_cu < 1)
-            per_cu = 1;
-        occ[i][hh][mb] = per_cu;
+    if (form == kBf16Strip) {
+        strip::layout(nb, &g.wrows, &g.wcols);
+        g.mb = strip::kMR;
+        const int w = nw < 1 ? 1 : (nw > strip::kMaxNW ? strip::kMaxNW : nw);
+        const int f = nf < 0 ? 0 : (nf > w ? w : nf);
+        static std::atomic<int> occ[strip::kMaxNW + 1][strip::kMaxNW + 1] = {};
+        if (occ[w][f] == 0) occ[w][f] = occupancy(strip::kernel_for(w, f), strip::kThreads);
+        g.wg_per_cu = occ[w][f];
+        return g;
     }
-    g.wg_per_cu = occ[i][hh][mb];
+    fold::layout(nb, nw, &g.wrows, &g.wcols, &g.mb);
+    static std::atomic<int> occ[fold::kMaxNW + 1][2][fold::kMaxMB + 1] = {};
+    const int i = nw < 1 ? 1 : (nw > fold::kMaxNW ? fold::kMaxNW : nw);
+    const int hh = nf < i ? 1 : 0;
+    if (occ[i][hh][g.mb] == 0)
+        occ[i][hh][g.mb] = occupancy(fold::kernel_for(i, nf, g.mb), fold::kThreads);
+    g.wg_per_cu = occ[i][hh][g.mb];
     return g;
 }
 
-// Share of the launch's block slots that hold an upper-triangle block.
-double contract_symbf16_slot_use(int Np, int nw)
+// Share of a launch's block slots that hold an upper-triangle block.
+double contract_symbf16_slot_use(int form, int Np, int nw)
 {
-    const Bf16Geometry g = contract_symbf16_geometry(Np, nw, nw);
+    const Bf16Geometry g = contract_symbf16_geometry(form, Np, nw, nw);
     const int nb = Np / 16;
-    return (double)nb * (nb + 1) / 2 / ((double)g.wcols * g.wrows * kWaves * g.mb);
+    const double slots = form == kBf16Strip
+                             ? (double)g.wcols * g.wrows * strip::kWaves * strip::kMR * 2
+                             : (double)g.wcols * g.wrows * fold::kWaves * g.mb;
+    return (double)nb * (nb + 1) / 2 / slots;
 }
 
-hipError_t contract_symbf16_launch(int nw, int nf, int mb, const ContractArgs& a,
+hipError_t contract_symbf16_launch(int form, int nw, int nf, int mb, const ContractArgs& a,
                                    hipStream_t stream)
 {
+    if (nw < 1 || nw > contract_symbf16_max_sets(form)) return hipErrorInvalidValue;
     const unsigned grid = (unsigned)(a.n_gangs * a.wrows * a.wcols);
     ContractArgs args = a;
     void* params[] = {(void*)&args};
-    return hipLaunchKernel(kernel_for(nw, nf, mb), dim3(grid), dim3(kThreads), params, 0, stream);
+    if (form == kBf16Strip)
+        return hipLaunchKernel(strip::kernel_for(nw, nf), dim3(grid), dim3(strip::kThreads),
+                               params, 0, stream);
+    return hipLaunchKernel(fold::kernel_for(nw, nf, mb), dim3(grid), dim3(fold::kThreads), params,
+                           0, stream);
 }
 
 int64_t symbf16_records(int64_t Kp) { return (Kp + kKq - 1) / kKq * kKq / 8; }

@@ -1109,6 +1109,9 @@ struct klsp_ctx {
     // otherwise; 1 / 2 force one (2 only up to order kLuStepMaxN)
     void* d_luaux = nullptr;  size_t luaux_cap = 0;   // step-wise LU bookkeeping (bytes)
     int lu_form = 0;
+    // real-basis BF16 contraction form (klsp_set_bf16_form): 0 = chosen per
+    // problem (bf16_form), kBf16Fold / kBf16Strip force one
+    int bf16_form = 0;
     int64_t scan_graph_builds = 0, scan_graph_launches = 0;
     // profiling
     bool prof = false;
@@ -1470,21 +1473,36 @@ static GangPlan plan_gangs_bf16(const klsp_ctx* ctx, int nw, int nf)
     return gp;
 }
 
-// Gang geometry of the real-basis BF16 contraction: folded column pairs
-// (wcols) x workgroups per pair (wrows); gangs per XCD as for the complex
-// kernel; K ranges per gang in multiples of 32 rows.
-// Whether BF16x3 real-basis passes of three or four weight sets are better
-// run as passes of two: their block slots fill the triangle clearly better.
-static bool bf16_split(const klsp_ctx* ctx)
+// Work decomposition of the real-basis BF16 contraction (klsp_set_bf16_form):
+// the strip form where its block slots fill the triangle nearly as well as
+// the fold form's (it reads half the A operands), else the fold form.
+constexpr double kBf16StripFill = 0.8;
+static int bf16_form(const klsp_ctx* ctx)
 {
-    return contract_symbf16_slot_use((int)ctx->Np, 2) >=
-           std::max(contract_symbf16_slot_use((int)ctx->Np, 3),
-                    contract_symbf16_slot_use((int)ctx->Np, 4)) + 0.05;
+    if (ctx->bf16_form) return ctx->bf16_form;
+    const int Np = (int)ctx->Np;
+    return contract_symbf16_slot_use(kBf16Strip, Np, 1) >=
+                   kBf16StripFill * contract_symbf16_slot_use(kBf16Fold, Np, 1)
+               ? kBf16Strip
+               : kBf16Fold;
 }
 
-static GangPlan plan_gangs_symbf16(const klsp_ctx* ctx, int nw, int nf, int* mb)
+// Whether fold-form passes of three or four weight sets are better run as
+// passes of two: their block slots fill the triangle clearly better.
+static bool bf16_split(const klsp_ctx* ctx)
 {
-    const Bf16Geometry geo = contract_symbf16_geometry(ctx->Np, nw, nf);
+    const int Np = (int)ctx->Np;
+    return contract_symbf16_slot_use(kBf16Fold, Np, 2) >=
+           std::max(contract_symbf16_slot_use(kBf16Fold, Np, 3),
+                    contract_symbf16_slot_use(kBf16Fold, Np, 4)) + 0.05;
+}
+
+// Gang geometry of the real-basis BF16 contraction: column groups (wcols) x
+// workgroups per group (wrows) of the given form; gangs per XCD as for the
+// complex kernel; K ranges per gang in multiples of 64 rows.
+static GangPlan plan_gangs_symbf16(const klsp_ctx* ctx, int form, int nw, int nf, int* mb)
+{
+    const Bf16Geometry geo = contract_symbf16_geometry(form, ctx->Np, nw, nf);
     GangPlan gp;
     gp.wrows = geo.wrows;
     gp.wcols = geo.wcols;
@@ -1528,12 +1546,12 @@ static int max_gangs(klsp_ctx* ctx, int* rc)
             g = std::max(g, b->n_slots);
         }
     int mb = 0;
-    for (int nw = 1; nw <= kMaxW; ++nw) {
-        g = std::max(g, plan_gangs_bf16(ctx, nw, nw).n_gangs);
-        g = std::max(g, plan_gangs_symbf16(ctx, nw, nw, &mb).n_gangs);
-    }
+    for (int nw = 1; nw <= kMaxW; ++nw) g = std::max(g, plan_gangs_bf16(ctx, nw, nw).n_gangs);
+    for (int form : {kBf16Fold, kBf16Strip})
+        for (int nw = 1; nw <= contract_symbf16_max_sets(form); ++nw)
+            for (int nf = std::max(0, nw - 1); nf <= nw; ++nf)
+                g = std::max(g, plan_gangs_symbf16(ctx, form, nw, nf, &mb).n_gangs);
     g = std::max(g, plan_gangs_bf16(ctx, kMaxW, kMaxW - 1).n_gangs);
-    g = std::max(g, plan_gangs_symbf16(ctx, kMaxW, kMaxW - 1, &mb).n_gangs);
     const klsp_ctx::Balanced* b32 = nullptr;
     if ((*rc = get_balanced_sym32(ctx, &b32))) return 0;
     g = std::max(g, b32->n_slots);
@@ -2172,28 +2190,35 @@ static int launch_contract(klsp_ctx* ctx, int precision, int nb, int nf)
         return KLSP_OK;
     }
     if (bf && ctx->use_sym && ctx->sym == 1 && ctx->have_Rb) {
-        // four weight sets as two launches of two where that fills the
-        // block slots better (N = 301: 95% against 79%; 3.35 ms a weight set
-        // against 3.6); the second launch's weights and partial slices are
-        // those of sets 2 and 3, so the reduce sees one launch of four
-        const int nl = nb == 4 && bf16_split(ctx) ? 2 : 1;
-        const int nw = nb / nl;
+        // weight sets in launches of at most `per`: two in the strip form
+        // (its register budget); in the fold form all of them, or four as
+        // two of two where that fills the block slots better (N = 301: 95%
+        // against 79%).  Launch h takes sets from per * h on, with their
+        // weights and partial slices, so the reduce sees one pass of nb
+        // sets; all launches use the first one's gang plan (the partial
+        // slices of a pass share one gang count)
+        const int form = bf16_form(ctx);
+        const int per = form == kBf16Strip ? contract_symbf16_max_sets(form)
+                        : nb == 4 && bf16_split(ctx) ? 2
+                                                     : nb;
+        const int nl = (nb + per - 1) / per;
         int mb = 0;
-        gp = plan_gangs_symbf16(ctx, nw, nl == 2 ? nw : nf, &mb);
+        gp = plan_gangs_symbf16(ctx, form, std::min(nb, per), nl > 1 ? std::min(nf, per) : nf, &mb);
         for (int h = 0; h < nl; ++h) {
+            const int nw = std::min(per, nb - h * per);
             ContractArgs a{};
             a.W = ctx->d_Rb;
             a.cw = ctx->d_c;
-            a.cwf = ctx->d_cf + (size_t)h * nw * (ctx->Kp + kRowPad);
-            a.P = ctx->d_P + (size_t)h * nw * gp.n_gangs * ctx->Np * ctx->Np;
+            a.cwf = ctx->d_cf + (size_t)h * per * (ctx->Kp + kRowPad);
+            a.P = ctx->d_P + (size_t)h * per * gp.n_gangs * ctx->Np * ctx->Np;
             a.Kp = ctx->Kp;
             a.Np = ctx->Np;
             a.Kg = gp.Kg;
             a.n_gangs = gp.n_gangs;
             a.wrows = gp.wrows;
             a.wcols = gp.wcols;
-            const int nfh = nl == 1 ? nf : std::min(nw, std::max(0, nf - h * nw));
-            const hipError_t e = contract_symbf16_launch(nw, nfh, mb, a, ctx->stream);
+            const int nfh = std::min(nw, std::max(0, nf - h * per));
+            const hipError_t e = contract_symbf16_launch(form, nw, nfh, mb, a, ctx->stream);
             if (e != hipSuccess)
                 return fail(ctx, KLSP_ERR_HIP,
                             std::string("contraction launch: ") + hipGetErrorString(e));
@@ -2419,12 +2444,15 @@ static int check_ready(klsp_ctx* ctx, int precision, const char* who)
 
 // Omegas per contraction pass: FP64 is MFMA-bound at one omega per pass
 // (larger batches only lower the occupancy); the low-precision passes share
-// W loads among up to three, or two in the real basis in BF16x3 where passes
-// of two fill the block slots better (bf16_split).
+// W loads among up to three, or in the real basis in BF16x3 two where one
+// launch takes no more (strip form) or passes of two fill the block slots
+// better (fold form, bf16_split).
 static int omega_batch(const klsp_ctx* ctx, int precision)
 {
     if (precision == KLSP_PREC_FP64) return 1;
-    if (precision == KLSP_PREC_BF16X3 && ctx->use_sym && ctx->sym == 1 && bf16_split(ctx)) return 2;
+    if (precision == KLSP_PREC_BF16X3 && ctx->use_sym && ctx->sym == 1 &&
+        (bf16_form(ctx) == kBf16Strip || bf16_split(ctx)))
+        return 2;
     return kMaxBatch;
 }
 
@@ -2472,7 +2500,7 @@ static std::vector<int64_t> scan_graph_key(const klsp_ctx* ctx, int64_t n, int p
         P(ctx->d_scan_det), P(ctx->bal_sym.d_units), P(ctx->bal_sym.d_count),
         ctx->bal_sym.n_units, ctx->bal_sym.n_slots, P(ctx->d_R32), ctx->have_R32,
         P(ctx->bal_sym32.d_units), P(ctx->bal_sym32.d_count), ctx->bal_sym32.n_units,
-        ctx->bal_sym32.n_slots, P(ctx->d_luaux), ctx->lu_form};
+        ctx->bal_sym32.n_slots, P(ctx->d_luaux), ctx->lu_form, ctx->bf16_form};
     for (const auto& row : ctx->bal)
         for (const auto& b : row) {
             k.push_back(P(b.d_units));
@@ -3306,6 +3334,13 @@ int klsp_set_lu_form(klsp_ctx* ctx, int form)
 {
     if (!ctx || form < 0 || form > 2) return KLSP_ERR_INVALID_ARG;
     ctx->lu_form = form;
+    return KLSP_OK;
+}
+
+int klsp_set_bf16_form(klsp_ctx* ctx, int form)
+{
+    if (!ctx || form < 0 || form > 2) return KLSP_ERR_INVALID_ARG;
+    ctx->bf16_form = form;
     return KLSP_OK;
 }
 
