@@ -92,6 +92,36 @@ def test_mixed_search_same_root_in_both_forms(oracle_small):
         pre.ctx.close()
 
 
+@pytest.mark.parametrize("form", [FOLD, STRIP])
+def test_graph_scan_in_each_form(oracle_small, form):
+    """A BF16x3 scan captured as one HIP graph equals the same scan launched
+    kernel by kernel, bit for bit, in each form, on repeated graph launches
+    with eager dets between them; changing the form instantiates a new
+    graph."""
+    o = oracle_small
+    pre = from_arrays(o["m"], o["G"], o["W"], o["DJ"], o["F0l"], o["Omega_res"], o["N_kernel"])
+    ctx = pre.ctx
+    try:
+        win = (7, 5, 0.30, 0.60, 0.02, 0.25)
+        ctx.set_bf16_form(form)
+        ctx.set_scan_graph(False)
+        ref = ctx.grid_scan(*win, precision=_ffi.PREC_BF16X3)
+        ctx.set_scan_graph(True)
+        b0, _ = ctx.scan_graph_stats()
+        for k in range(3):
+            ctx.det([0.88 + 0.12j, 0.7 + 0.05j], _ffi.PREC_BF16X3)
+            dg, op0, g0 = ctx.grid_scan(*win, precision=_ffi.PREC_BF16X3)
+            assert np.array_equal(dg, ref[0]), (form, k, np.max(np.abs(dg - ref[0])))
+            assert (op0, g0) == (ref[1], ref[2])
+        b1, _ = ctx.scan_graph_stats()
+        assert b1 - b0 == 1
+        ctx.set_bf16_form(FOLD if form == STRIP else STRIP)
+        ctx.grid_scan(*win, precision=_ffi.PREC_BF16X3)
+        assert ctx.scan_graph_stats()[0] == b1 + 1
+    finally:
+        ctx.close()
+
+
 def test_bf16_form_rejects_unknown_values(oracle_small):
     o = oracle_small
     pre = from_arrays(o["m"], o["G"], o["W"], o["DJ"], o["F0l"], o["Omega_res"], o["N_kernel"])
