@@ -231,10 +231,22 @@ class Context:
         st = lib().klsp_create(device, ct.byref(h))
         if st != KLSP_OK:
             raise KlspError(st, f"cannot create a context on device {device}")
-        self.handle = h
+        self._handle = h
         self.device = device
         self.dims = None
         self._fin = weakref.finalize(self, _destroy, h)
+
+    @property
+    def handle(self):
+        """The native context; a closed context raises instead of handing the
+        library a pointer to freed memory."""
+        if not self._fin.alive:
+            raise KlspError(KLSP_ERR_INVALID_ARG, "context is closed")
+        return self._handle
+
+    @property
+    def closed(self) -> bool:
+        return not self._fin.alive
 
     def close(self):
         self._fin()

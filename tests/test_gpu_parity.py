@@ -662,6 +662,29 @@ def test_context_reuse_across_problems():
         shared.close()
 
 
+def test_closed_context_raises(oracle_small):
+    """Calls on a closed context, and to_device on data whose context was
+    closed, raise KlspError instead of passing freed memory to the library;
+    closing twice is harmless and other contexts keep working."""
+    pre = device_pre(oracle_small)
+    ctx = pre.ctx
+    d0 = ctx.det([0.6 + 0.05j], _ffi.PREC_FP64)
+    ctx.close()
+    assert ctx.closed
+    ctx.close()
+    for call in (lambda: ctx.det([0.6 + 0.05j], _ffi.PREC_FP64),
+                 lambda: ctx.prepare(_ffi.PREC_FP64),
+                 lambda: ctx.newton(0.45, 0.11, precision=_ffi.PREC_MIXED),
+                 lambda: K.to_device(pre, K.GPUType.AMD, np.float64)):
+        with pytest.raises(_ffi.KlspError, match="closed"):
+            call()
+    again = device_pre(oracle_small)
+    try:
+        assert np.array_equal(again.ctx.det([0.6 + 0.05j], _ffi.PREC_FP64), d0)
+    finally:
+        again.ctx.close()
+
+
 # --------------------------------------------------------------------------
 # Non-finite input and the convergence flag of the mixed search
 # --------------------------------------------------------------------------
