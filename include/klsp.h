@@ -335,12 +335,16 @@ typedef struct klsp_newton_result {
  *      anchor_tol below) is confirmed by Newton steps on it (FP64 LUs) until
  *      the step stops shrinking; converged if then |det| < tol.  A new anchor is built if
  *      the root moves so far from the last one that the model's slope error
- *      times that move could exceed 5e-11: the slope error is estimated as
- *      twice the truncation of the pass's series (its first omitted term,
- *      bounded by |T_nt| / Im w_c) plus four times the rounding error the
- *      low-precision terms showed at the first anchor (the move of the
- *      pass's zero times |T_2| / |T_1|); the new anchor is built at the
- *      anchored model's own zero (converged on first).
+ *      times that move could exceed 5e-11.  The slope error is taken along
+ *      the null vectors of I - M, where a term's weight is the partial
+ *      derivative of the model's det in its coefficient (central differences
+ *      at the anchor, one LU launch): twice the truncation of the pass's
+ *      series (its first omitted term, the ratio of the last built terms
+ *      continued) plus the rounding of the low-precision terms (four times
+ *      what moved the pass's zero at the first anchor, at least 4e-5 for
+ *      BF16x3 and 2e-6 for FP32); the new anchor is built at the anchored
+ *      model's own zero (converged on first).  Passes after one that left
+ *      the root within 1e-4 build three terms.
  * iterations = low-precision passes + anchors + 1; history rows record each
  * centre with its low-precision |det|, each anchor with its FP64 |det|, and
  * the final point.  Bounds (gamma >= 1e-4, Omega_p in [0, 1]) apply to

@@ -2743,10 +2743,13 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
     // |s| < Im w).  The anchored finish below corrects e with that slope, so
     // the passes stop once e < 1e-6 and e nt r^(nt-1) < 1e-12 (or m < 1e-6);
     // from the default start (m = 4e-3, r = 0.03) one 4-term BF16x3 pass
-    // suffices.  When e < 1e-4 the next pass builds two terms only (its own
-    // truncation, ~(1e-4 / Im w)^2 1e-4, is far below the low-precision
-    // error).  A pass ends once a model Newton step below accept_step is
-    // applied: from there the step error is ~|step|^2 / Im w.
+    // suffices.  When e < 1e-4 the next pass builds three terms: the
+    // anchored finish takes its slope between the anchor and the root from
+    // T_2 and T_3, and along the det's null vectors T_3 can be ~150 times T_2
+    // per unit of omega (Im w ~ 0.017), so a two-term model's slope would be
+    // off by ~300 |s_a| ~ 1e-4 there.  A pass ends once a model Newton step
+    // below accept_step is applied: from there the step error is
+    // ~|step|^2 / Im w.
     const int nt_max = lp_prec == KLSP_PREC_BF16X3 ? kMaxTerms : kMaxBatch;
     const int64_t NN = ctx->N * ctx->N;
     double tnorm[kMaxW] = {};           // Frobenius norms of the last pass's terms
@@ -2815,7 +2818,7 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
         const double e = std::pow(r, nt) * moved, slope_err = nt * std::pow(r, nt - 1);
         slope_last = slope_err;
         if (e < switch_step && e * slope_err < 1e-12) break;
-        nt = e < 1e-4 ? 2 : nt_max;
+        nt = e < 1e-4 ? 3 : nt_max;
     }
     // 2. FP64 finish on the anchored model.  One FP64 build at the
     // low-precision root w_a gives M64(w_a); with s = w - w_c and s_a = w_a - w_c
@@ -2828,24 +2831,36 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
     // FP64 LUs only; the anchor is rebuilt where the root moves too far.
     const double h64 = 1e-5;            // central-difference step on the model
     // The anchored model's error moves its zero by ~sigma |s - s_a|, sigma the
-    // relative error of the model's slope, which has two parts:
+    // error of the model's slope relative to the slope itself, both taken
+    // along the null vectors u, v of I - M at the zero: a term E of M moves
+    // the zero by v^H E u / v^H T_2 u.  Norms do not bound these ratios
+    // (v^H T_2 u is small where the resonant parts of T_2 cancel; at
+    // Im w ~ 0.017 |v^H T_3 u| / |v^H T_2 u| ~ 150 where |T_3| / |T_2| ~ 15),
+    // so they are measured: near the zero adj(I - M) ~ kappa u v^H, and the
+    // partial derivatives of the anchored model's det in its coefficients,
+    //   g_j = |d det / d c_j| ~ |kappa| |v^H T_{j+1} u|   (T_1 = the anchor),
+    // come from one LU launch of central differences at s = s_a.  Then
     //  - truncation of the pass's series: the first term left out enters the
-    //    slope as nt s_a^(nt-1) T_{nt+1}, and |T_{nt+1}| <= |T_nt| / Im w_c
-    //    (every pole of the weights lies on the real axis), or, from three
-    //    terms on, the geometric continuation |T_nt|^2 / |T_{nt-1}| where
-    //    that is smaller;
-    //  - rounding of the low-precision terms, which moved the pass's zero by
-    //    delta_b = |s - s_a| at the first anchor: the slope T_2 carries
-    //    ~delta_b |T_2| / |T_1| of relative error.
+    //    slope as nt s_a^(nt-1) T_{nt+1}; its ratio q = g_nt / g_(nt-1) is
+    //    continued from the last built ones as max(1 / Im w_c, q_last,
+    //    q_last^2 / q_prev);
+    //  - rounding of the low-precision terms: T_2's moved the pass's zero by
+    //    delta_b = |s - s_a| at the first anchor, ~delta_b g_1 / g_0 of
+    //    relative error (the rounding of T_1 along u, v); that of T_2 itself
+    //    is held to at least eps_lp (the largest seen: 2.1e-5 BF16x3,
+    //    5.8e-7 FP32), and T_{j+1}, j >= 2, adds j |s_a|^(j-1) g_j / g_1 times
+    //    eps_lp (BF16x3's hi x hi fourth term: 1e-2).
     // A zero is accepted where
-    //   |s - s_a| (2 trunc + 4 delta_b |T_2| / |T_1|) <= 5e-11,
+    //   |s - s_a| (2 trunc + max(4 delta_b g_1 / g_0, eps_lp) + higher) <= 5e-11,
     // half the 1e-10 the roots are held to; further out a new anchor is built
-    // at the model's zero.  (Over the default grid and the 48 uneven draws
-    // the first anchor's zero lies at most 2.0x this estimate from the FP64
-    // root; default grid: truncation 2.2e-5, the BF16x3 zero 5.0e-7 off,
-    // anchor_tol 9.3e-7, one FP64 build.)  Without finite term
-    // norms the bound falls back to nt r^(nt-1) + 1e-4 (BF16x3) or 1e-5 (FP32).
+    // at the model's zero.  (At 47 pass centres, 3e-7 to 4e-3 from three roots
+    // with Im w from 0.015 to 0.127, this estimate was 2.6x to 150x the
+    // anchored zero's distance from the FP64 root.  Default grid: the BF16x3
+    // zero 5.8e-7 off, estimate 3.1e-11, one FP64 build.)  Without finite
+    // partials the bound falls back to nt r^(nt-1) + 1e-4 (BF16x3) or 1e-5
+    // (FP32).
     const double eps_fallback = lp_prec == KLSP_PREC_BF16X3 ? 1e-4 : 1e-5;
+    const double eps_lp = lp_prec == KLSP_PREC_BF16X3 ? 4e-5 : 2e-6;
     double delta_b = NAN;               // |s - s_a| at the first anchor
     cd D(NAN, NAN);
     int conv = 0;
@@ -2862,22 +2877,51 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
             ++n_64;
         }
         const cd sa = w - wc;
+        // g_j: the anchored model's det at s = s_a (c_0 = 1, c_j = 0) with c_j
+        // moved by +-h_j, h_j |T_{j+1}| = 1e-6 |T_1|, in one LU launch
+        double g[kMaxW] = {};
+        bool have_g = nt_last >= 3 && 2 * nt_last <= kModelPts && tnorm[0] > 0.0;
+        if (have_g) {
+            double hj[kMaxW];
+            cd cf[2 * kMaxW * kMaxW], d[2 * kMaxW];
+            for (int j = 0; j < nt_last; ++j) {
+                hj[j] = 1e-6 * tnorm[0] / tnorm[j];
+                if (!(hj[j] > 0.0) || !std::isfinite(hj[j])) have_g = false;
+                for (int e = 0; e < 2; ++e) {
+                    cd* c = cf + (2 * j + e) * nt_last;
+                    for (int i = 0; i < nt_last; ++i) c[i] = i == 0 ? 1.0 : 0.0;
+                    c[j] += e ? -hj[j] : hj[j];
+                }
+            }
+            if (have_g) {
+                if ((rc = model_dets(ctx, nt_last, cf, 2 * nt_last, d))) return rc;
+                acc_timing(acc, ctx);
+                for (int j = 0; j < nt_last; ++j) {
+                    g[j] = std::abs(d[2 * j] - d[2 * j + 1]) / (2.0 * hj[j]);
+                    if (!(g[j] > 0.0) || !std::isfinite(g[j])) have_g = false;
+                }
+            }
+        }
         // anchor_tol: the largest |s - s_a| the bound above accepts (at the
         // first anchor delta_b is that same distance: a quadratic)
         double anchor_tol = NAN;
-        if (nt_last >= 2 && tnorm[0] > 0.0 && tnorm[1] > 0.0 && std::isfinite(tnorm[nt_last - 1])) {
-            // |T_{nt+1}| <= |T_nt| / Im w_c; from three terms on, also the
-            // geometric continuation of the last two (T_1 holds the
-            // non-resonant bulk, so T_2 / T_1 says nothing about the tail)
-            double t_next = tnorm[nt_last - 1] / std::max(wc.imag(), 1e-300);
-            if (nt_last >= 3 && tnorm[nt_last - 2] > 0.0)
-                t_next = std::min(t_next, tnorm[nt_last - 1] * tnorm[nt_last - 1] /
-                                              tnorm[nt_last - 2]);
-            const double a = 2.0 * nt_last * std::pow(std::abs(sa), nt_last - 1) * t_next /
-                             tnorm[1];
-            const double b = 4.0 * tnorm[1] / tnorm[0];
-            anchor_tol = anchor == 0 ? 2.0 * 5e-11 / (a + std::sqrt(a * a + 4.0 * b * 5e-11))
-                                     : 5e-11 / (a + b * delta_b);
+        if (have_g) {
+            const double asa = std::abs(sa);
+            const double q_last = g[nt_last - 1] / g[nt_last - 2];
+            const double q_prev = g[nt_last - 2] / g[nt_last - 3];
+            const double q_next = std::max(std::max(1.0 / std::max(wc.imag(), 1e-300), q_last),
+                                           q_last * q_last / q_prev);
+            double a = 2.0 * nt_last * std::pow(asa, nt_last - 1) * g[nt_last - 1] * q_next / g[1];
+            for (int j = 2; j < nt_last; ++j) {
+                // a pass of kMaxW terms builds the last from hi x hi alone
+                const double eps_j = nt_last == kMaxW && j == kMaxW - 1 ? 1e-2 : eps_lp;
+                a += j * std::pow(asa, j - 1) * g[j] / g[1] * eps_j;
+            }
+            const double b = 4.0 * g[1] / g[0];
+            anchor_tol = anchor == 0
+                ? std::min(5e-11 / (a + eps_lp),
+                           2.0 * 5e-11 / (a + std::sqrt(a * a + 4.0 * b * 5e-11)))
+                : 5e-11 / (a + std::max(b * delta_b, eps_lp));
         }
         if (!(anchor_tol > 0.0) || !std::isfinite(anchor_tol))
             anchor_tol = 5e-11 / (std::min(slope_last, 1.0) + eps_fallback);

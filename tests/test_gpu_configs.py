@@ -220,6 +220,29 @@ def test_config3_default_grid_mixed_root(default_grid, default_fp64_root):
     assert rm.n_fp64_builds == 1 and rm.n_lp_builds == 1
 
 
+# starts (Omega_p, gamma) from which the mixed search reaches roots with
+# gamma ~ 0.015-0.017; the first is a seeded random start whose root a bound
+# on Frobenius norms of the Taylor terms once accepted 1.2e-10 off
+SMALL_GAMMA_STARTS = [(0.38502504623867634, 0.12373476766636918), (0.3978, 0.0999),
+                      (0.3229, 0.1477), (0.3621, 0.2108)]
+
+
+@pytest.mark.parametrize("start", SMALL_GAMMA_STARTS)
+def test_config3_default_grid_mixed_small_gamma_roots(default_grid, start):
+    """Near roots with small gamma, |v^H T_3 u| / |v^H T_2 u| (u, v the null
+    vectors of I - M) is ~10 times what the term norms suggest; the mixed
+    search's roots there are still FP64 roots: two FP64 Newton steps with
+    tol = 0 move them by at most 1e-10."""
+    cfg, pre, dev = default_grid
+    rm, _ = dev.ctx.newton(*start, max_iter=50, tol=1e-12, precision=_ffi.PREC_MIXED)
+    assert rm.converged and rm.gamma < 0.02
+    pol, _ = dev.ctx.newton(rm.Omega_p, rm.gamma, max_iter=2, tol=0.0, precision=_ffi.PREC_FP64)
+    move = abs(complex(pol.omega[0], pol.omega[1]) - complex(rm.omega[0], rm.omega[1]))
+    print(f"start {start}: root ({rm.Omega_p:.10f}, {rm.gamma:.10f}), {rm.n_lp_builds} low + "
+          f"{rm.n_fp64_builds} FP64 builds, polish move {move:.2e}", flush=True)
+    assert move <= 1e-10
+
+
 def test_config4_default_grid_64x64_scan(default_grid):
     """The 64 x 64 window (Re w/m in linspace(0.30, 0.60, 64), Im w in
     linspace(0.02, 0.25, 64)) in BF16x3 against FP64 dets at 64 SplitMix64
