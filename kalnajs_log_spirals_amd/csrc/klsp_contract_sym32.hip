@@ -19,8 +19,18 @@
 // K-steps ahead.
 #include <algorithm>
 
+// Build-time knobs (defaults are the measured best at the default grid):
+// operand sets in flight, tile shape in 16 x 16 blocks, and an optional
+// waves-per-SIMD request (2 fits the 4 x 2 tile in 247 VGPRs without AGPRs;
+// measured no faster, 13.8 vs 13.7 ms).
 #ifndef KLSP_SYM32_DEPTH
 #define KLSP_SYM32_DEPTH 4
+#endif
+#ifndef KLSP_SYM32_MB
+#define KLSP_SYM32_MB 4
+#endif
+#ifndef KLSP_SYM32_NB
+#define KLSP_SYM32_NB 2
 #endif
 
 #include "klsp_contract.hpp"
@@ -28,7 +38,7 @@
 namespace klsp {
 namespace {
 
-constexpr int kS32MB = 4, kS32NB = 2;
+constexpr int kS32MB = KLSP_SYM32_MB, kS32NB = KLSP_SYM32_NB;
 constexpr int kSym32Depth = KLSP_SYM32_DEPTH;
 
 template <int MB, int NB>
@@ -172,8 +182,13 @@ __device__ __forceinline__ void dispatch32(int mv, int nv, const ContractArgs& a
     wave_sym32<D, MB, NB>(a, Kb, Ke, b0, a0, slot);
 }
 
+#ifdef KLSP_SYM32_WPE
+#define KLSP_SYM32_ATTR __attribute__((amdgpu_waves_per_eu(KLSP_SYM32_WPE, KLSP_SYM32_WPE)))
+#else
+#define KLSP_SYM32_ATTR
+#endif
 template <int D>
-__global__ __launch_bounds__(64) void k_contract_sym32(ContractArgs a)
+__global__ __launch_bounds__(64) KLSP_SYM32_ATTR void k_contract_sym32(ContractArgs a)
 {
     const WorkUnit u = a.units[blockIdx.x];
     if (u.widx < 0) return;
