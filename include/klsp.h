@@ -340,9 +340,10 @@ typedef struct klsp_newton_result {
  *      derivative of the model's det in its coefficient (central differences
  *      at the anchor, one LU launch): twice the truncation of the pass's
  *      series (its first omitted term, the ratio of the last built terms
- *      continued) plus the rounding of the low-precision terms (four times
- *      what moved the pass's zero at the first anchor, at least 4e-5 for
- *      BF16x3 and 2e-6 for FP32); the new anchor is built at the anchored
+ *      continued) plus the rounding of the low-precision terms (sixteen
+ *      times the move of the pass's zero at the first anchor times
+ *      |T_2| / |T_1|, at least 4e-5 for BF16x3 and 2e-6 for FP32); the new
+ *      anchor is built at the anchored
  *      model's own zero (converged on first).  Passes after one that left
  *      the root within 1e-4 build three terms.
  * iterations = low-precision passes + anchors + 1; history rows record each

@@ -2844,23 +2844,30 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
     //    slope as nt s_a^(nt-1) T_{nt+1}; its ratio q = g_nt / g_(nt-1) is
     //    continued from the last built ones as max(1 / Im w_c, q_last,
     //    q_last^2 / q_prev);
-    //  - rounding of the low-precision terms: T_2's moved the pass's zero by
-    //    delta_b = |s - s_a| at the first anchor, ~delta_b g_1 / g_0 of
-    //    relative error (the rounding of T_1 along u, v); that of T_2 itself
-    //    is held to at least eps_lp (the largest seen: 2.1e-5 BF16x3,
-    //    5.8e-7 FP32), and T_{j+1}, j >= 2, adds j |s_a|^(j-1) g_j / g_1 times
-    //    eps_lp (BF16x3's hi x hi fourth term: 1e-2).
+    //  - rounding of the low-precision terms.  The rounding E_1 of T_1 moved
+    //    the pass's zero by delta_b = |s - s_a| = |v^H E_1 u| / |v^H T_2 u| at
+    //    the first anchor, and T_2's relative error along u, v is
+    //    |v^H E_2 u| / |v^H T_2 u| = delta_b |v^H E_2 u| / |v^H E_1 u|: a ratio
+    //    of two projections of rounding matrices, which follows |E_2| / |E_1|
+    //    ~ |T_2| / |T_1| up to a heavy-tailed factor (0.5 to 12.7 where
+    //    delta_b >= 1e-7 over 48 measured anchors; kRoundRatio = 16 covers
+    //    it: the largest, 9.6e-5 of BF16x3 rounding where |v^H u| = 0.016,
+    //    came with delta_b = 1e-6).  Where delta_b happens to be small the
+    //    factor can be larger (20 at delta_b = 5e-8), so the error is also
+    //    held to at least eps_lp (about twice the largest seen at such
+    //    anchors: 2.1e-5 BF16x3, 5.8e-7 FP32).  T_{j+1}, j >= 2, adds
+    //    j |s_a|^(j-1) times the larger of
+    //    g_j / g_1 eps_lp (BF16x3's hi x hi fourth term: 1e-2) and
+    //    kRoundRatio delta_b |T_{j+1}| / |T_1|.
     // A zero is accepted where
-    //   |s - s_a| (2 trunc + max(4 delta_b g_1 / g_0, eps_lp) + higher) <= 5e-11,
+    //   |s - s_a| (2 trunc + max(eps_lp, 16 delta_b |T_2| / |T_1|) + higher) <= 5e-11,
     // half the 1e-10 the roots are held to; further out a new anchor is built
-    // at the model's zero.  (At 47 pass centres, 3e-7 to 4e-3 from three roots
-    // with Im w from 0.015 to 0.127, this estimate was 2.6x to 150x the
-    // anchored zero's distance from the FP64 root.  Default grid: the BF16x3
-    // zero 5.8e-7 off, estimate 3.1e-11, one FP64 build.)  Without finite
-    // partials the bound falls back to nt r^(nt-1) + 1e-4 (BF16x3) or 1e-5
-    // (FP32).
+    // at the model's zero.  (Default grid: the BF16x3 zero 5.8e-7 off,
+    // estimate 3.1e-11, one FP64 build.)  Without finite partials the bound
+    // falls back to nt r^(nt-1) + 1e-4 (BF16x3) or 1e-5 (FP32).
     const double eps_fallback = lp_prec == KLSP_PREC_BF16X3 ? 1e-4 : 1e-5;
     const double eps_lp = lp_prec == KLSP_PREC_BF16X3 ? 4e-5 : 2e-6;
+    const double kRoundRatio = 16.0;
     double delta_b = NAN;               // |s - s_a| at the first anchor
     cd D(NAN, NAN);
     int conv = 0;
@@ -2911,17 +2918,30 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
             const double q_prev = g[nt_last - 2] / g[nt_last - 3];
             const double q_next = std::max(std::max(1.0 / std::max(wc.imag(), 1e-300), q_last),
                                            q_last * q_last / q_prev);
-            double a = 2.0 * nt_last * std::pow(asa, nt_last - 1) * g[nt_last - 1] * q_next / g[1];
-            for (int j = 2; j < nt_last; ++j) {
-                // a pass of kMaxW terms builds the last from hi x hi alone
-                const double eps_j = nt_last == kMaxW && j == kMaxW - 1 ? 1e-2 : eps_lp;
-                a += j * std::pow(asa, j - 1) * g[j] / g[1] * eps_j;
+            const double trunc =
+                nt_last * std::pow(asa, nt_last - 1) * g[nt_last - 1] * q_next / g[1];
+            // sigma for a pass zero that lay x from the anchored one
+            auto sigma = [&](double x) {
+                double sg = 2.0 * trunc + std::max(eps_lp, kRoundRatio * x * tnorm[1] / tnorm[0]);
+                for (int j = 2; j < nt_last; ++j) {
+                    // a pass of kMaxW terms builds the last from hi x hi alone
+                    const double eps_j = nt_last == kMaxW && j == kMaxW - 1 ? 1e-2 : eps_lp;
+                    sg += j * std::pow(asa, j - 1) *
+                          std::max(g[j] / g[1] * eps_j, kRoundRatio * x * tnorm[j] / tnorm[0]);
+                }
+                return sg;
+            };
+            if (anchor == 0) {
+                // x sigma(x) increases with x: the largest x it keeps <= 5e-11
+                double lo = 0.0, hi = 5e-11 / sigma(0.0);
+                for (int it = 0; it < 60; ++it) {
+                    const double mid = 0.5 * (lo + hi);
+                    (mid * sigma(mid) <= 5e-11 ? lo : hi) = mid;
+                }
+                anchor_tol = lo;
+            } else {
+                anchor_tol = 5e-11 / sigma(delta_b);
             }
-            const double b = 4.0 * g[1] / g[0];
-            anchor_tol = anchor == 0
-                ? std::min(5e-11 / (a + eps_lp),
-                           2.0 * 5e-11 / (a + std::sqrt(a * a + 4.0 * b * 5e-11)))
-                : 5e-11 / (a + std::max(b * delta_b, eps_lp));
         }
         if (!(anchor_tol > 0.0) || !std::isfinite(anchor_tol))
             anchor_tol = 5e-11 / (std::min(slope_last, 1.0) + eps_fallback);
