@@ -314,9 +314,11 @@ typedef struct klsp_newton_result {
  * min(iterations, max_iter) rows are filled.  May be NULL.
  *
  * precision FP64 / FP32 / BF16X3: the reference iteration, every det in
- * that precision.  precision MIXED (no counterpart in the reference; it
- * returns the same root to |d omega| <= 1e-10 at a fraction of the FP64
- * builds):
+ * that precision.  precision MIXED (no counterpart in the reference; at a
+ * fraction of the FP64 builds it returns an FP64 root to |d omega| <= 1e-10
+ * by a calibrated estimate, the FP64 search's root where both searches
+ * reach the same one; from far starts the two paths can end at different
+ * roots):
  *   1. low-precision passes: at a centre w_c one BF16x3 (MIXED_FP32: FP32)
  *      contraction builds the Taylor terms T_j = G N sum W c/d^j conj W
  *      (d = w_c - Omega_res, j = 1..4 for BF16x3, 1..3 for FP32), so
