@@ -313,7 +313,9 @@ NewtonSolver.jl:36-172 (forward-difference Newton on (Omega_p, gamma), the
 optional halving line search, the bounds gamma >= 1e-4 and Omega_p in [0, 1],
 the final re-evaluation) run by klsp_newton with every det on the device.
 precision = :mixed selects the mixed-precision search (no counterpart in the
-reference; klsp.h documents it: the same root to |d omega| <= 1e-10).
+reference; klsp.h documents it: an FP64 root to |d omega| <= 1e-10 by a
+calibrated estimate of its error, which from far starts may differ from the
+root the FP64 iteration reaches).
 """
 function newton_search(config::KalnajsConfig, precomputed::PrecomputedData,
                        devprecomp::Union{Nothing,DevicePrecomputed}, gpu_type, Tcpu::Type,

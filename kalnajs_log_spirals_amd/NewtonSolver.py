@@ -8,8 +8,11 @@ difference pair of each step evaluated as one two-omega batch.
 ``precision="mixed"`` (no counterpart in the reference) selects libklsp's
 mixed-precision search: BF16x3 passes ("mixed_fp32": FP32 passes) that each
 build the first four (FP32: three) Taylor terms of M around a centre and run Newton on
-that model, then one FP64 build that anchors the model for the FP64 finish;
-it returns the FP64 root to |d omega| <= 1e-10 with one FP64 build.  In
+that model, then an FP64 build that anchors the model for the FP64 finish
+(a second one where the estimated error of the anchored model's zero exceeds
+5e-11); it returns an FP64 root to |d omega| <= 1e-10 by that calibrated
+estimate, which from far starts may differ from the root the FP64 iteration
+reaches.  In
 that mode ``det_val`` is the anchored model's det at the returned omega (the
 last FP64 build plus the low-precision Taylor corrections, which enter
 multiplied by the distance from that anchor), not a fresh FP64 build; the
