@@ -2843,7 +2843,9 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
     //  - truncation of the pass's series: the first term left out enters the
     //    slope as nt s_a^(nt-1) T_{nt+1}; its ratio q = g_nt / g_(nt-1) is
     //    continued from the last built ones as max(1 / Im w_c, q_last,
-    //    q_last^2 / q_prev);
+    //    q_last^2 / q_prev), and taken four times: ratios along u, v can grow
+    //    faster than geometrically (1.7, 3.4, then ~24 where 1/Im w = 8.1, on
+    //    the small grid's published root after a three-term FP32 pass);
     //  - rounding of the low-precision terms.  The rounding E_1 of T_1 moved
     //    the pass's zero by delta_b = |s - s_a| = |v^H E_1 u| / |v^H T_2 u| at
     //    the first anchor, and T_2's relative error along u, v is
@@ -2860,10 +2862,10 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
     //    g_j / g_1 eps_lp (BF16x3's hi x hi fourth term: 1e-2) and
     //    kRoundRatio delta_b |T_{j+1}| / |T_1|.
     // A zero is accepted where
-    //   |s - s_a| (2 trunc + max(eps_lp, 16 delta_b |T_2| / |T_1|) + higher) <= 5e-11,
+    //   |s - s_a| (4 trunc + max(eps_lp, 16 delta_b |T_2| / |T_1|) + higher) <= 5e-11,
     // half the 1e-10 the roots are held to; further out a new anchor is built
     // at the model's zero.  (Default grid: the BF16x3 zero 5.8e-7 off,
-    // estimate 3.1e-11, one FP64 build.)  Without finite partials the bound
+    // estimate 3.7e-11, one FP64 build.)  Without finite partials the bound
     // falls back to nt r^(nt-1) + 1e-4 (BF16x3) or 1e-5 (FP32).
     const double eps_fallback = lp_prec == KLSP_PREC_BF16X3 ? 1e-4 : 1e-5;
     const double eps_lp = lp_prec == KLSP_PREC_BF16X3 ? 4e-5 : 2e-6;
@@ -2922,7 +2924,7 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
                 nt_last * std::pow(asa, nt_last - 1) * g[nt_last - 1] * q_next / g[1];
             // sigma for a pass zero that lay x from the anchored one
             auto sigma = [&](double x) {
-                double sg = 2.0 * trunc + std::max(eps_lp, kRoundRatio * x * tnorm[1] / tnorm[0]);
+                double sg = 4.0 * trunc + std::max(eps_lp, kRoundRatio * x * tnorm[1] / tnorm[0]);
                 for (int j = 2; j < nt_last; ++j) {
                     // a pass of kMaxW terms builds the last from hi x hi alone
                     const double eps_j = nt_last == kMaxW && j == kMaxW - 1 ? 1e-2 : eps_lp;
