@@ -2865,7 +2865,7 @@ static int newton_mixed(klsp_ctx* ctx, const klsp_newton_params* np, int lp_prec
     //   |s - s_a| (4 trunc + max(eps_lp, 16 delta_b |T_2| / |T_1|) + higher) <= 5e-11,
     // half the 1e-10 the roots are held to; further out a new anchor is built
     // at the model's zero.  (Default grid: the BF16x3 zero 5.8e-7 off,
-    // estimate 3.7e-11, one FP64 build.)  Without finite partials the bound
+    // estimate about 4e-11, one FP64 build.)  Without finite partials the bound
     // falls back to nt r^(nt-1) + 1e-4 (BF16x3) or 1e-5 (FP32).
     const double eps_fallback = lp_prec == KLSP_PREC_BF16X3 ? 1e-4 : 1e-5;
     const double eps_lp = lp_prec == KLSP_PREC_BF16X3 ? 4e-5 : 2e-6;
